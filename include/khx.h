@@ -1,0 +1,131 @@
+/*
+ * khx -- C-ABI of the MI355X RCWA solve path (libkhx.so, gfx950).
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer.  Complex arrays are
+ *     interleaved (re, im) double pairs: numpy/torch complex128 layout.
+ *   - Matrices are row-major and batch-major: item b of a batch of n x n
+ *     matrices starts at b*n*n complex elements.  S-matrices are stored
+ *     [batch][2][2][n][n] (blocks S11, S12, S21, S22).
+ *   - n = 2*N where N = P*Q is the number of harmonics; harmonic h = q*P + p
+ *     with p (the x index) fastest; a field vector is [x part; y part].
+ *   - All calls are asynchronous on `stream` (a hipStream_t; NULL = default
+ *     stream), allocate nothing and never synchronise.  Scratch memory is
+ *     passed in `ws` with the size given by the matching *_ws_bytes query.
+ *   - Return value: 0 on success, -1 invalid argument, -3 HIP error; the
+ *     message is in khx_last_error() (thread-local).  Numerical status per
+ *     batch item is written to `info` (device int array) where noted.
+ *
+ * The reference (Kaeryv/Khepri) has no FFI: its boundary is the Python
+ * Crystal/Layer API.  Each entry point below names the reference function it
+ * replaces; the Python package khepri_amd binds them with ctypes.
+ */
+#ifndef KHX_H
+#define KHX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Last error message of the calling thread ("" if none). */
+const char* khx_last_error(void);
+/* ABI version of the library (incremented on signature changes). */
+int khx_abi_version(void);
+
+/* k vectors of the expansion (khepri/expansion.py:43-50 _compute_k_vector,
+ * x/y rows): kx[b][h] = (kp[b][0] + g[0][h]) / k0_b, ky likewise, with
+ * k0_b = 2 pi / wl[b].
+ *   g   : real (2, N)            kp : complex (batch, 2)
+ *   wl  : real (batch)           kx, ky : complex (batch, N)               */
+int khx_kvectors(const double* g, int N, const double* kp, const double* wl, int batch, double* kx,
+                 double* ky, void* stream);
+
+/* Convolution (Toeplitz) matrix of `count` pixmaps
+ * (khepri/tools.py:33-56 convolution_matrix):
+ *   C[q*P+p][q'*P+p'] = fftshift(fft2(eps))[nx/2 + p-p'][ny/2 + q-q'] / (nx*ny)
+ * eps: (count, nx, ny) real (is_complex = 0) or complex (is_complex = 1),
+ * indexed eps[x][y].  C: complex (count, N, N).  Fails (-1) when the pixmap
+ * is too small for the harmonics, where the reference raises IndexError.   */
+size_t khx_convmat_ws_bytes(int P, int Q, int count);
+int khx_convmat(const double* eps, int is_complex, int nx, int ny, int P, int Q, int count, double* C,
+                void* ws, void* stream);
+
+/* Batched complex LU solve A X = B (numpy.linalg.solve / inv as used in
+ * khepri/layer.py:158 IC = inv(C) and throughout khepri/alternative.py).
+ * A (batch, n, n) is overwritten by its factors, B (batch, n, nrhs) by X.
+ * info[b] = k+1 when the k-th pivot is exactly zero (numpy: LinAlgError). */
+int khx_zgesv(int n, int nrhs, double* A, double* B, int batch, int* info, void* stream);
+
+/* Batched complex GEMM C = alpha A B + beta C, A (batch,M,K), B (batch,K,N),
+ * C (batch,M,N); a_bcast / b_bcast != 0 shares one A / B across the batch.
+ * (numpy matmul as used throughout khepri/alternative.py)                  */
+int khx_zgemm(int M, int N, int K, const double* alpha, const double* A, int a_bcast, const double* B,
+              int b_bcast, const double* beta, double* C, int batch, void* stream);
+
+/* Batched eigen-decomposition of general complex matrices
+ * (numpy.linalg.eig / LAPACK zgeev as called at khepri/alternative.py:172).
+ * A (batch, n, n) is destroyed; lam (batch, n) eigenvalues; W (batch, n, n)
+ * eigenvectors in columns, unit 2-norm, largest component real.
+ * info[b] > 0: QR iteration did not converge.                             */
+size_t khx_eig_ws_bytes(int n, int batch);
+int khx_eig(int n, double* A, int batch, double* lam, double* W, int* info, void* ws, void* stream);
+
+/* Patterned layer: modes + S-matrix for a batch of (kx, ky, k0*depth) that
+ * share one convolution matrix C and IC = C^-1.
+ * Replaces khepri/layer.py:157-160 (Layer.solve, FFT formulation):
+ *   khepri/alternative.py:158-178 solve_structured_layer -> (W, V, lam)
+ *   khepri/alternative.py:181-195 build_scatmat          -> S
+ * k0d[b] = k0_b * depth.  S: (batch, 2, 2, n, n).  W, V (batch, n, n) and
+ * lam (batch, n) are written when non-NULL (needed for fields).
+ * info[b] > 0: eigen-solver did not converge.                              */
+size_t khx_patterned_layer_ws_bytes(int n, int batch);
+int khx_patterned_layer(const double* C, const double* IC, const double* kx, const double* ky, int N,
+                        int batch, const double* k0d, double* S, double* W, double* V, double* lam,
+                        int* info, void* ws, void* stream);
+
+/* Uniform layer of permittivity eps (khepri/layer.py:166-168:
+ * khepri/alternative.py:130-156 solve_uniform_layer + build_scatmat) and the
+ * half spaces (khepri/layer.py:169-178: khepri/alternative.py:32-56
+ * scattering_reflection, side = 1; :59-82 scattering_transmission, side = 2).
+ * Every harmonic decouples into a 2x2 system; S, W, V are written dense.
+ * eps: one complex scalar (2 doubles, host memory).  k0d ignored for half
+ * spaces (may be NULL).                                                     */
+int khx_uniform_layer(const double* kx, const double* ky, int N, int batch, const double* eps_host,
+                      const double* k0d, double* S, double* W, double* V, double* lam, void* stream);
+int khx_halfspace(int side, const double* kx, const double* ky, int N, int batch, const double* eps_host,
+                  double* S, double* W, double* V, double* lam, void* stream);
+
+/* Redheffer star product S = SA * SB (khepri/alternative.py:19-30
+ * redheffer_product), batched; a_bcast / b_bcast share one operand.  S must
+ * not alias SA or SB.  info[b] = first zero pivot of D or F (0 = none).    */
+size_t khx_redheffer_ws_bytes(int n, int batch);
+int khx_redheffer(const double* SA, int a_bcast, const double* SB, int b_bcast, int n, int batch, double* S,
+                  int* info, void* ws, void* stream);
+
+/* Reflected and transmitted power (khepri/crystal.py:363-396
+ * poynting_flux_end, with khepri/alternative.py:101-128 incident and
+ * :235-245 poynting_fluxes).  Stot (batch, 2, 2, n, n); g real (2, N);
+ * kp complex (batch, 2); wl (batch); te, tm complex (batch);
+ * epsi, epse: complex scalars in host memory.  R, T (batch); Rord, Tord
+ * (batch, N) per diffraction order when non-NULL.                          */
+int khx_poynting(const double* Stot, int N, const double* g, const double* kp, const double* wl,
+                 const double* te, const double* tm, const double* epsi_host, const double* epse_host, int batch,
+                 double* R, double* T, double* Rord, double* Tord, void* stream);
+
+/* Extended (twisted-bilayer) basis: scatter the base-layer matrices solved at
+ * kp + g_m (m = 0..Nb-1, one per harmonic of the other lattice) into the
+ * 2Nb^2 x 2Nb^2 joint space (khepri/extension.py:10-63 _joint_subspace /
+ * joint_subspace).  in: (items, Nb, blocks, 2Nb, 2Nb) -- blocks = 4 for
+ * S-matrices, 1 for W / V; out: (items, blocks, 2Nb^2, 2Nb^2), fully written.
+ * kind 0: base is the right-hand expansion, 1: the left-hand one.
+ * The _diag variant scatters eigenvalue vectors (items, Nb, 2Nb) ->
+ * (items, 2Nb^2) (khepri/extension.py:110).                                */
+int khx_joint_subspace(const double* in, int Nb, int blocks, int kind, int items, double* out, void* stream);
+int khx_joint_subspace_diag(const double* in, int Nb, int kind, int items, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KHX_H */

@@ -1,0 +1,68 @@
+"""ctypes binding of libkhx.so (the C-ABI declared in include/khx.h).
+
+The library is required: importing an op when libkhx.so is missing raises
+immediately -- there is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("KHX_LIBRARY", os.path.join(_HERE, "libkhx.so"))
+
+_vp, _dp, _ip, _i, _sz = C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_size_t
+
+# name -> (restype, argtypes); every pointer is passed as c_void_p
+SIGNATURES = {
+    "khx_last_error": (C.c_char_p, []),
+    "khx_abi_version": (_i, []),
+    "khx_kvectors": (_i, [_dp, _i, _dp, _dp, _i, _dp, _dp, _vp]),
+    "khx_convmat_ws_bytes": (_sz, [_i, _i, _i]),
+    "khx_convmat": (_i, [_dp, _i, _i, _i, _i, _i, _i, _dp, _vp, _vp]),
+    "khx_zgesv": (_i, [_i, _i, _dp, _dp, _i, _ip, _vp]),
+    "khx_zgemm": (_i, [_i, _i, _i, _dp, _dp, _i, _dp, _i, _dp, _dp, _i, _vp]),
+    "khx_eig_ws_bytes": (_sz, [_i, _i]),
+    "khx_eig": (_i, [_i, _dp, _i, _dp, _dp, _ip, _vp, _vp]),
+    "khx_patterned_layer_ws_bytes": (_sz, [_i, _i]),
+    "khx_patterned_layer": (_i, [_dp, _dp, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _ip, _vp, _vp]),
+    "khx_uniform_layer": (_i, [_dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "khx_halfspace": (_i, [_i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "khx_redheffer_ws_bytes": (_sz, [_i, _i]),
+    "khx_redheffer": (_i, [_dp, _i, _dp, _i, _i, _i, _dp, _ip, _vp, _vp]),
+    "khx_poynting": (_i, [_dp, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _dp, _dp, _dp, _dp, _vp]),
+    "khx_joint_subspace": (_i, [_dp, _i, _i, _i, _i, _dp, _vp]),
+    "khx_joint_subspace_diag": (_i, [_dp, _i, _i, _i, _dp, _vp]),
+}
+
+_lib = None
+
+
+class KhxError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libkhx.so once; raise if it is absent (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m khepri_amd.build` "
+            "(the RCWA path has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().khx_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(f"{what}: {msg}")
+        raise KhxError(f"{what} failed ({rc}): {msg}")

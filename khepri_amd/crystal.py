@@ -1,0 +1,260 @@
+"""User-facing device object (mirrors khepri/crystal.py ``Crystal``).
+
+Same methods, arguments and return conventions as the reference, plus
+batching: ``set_source`` accepts arrays of wavelengths / angles / in-plane
+wave vectors (one solve per entry) and ``poynting_flux_end`` then returns
+arrays.  All numerics run on the HIP device through libkhx; the Python code
+only orchestrates.
+"""
+from __future__ import annotations
+
+import cmath
+import os
+from copy import copy
+
+import numpy as np
+import torch
+
+from . import ops
+from .device import default_device
+from .expansion import Expansion, compute_kplanar
+from .extension import ExtendedLayer
+from .layer import Layer, stack_layers
+
+
+def _bytes_per_item(n, unique, length):
+    # layer S-matrices, patterned-layer workspace, star workspace, chain
+    return 16 * n * n * (4 * unique + 22 + 8 + 4 * (length + 2) + 4)
+
+
+class Crystal:
+    def __init__(self, pw, lattice="square", lattice_pitch=1, void=False, epsi=1, epse=1, device=None) -> None:
+        self.pw = pw
+        self.a = lattice_pitch
+        self.void = void
+        self.epsi = epsi
+        self.epse = epse
+        self.source = None
+        self.device = torch.device(device) if device is not None else None
+        if isinstance(lattice, str):
+            if lattice == "square":
+                self.lattice = self.a * np.asarray([[1, 0], [0, 1]])
+            elif lattice == "hexagonal":
+                self.lattice = self.a * np.asarray([[np.sqrt(3) / 2, 0.5], [np.sqrt(3) / 2, -0.5]])
+            else:
+                raise NotImplementedError(f"This {lattice} magic-string is not emplemented.")
+        else:
+            self.lattice = lattice
+        self.expansion = Expansion(pw, self.lattice)
+        self.layers = dict()
+        self.stacking_matrices = list()
+        self.stacking_reverse_matrices = None
+        self.stack_positions = []
+        self.S = None
+        self._RT = None
+
+    @classmethod
+    def from_expansion(cls, expansion, **kwargs):
+        obj = cls(expansion.pw, **kwargs)
+        obj.expansion = expansion
+        return obj
+
+    # ------------------------------------------------------------------ layers
+    def add_layer_uniform(self, name, epsilon, depth):
+        self.layers[name] = Layer.uniform(self.expansion, epsilon, depth)
+
+    def add_pixmap_or_uniform(self, name, epsilon, depth):
+        sample = epsilon.flatten()[0]
+        if np.all(epsilon == sample):
+            self.add_layer_uniform(name, sample, depth)
+        else:
+            self.add_layer_pixmap(name, epsilon, depth)
+
+    def add_layer_pixmap(self, name, epsilon, depth):
+        self.layers[name] = Layer.pixmap(self.expansion, epsilon, depth)
+
+    def add_layer_analytical(self, name, epsilon, epsilon_host, depth):
+        self.layers[name] = Layer.analytical(self.expansion, epsilon, epsilon_host, depth)
+
+    def add_layer(self, name, layer, extended=False):
+        self.layers[name] = ExtendedLayer(self.expansion, layer) if extended else layer
+
+    def set_device(self, layers_stack, fields_mask=False):
+        """Prepends / appends the half spaces (khepri/crystal.py:131-164)."""
+        self.device_stack = copy(layers_stack)
+        self.global_stacking = []
+        if not self.void:
+            self.global_stacking.append("Sref")
+        self.global_stacking.extend(layers_stack)
+        if not self.void:
+            self.global_stacking.append("Strans")
+        required = set(self.global_stacking)
+        if "Sref" in required and "Sref" not in self.layers:
+            self.layers["Sref"] = Layer.half_infinite(self.expansion, "reflexion", self.epsi)
+            self.layers["Sref"].fields = True
+        if "Strans" in required and "Strans" not in self.layers:
+            self.layers["Strans"] = Layer.half_infinite(self.expansion, "transmission", self.epse)
+            self.layers["Strans"].fields = True
+        fields_mask = [fields_mask] * len(layers_stack) if isinstance(fields_mask, bool) else fields_mask
+        self.stack_retain_mask = [True] + list(fields_mask) + [True]
+        self._fields_requested = any(fields_mask)
+        for name, enabled in zip(self.global_stacking, self.stack_retain_mask):
+            self.layers[name].fields |= enabled
+            if hasattr(self.layers[name], "base"):
+                self.layers[name].base.fields |= enabled
+
+    @property
+    def depth(self):
+        return sum(self.layers[name].depth for name in self.device_stack)
+
+    @property
+    def source_defined(self):
+        return self.source is not None
+
+    @property
+    def solved(self):
+        return self.S is not None or self._RT is not None
+
+    @property
+    def zmax(self):
+        return self.stack_positions[-2]
+
+    # ------------------------------------------------------------------ source
+    def set_source(self, wavelength, te=1.0, tm=1.0, theta=0.0, phi=0.0, kp=None):
+        """khepri/crystal.py:345-360; every argument may be an array (batch)."""
+        wl = np.atleast_1d(np.asarray(wavelength, dtype=np.float64))
+        self._scalar = np.ndim(wavelength) == 0 and (kp is None or np.ndim(kp) == 1) \
+            and np.ndim(theta) == 0 and np.ndim(phi) == 0
+        if kp is not None:
+            kpa = np.asarray(kp, dtype=np.complex128).reshape(-1, 2)
+        else:
+            th = np.atleast_1d(np.asarray(theta, dtype=np.float64))
+            ph = np.atleast_1d(np.asarray(phi, dtype=np.float64))
+            B0 = max(len(wl), len(th), len(ph))
+            wlb, thb, phb = (np.broadcast_to(v, (B0,)) for v in (wl, th, ph))
+            kpa = np.stack([compute_kplanar(self.epsi, w, t, p) for w, t, p in zip(wlb, thb, phb)])
+        B = max(len(wl), kpa.shape[0])
+        wl = np.broadcast_to(wl, (B,)).copy()
+        kpa = np.broadcast_to(kpa, (B, 2)).copy()
+        te_a = np.broadcast_to(np.asarray(te, dtype=np.complex128), (B,)).copy()
+        tm_a = np.broadcast_to(np.asarray(tm, dtype=np.complex128), (B,)).copy()
+        self.source = type("Source", (), {})()
+        self.source.te, self.source.tm, self.source.theta, self.source.phi = te, tm, theta, phi
+        self.source.wavelength = wavelength
+        self._wl, self._kp, self._te, self._tm = wl, kpa, te_a, tm_a
+        self.batch = B
+        if self._scalar:
+            self.kp = (kpa[0, 0], kpa[0, 1]) if kp is None else tuple(kp)
+            self.k0 = 2 * np.pi / wavelength
+            kxi, kyi = self.kp
+            self.kzi = np.conj(cmath.sqrt(self.k0 ** 2 * self.epsi - kxi ** 2 - kyi ** 2))
+        else:
+            self.kp = kpa
+            self.k0 = 2 * np.pi / wl
+            self.kzi = np.conj(np.sqrt((self.k0 ** 2 * self.epsi - kpa[:, 0] ** 2 - kpa[:, 1] ** 2).astype(complex)))
+        self.S = None
+        self._RT = None
+
+    # ------------------------------------------------------------------ solve
+    def _dev(self):
+        return self.device or default_device()
+
+    def _chunk(self, B):
+        env = os.environ.get("KHX_CHUNK")
+        if env:
+            return max(1, int(env))
+        budget = float(os.environ.get("KHX_MEM_BUDGET_GB", "48")) * 2 ** 30
+        n = 2 * self.expansion.N
+        per = _bytes_per_item(n, len(set(self.global_stacking)), len(self.global_stacking))
+        return int(max(1, min(B, budget // per)))
+
+    def solve(self):
+        """khepri/crystal.py:180-206.  Batches larger than one memory chunk
+        are streamed: each chunk's R/T is computed and its S-matrices freed."""
+        assert self.source_defined, "Call set_source before solving."
+        dev = self._dev()
+        names = list(dict.fromkeys(self.global_stacking))   # unique, in stack order
+        stacked = [self.layers[name] for name in self.global_stacking]
+        self.stack_positions = list(np.cumsum([layer.depth for layer in stacked]))
+        self.stack_positions[-1] = np.inf
+        if not self.void:
+            self.stack_positions.insert(0, -np.inf)
+
+        B = self.batch
+        chunk = self._chunk(B)
+        kp_all = torch.as_tensor(self._kp, device=dev)
+        wl_all = torch.as_tensor(self._wl, device=dev)
+        te_all = torch.as_tensor(self._te, device=dev)
+        tm_all = torch.as_tensor(self._tm, device=dev)
+        self._sources_dev = (kp_all, wl_all, te_all, tm_all)
+        want_reverse = getattr(self, "_fields_requested", False)
+        if chunk >= B:
+            for name in names:
+                self.layers[name].solve_batch(kp_all, wl_all)
+            self._check_layers(names)
+            self.stacking_matrices, self.stacking_reverse_matrices, self.Stot = stack_layers(
+                self.expansion.pw, stacked, self.stack_retain_mask, reverse=want_reverse)
+            self.S = self.Stot
+            return
+        if want_reverse:
+            raise ValueError("fields need the whole batch resident; lower the batch size (KHX_CHUNK)")
+        R = torch.empty(B, dtype=torch.float64, device=dev)
+        T = torch.empty_like(R)
+        for c0 in range(0, B, chunk):
+            c1 = min(B, c0 + chunk)
+            for name in names:
+                self.layers[name].solve_batch(kp_all[c0:c1], wl_all[c0:c1])
+            self._check_layers(names)
+            _, _, Stot = stack_layers(self.expansion.pw, stacked, self.stack_retain_mask, reverse=False)
+            r, t, _, _ = ops.poynting(Stot, self.expansion.g_device(dev), kp_all[c0:c1], wl_all[c0:c1],
+                                      te_all[c0:c1], tm_all[c0:c1], self.epsi, self.epse)
+            R[c0:c1] = r
+            T[c0:c1] = t
+            for name in names:
+                self.layers[name].S = None
+        self._RT = (R, T)
+        self.S = None
+
+    def _check_layers(self, names):
+        for name in names:
+            info = getattr(self.layers[name], "info", None)
+            if info is not None and os.environ.get("KHX_CHECK_INFO", "1") == "1":
+                ops.raise_on_info(info, f"layer {name}")
+
+    # ------------------------------------------------------------------ outputs
+    def poynting_flux_end(self, only_total=True):
+        """(R, T) -- khepri/crystal.py:363-396."""
+        assert self.solved, "Call solve first"
+        if self._RT is not None:
+            if not only_total:
+                raise ValueError("per-order fluxes need the whole batch resident (lower KHX_CHUNK)")
+            R, T = (x.cpu().numpy() for x in self._RT)
+        else:
+            kp, wl, te, tm = self._sources_dev
+            R, T, Ro, To = ops.poynting(self.Stot, self.expansion.g_device(self._dev()), kp, wl, te, tm,
+                                        self.epsi, self.epse, orders=not only_total)
+            R, T = R.cpu().numpy(), T.cpu().numpy()
+            if not only_total:
+                Ro, To = Ro.cpu().numpy(), To.cpu().numpy()
+                if self._scalar:
+                    return (R[0], Ro[0]), (T[0], To[0])
+                return (R, Ro), (T, To)
+        if self._scalar:
+            return R[0], T[0]
+        return R, T
+
+    # fields are provided by khepri_amd.fields (bound below)
+
+
+from . import fields as _fields  # noqa: E402
+
+Crystal.locate_layer = _fields.locate_layer
+Crystal.get_source_as_field_vectors = _fields.get_source_as_field_vectors
+Crystal.fields_coords_xy = _fields.fields_coords_xy
+Crystal.fields_volume = _fields.fields_volume
+Crystal._fourier_fields = _fields.fourier_fields
+
+
+class Multilayer(Crystal):
+    def __init__(self, epsi=1, epse=1):
+        super().__init__((1, 1), lattice="square", lattice_pitch=1, void=False, epsi=epsi, epse=epse)

@@ -1,0 +1,336 @@
+// Batched complex FP64 building blocks: GEMM, LU solve, elementwise helpers.
+//
+// All matrices are row-major complex128 (numpy layout), batch-major; a batch
+// stride of 0 broadcasts one operand to every item (e.g. the convolution
+// matrix of a layer, shared by all frequencies and Bloch vectors).
+#include "khx_internal.h"
+
+namespace khx {
+
+// ======================================================================== GEMM
+// One workgroup = one BM x BN tile of one batch item; 256 threads arranged
+// (BM/TM) x (BN/TN), each owning a TM x TN register tile.  Operand tiles are
+// staged in LDS, A transposed so a wave reads a few broadcast rows of A and
+// consecutive columns of B.
+template <int BM, int BN, int BK, int TM, int TN>
+__global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, cd alpha,
+                                                    const cd* __restrict__ A, int lda, long long sA,
+                                                    const cd* __restrict__ B, int ldb, long long sB,
+                                                    cd beta, cd* __restrict__ C, int ldc, long long sC,
+                                                    int tilesN, int tilesPerItem) {
+  constexpr int TX = BN / TN;  // threads along N
+  constexpr int TY = BM / TM;  // threads along M
+  static_assert(TX * TY == 256, "256 threads");
+  __shared__ cd As[BK][BM];
+  __shared__ cd Bs[BK][BN];
+
+  const long long item = blockIdx.x / tilesPerItem;
+  const int tile = blockIdx.x % tilesPerItem;
+  const int row0 = (tile / tilesN) * BM;
+  const int col0 = (tile % tilesN) * BN;
+  const cd* a = A + item * sA;
+  const cd* b = B + item * sB;
+  cd* c = C + item * sC;
+
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+
+  cd acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = mkc(0.0, 0.0);
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    // stage A tile (BM x BK) -> As[k][m]
+#pragma unroll
+    for (int e = tid; e < BM * BK; e += 256) {
+      const int r = e / BK, kk = e % BK;
+      const int gr = row0 + r, gk = k0 + kk;
+      As[kk][r] = (gr < M && gk < K) ? a[(long long)gr * lda + gk] : mkc(0.0, 0.0);
+    }
+    // stage B tile (BK x BN)
+#pragma unroll
+    for (int e = tid; e < BK * BN; e += 256) {
+      const int kk = e / BN, cc = e % BN;
+      const int gk = k0 + kk, gc = col0 + cc;
+      Bs[kk][cc] = (gk < K && gc < N) ? b[(long long)gk * ldb + gc] : mkc(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < BK; ++kk) {
+      cd av[TM], bv[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) av[i] = As[kk][ty + TY * i];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bv[j] = Bs[kk][tx + TX * j];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = fmac(acc[i][j], av[i], bv[j]);
+    }
+    __syncthreads();
+  }
+
+  const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int gr = row0 + ty + TY * i;
+    if (gr >= M) continue;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int gc = col0 + tx + TX * j;
+      if (gc >= N) continue;
+      cd v = alpha * acc[i][j];
+      cd* dst = c + (long long)gr * ldc + gc;
+      if (use_beta) v = fmac(v, beta, *dst);
+      *dst = v;
+    }
+  }
+}
+
+template <int BM, int BN, int BK, int TM, int TN>
+static int launch_zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C,
+                        int batch) {
+  const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
+  const long long blocks = (long long)tilesM * tilesN * batch;
+  KHX_REQUIRE(blocks < (1ll << 31), "zgemm: grid too large");
+  zgemm_kernel<BM, BN, BK, TM, TN><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+      M, N, K, alpha, A.p, A.ld, A.stride, B.p, B.ld, B.stride, beta, C.p, C.ld, C.stride, tilesN,
+      tilesM * tilesN);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+int zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
+  if (batch <= 0 || M <= 0 || N <= 0) return 0;
+  if (M <= 40 || N <= 40) return launch_zgemm<32, 32, 16, 2, 2>(s, M, N, K, alpha, A, B, beta, C, batch);
+  return launch_zgemm<64, 64, 16, 4, 4>(s, M, N, K, alpha, A, B, beta, C, batch);
+}
+
+// ======================================================================== LU solve
+// One workgroup per system.  Gaussian elimination with partial pivoting
+// (largest |re|+|im| in the column, first index on ties) applied to A and the
+// right-hand sides together, then back substitution on the right-hand sides.
+// The pivot row and the column multipliers of each step are staged in LDS.
+constexpr int GESV_THREADS = 256;
+
+__global__ __launch_bounds__(GESV_THREADS) void zgesv_kernel(int n, int m, cd* A, int lda, long long sA,
+                                                             cd* B, int ldb, long long sB, int* info) {
+  extern __shared__ cd smem[];
+  cd* prowA = smem;        // n
+  cd* prowB = smem + n;    // m
+  cd* lcol = prowB + m;    // n
+  __shared__ double red_v[GESV_THREADS / 64];
+  __shared__ int red_i[GESV_THREADS / 64];
+  __shared__ cd red_c[GESV_THREADS / 64];
+  __shared__ cd s_akk;
+
+  const long long item = blockIdx.x;
+  cd* a = A + item * sA;
+  cd* bm = B + item * sB;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  constexpr int NW = GESV_THREADS / 64;
+  int first_zero = 0;
+
+  for (int k = 0; k < n; ++k) {
+    // ---- pivot search in column k
+    double best = -1.0;
+    int bi = n;
+    cd bv = mkc(0.0, 0.0);
+    for (int i = k + tid; i < n; i += GESV_THREADS) {
+      const cd v = a[(long long)i * lda + k];
+      const double av = cabs1(v);
+      if (av > best) {
+        best = av;
+        bi = i;
+        bv = v;
+      }
+    }
+    if (tid == 0) s_akk = a[(long long)k * lda + k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      const double ox = __shfl_xor(bv.x, o, 64), oy = __shfl_xor(bv.y, o, 64);
+      if (ov > best || (ov == best && oi < bi)) {
+        best = ov;
+        bi = oi;
+        bv = mkc(ox, oy);
+      }
+    }
+    if (lane == 0) {
+      red_v[wid] = best;
+      red_i[wid] = bi;
+      red_c[wid] = bv;
+    }
+    __syncthreads();
+    best = red_v[0];
+    bi = red_i[0];
+    bv = red_c[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      if (red_v[w] > best || (red_v[w] == best && red_i[w] < bi)) {
+        best = red_v[w];
+        bi = red_i[w];
+        bv = red_c[w];
+      }
+    }
+    const cd akk = s_akk;
+    int p = bi;
+    cd piv = bv;
+    if (p >= n) {  // column of NaNs: keep the diagonal
+      p = k;
+      piv = akk;
+    }
+    const bool zero = (piv.x == 0.0 && piv.y == 0.0);
+    if (zero && first_zero == 0) first_zero = k + 1;
+    const cd rp = zero ? mkc(0.0, 0.0) : recipc(piv);
+
+    // ---- swap rows k <-> p; new row k goes to LDS; multipliers for rows > k
+    for (int j = k + tid; j < n; j += GESV_THREADS) {
+      const cd vk = a[(long long)k * lda + j], vp = a[(long long)p * lda + j];
+      if (p != k) {
+        a[(long long)k * lda + j] = vp;
+        a[(long long)p * lda + j] = vk;
+      }
+      prowA[j] = vp;
+    }
+    for (int j = tid; j < m; j += GESV_THREADS) {
+      const cd vk = bm[(long long)k * ldb + j], vp = bm[(long long)p * ldb + j];
+      if (p != k) {
+        bm[(long long)k * ldb + j] = vp;
+        bm[(long long)p * ldb + j] = vk;
+      }
+      prowB[j] = vp;
+    }
+    for (int i = k + 1 + tid; i < n; i += GESV_THREADS) {
+      const cd v = (i == p) ? akk : a[(long long)i * lda + k];
+      lcol[i] = v * rp;
+    }
+    __syncthreads();
+
+    // ---- rank-1 update of the trailing rows (A columns > k and all of B)
+    for (int i = k + 1 + wid; i < n; i += NW) {
+      const cd l = lcol[i];
+      cd* ar = a + (long long)i * lda;
+      for (int j = k + 1 + lane; j < n; j += 64) ar[j] = fmsc(ar[j], l, prowA[j]);
+      cd* br = bm + (long long)i * ldb;
+      for (int j = lane; j < m; j += 64) br[j] = fmsc(br[j], l, prowB[j]);
+    }
+    __syncthreads();
+  }
+
+  // ---- back substitution U X = B'
+  for (int k = n - 1; k >= 0; --k) {
+    const cd ukk = a[(long long)k * lda + k];
+    for (int j = tid; j < m; j += GESV_THREADS) {
+      const cd x = divc(bm[(long long)k * ldb + j], ukk);
+      bm[(long long)k * ldb + j] = x;
+      prowB[j] = x;
+    }
+    __syncthreads();
+    for (int i = wid; i < k; i += NW) {
+      const cd u = a[(long long)i * lda + k];
+      cd* br = bm + (long long)i * ldb;
+      for (int j = lane; j < m; j += 64) br[j] = fmsc(br[j], u, prowB[j]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && info) info[item] = first_zero;
+}
+
+int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info) {
+  if (batch <= 0 || n <= 0) return 0;
+  KHX_REQUIRE(A.stride != 0 && B.stride != 0 || batch == 1, "zgesv: operands cannot be broadcast");
+  const size_t lds = (size_t)(2 * n + m) * sizeof(cd);
+  KHX_REQUIRE(lds <= 160 * 1024, "zgesv: system too large for the LDS staging buffers");
+  static bool attr_set = false;
+  if (!attr_set && lds > 64 * 1024) {
+    KHX_CHECK_HIP(hipFuncSetAttribute((const void*)zgesv_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  zgesv_kernel<<<dim3(batch), dim3(GESV_THREADS), lds, s>>>(n, m, A.p, A.ld, A.stride, B.p, B.ld,
+                                                           B.stride, info);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+// ======================================================================== elementwise
+// Grid: x over (rows*cols) chunks, y over batch chunks.
+__global__ void zset_identity_kernel(cd* A, int ld, long long sA, int rows, int cols, cd diag,
+                                     int batch) {
+  const int total = rows * cols;
+  for (long long b = blockIdx.y; b < batch; b += gridDim.y) {
+    cd* a = A + b * sA;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+      const int i = e / cols, j = e % cols;
+      a[(long long)i * ld + j] = (i == j) ? diag : mkc(0.0, 0.0);
+    }
+  }
+}
+
+__global__ void zaxpby_kernel(cd* D, int ldd, long long sD, cd al, const cd* A, int lda, long long sA,
+                              cd be, const cd* B, int ldb, long long sB, int rows, int cols,
+                              int batch) {
+  const int total = rows * cols;
+  for (long long b = blockIdx.y; b < batch; b += gridDim.y) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+      const int i = e / cols, j = e % cols;
+      cd v = al * A[b * sA + (long long)i * lda + j];
+      if (B) v = fmac(v, be, B[b * sB + (long long)i * ldb + j]);
+      D[b * sD + (long long)i * ldd + j] = v;
+    }
+  }
+}
+
+__global__ void zrow_scale_kernel(cd* D, int ldd, long long sD, const cd* d, long long sd,
+                                  const cd* A, int lda, long long sA, int rows, int cols, int batch) {
+  const int total = rows * cols;
+  for (long long b = blockIdx.y; b < batch; b += gridDim.y) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+      const int i = e / cols, j = e % cols;
+      D[b * sD + (long long)i * ldd + j] = d[b * sd + i] * A[b * sA + (long long)i * lda + j];
+    }
+  }
+}
+
+static dim3 ew_grid(long long total, int batch) {
+  long long gx = (total + 255) / 256;
+  if (gx > 64) gx = 64;
+  if (gx < 1) gx = 1;
+  int gy = batch < 65535 ? batch : 65535;
+  return dim3((unsigned)gx, (unsigned)gy);
+}
+
+int zset_identity(hipStream_t s, Mat A, int rows, int cols, cd diag, int batch) {
+  if (batch <= 0) return 0;
+  zset_identity_kernel<<<ew_grid((long long)rows * cols, batch), 256, 0, s>>>(A.p, A.ld, A.stride, rows,
+                                                                            cols, diag, batch);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+int zcopy(hipStream_t s, Mat D, Mat A, int rows, int cols, int batch) {
+  return zaxpby(s, D, mkc(1.0, 0.0), A, mkc(0.0, 0.0), Mat{nullptr, 0, 0}, rows, cols, batch);
+}
+
+int zaxpby(hipStream_t s, Mat D, cd a, Mat A, cd b, Mat B, int rows, int cols, int batch) {
+  if (batch <= 0) return 0;
+  zaxpby_kernel<<<ew_grid((long long)rows * cols, batch), 256, 0, s>>>(
+      D.p, D.ld, D.stride, a, A.p, A.ld, A.stride, b, B.p, B.ld, B.stride, rows, cols, batch);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+int zrow_scale(hipStream_t s, Mat D, const cd* d, long long dstride, Mat A, int rows, int cols,
+               int batch) {
+  if (batch <= 0) return 0;
+  zrow_scale_kernel<<<ew_grid((long long)rows * cols, batch), 256, 0, s>>>(
+      D.p, D.ld, D.stride, d, dstride, A.p, A.ld, A.stride, rows, cols, batch);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace khx

@@ -1,0 +1,155 @@
+// extern "C" entry points of libkhx.so (declared in include/khx.h).
+#include "../../include/khx.h"
+
+#include <string>
+
+#include "khx_internal.h"
+
+namespace khx {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+// defined in the other translation units
+int kvectors(hipStream_t s, const double* g, int N, const cd* kp, const double* wl, int batch, cd* kx, cd* ky);
+int convmat(hipStream_t s, const double* eps, int cplx, int nx, int ny, int P, int Q, int count, cd* C, cd* Fws);
+size_t patterned_layer_ws_bytes(int n, int batch);
+int patterned_layer(hipStream_t s, const cd* C, const cd* IC, const cd* kx, const cd* ky, int N, int batch,
+                    const double* k0d, cd* S, cd* Wout, cd* Vout, cd* lamout, int* info, void* ws);
+int analytic_layer(hipStream_t s, int kind, const cd* kx, const cd* ky, int N, int batch, cd eps,
+                   const double* k0d, cd* S, cd* W, cd* V, cd* lam);
+size_t star_ws_bytes(int n, int batch);
+int star(hipStream_t s, const cd* SA, long long sA, const cd* SB, long long sB, int n, int batch, cd* S,
+         int* info, void* ws);
+int poynting(hipStream_t s, const cd* S, long long sS, int N, const double* g, const cd* kp, const double* wl,
+             const cd* te, const cd* tm, cd epsi, cd epse, int batch, double* R, double* T, double* Rord,
+             double* Tord);
+
+int joint_subspace(hipStream_t s, const cd* in, int Nb, int M, int blocks, int kind, int items, cd* out);
+int joint_subspace_vec(hipStream_t s, const cd* in, int Nb, int M, int kind, int items, cd* out);
+
+}  // namespace khx
+
+
+static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+static inline cd* as_cd(double* p) { return reinterpret_cast<cd*>(p); }
+static inline const cd* as_cd(const double* p) { return reinterpret_cast<const cd*>(p); }
+
+#define KHX_ENTRY_BEGIN khx::set_error("");
+
+extern "C" {
+
+const char* khx_last_error(void) { return khx::last_error(); }
+int khx_abi_version(void) { return 1; }
+
+int khx_kvectors(const double* g, int N, const double* kp, const double* wl, int batch, double* kx, double* ky,
+                 void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0, "khx_kvectors: bad sizes");
+  return khx::kvectors(as_stream(stream), g, N, as_cd(kp), wl, batch, as_cd(kx), as_cd(ky));
+}
+
+size_t khx_convmat_ws_bytes(int P, int Q, int count) {
+  return (size_t)(2 * P - 1) * (2 * Q - 1) * sizeof(cd) * (size_t)(count > 0 ? count : 0) + 256;
+}
+
+int khx_convmat(const double* eps, int is_complex, int nx, int ny, int P, int Q, int count, double* C, void* ws,
+                void* stream) {
+  KHX_ENTRY_BEGIN
+  return khx::convmat(as_stream(stream), eps, is_complex, nx, ny, P, Q, count, as_cd(C), static_cast<cd*>(ws));
+}
+
+int khx_zgesv(int n, int nrhs, double* A, double* B, int batch, int* info, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && nrhs > 0 && batch >= 0, "khx_zgesv: bad sizes");
+  const long long nn = (long long)n * n;
+  return khx::zgesv(as_stream(stream), n, nrhs, khx::Mat{as_cd(A), n, nn}, khx::Mat{as_cd(B), nrhs, (long long)n * nrhs},
+                    batch, info);
+}
+
+int khx_zgemm(int M, int N, int K, const double* alpha, const double* A, int a_bcast, const double* B,
+              int b_bcast, const double* beta, double* C, int batch, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(M > 0 && N > 0 && K > 0 && batch >= 0, "khx_zgemm: bad sizes");
+  const cd al = mkc(alpha[0], alpha[1]), be = mkc(beta[0], beta[1]);
+  return khx::zgemm(as_stream(stream), M, N, K, al,
+                    khx::Mat{const_cast<cd*>(as_cd(A)), K, a_bcast ? 0 : (long long)M * K},
+                    khx::Mat{const_cast<cd*>(as_cd(B)), N, b_bcast ? 0 : (long long)K * N}, be,
+                    khx::Mat{as_cd(C), N, (long long)M * N}, batch);
+}
+
+size_t khx_eig_ws_bytes(int n, int batch) { return khx::zeig_workspace_bytes(n, batch) + 256; }
+
+int khx_eig(int n, double* A, int batch, double* lam, double* W, int* info, void* ws, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0, "khx_eig: bad sizes");
+  const long long nn = (long long)n * n;
+  return khx::zeig(as_stream(stream), n, khx::Mat{as_cd(A), n, nn}, as_cd(lam), n, khx::Mat{as_cd(W), n, nn},
+                   batch, info, ws);
+}
+
+size_t khx_patterned_layer_ws_bytes(int n, int batch) { return khx::patterned_layer_ws_bytes(n, batch); }
+
+int khx_patterned_layer(const double* C, const double* IC, const double* kx, const double* ky, int N, int batch,
+                        const double* k0d, double* S, double* W, double* V, double* lam, int* info, void* ws,
+                        void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0, "khx_patterned_layer: bad sizes");
+  return khx::patterned_layer(as_stream(stream), as_cd(C), as_cd(IC), as_cd(kx), as_cd(ky), N, batch, k0d, as_cd(S),
+                              as_cd(W), as_cd(V), as_cd(lam), info, ws);
+}
+
+int khx_uniform_layer(const double* kx, const double* ky, int N, int batch, const double* eps_host,
+                      const double* k0d, double* S, double* W, double* V, double* lam, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0 && eps_host && k0d, "khx_uniform_layer: bad arguments");
+  return khx::analytic_layer(as_stream(stream), 0, as_cd(kx), as_cd(ky), N, batch,
+                             mkc(eps_host[0], eps_host[1]), k0d, as_cd(S), as_cd(W), as_cd(V), as_cd(lam));
+}
+
+int khx_halfspace(int side, const double* kx, const double* ky, int N, int batch, const double* eps_host,
+                  double* S, double* W, double* V, double* lam, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(side == 1 || side == 2, "khx_halfspace: side must be 1 (reflection) or 2 (transmission)");
+  KHX_REQUIRE(N > 0 && batch >= 0 && eps_host, "khx_halfspace: bad arguments");
+  return khx::analytic_layer(as_stream(stream), side, as_cd(kx), as_cd(ky), N, batch,
+                             mkc(eps_host[0], eps_host[1]), nullptr, as_cd(S), as_cd(W), as_cd(V), as_cd(lam));
+}
+
+size_t khx_redheffer_ws_bytes(int n, int batch) { return khx::star_ws_bytes(n, batch); }
+
+int khx_redheffer(const double* SA, int a_bcast, const double* SB, int b_bcast, int n, int batch, double* S,
+                  int* info, void* ws, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0, "khx_redheffer: bad sizes");
+  KHX_REQUIRE(S != SA && S != SB, "khx_redheffer: S must not alias an operand");
+  const long long s4 = 4ll * n * n;
+  return khx::star(as_stream(stream), as_cd(SA), a_bcast ? 0 : s4, as_cd(SB), b_bcast ? 0 : s4, n, batch, as_cd(S),
+                   info, ws);
+}
+
+int khx_poynting(const double* Stot, int N, const double* g, const double* kp, const double* wl, const double* te,
+                 const double* tm, const double* epsi_host, const double* epse_host, int batch, double* R, double* T,
+                 double* Rord, double* Tord, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0 && epsi_host && epse_host, "khx_poynting: bad arguments");
+  const long long s4 = 4ll * (2 * N) * (2 * N);
+  return khx::poynting(as_stream(stream), as_cd(Stot), s4, N, g, as_cd(kp), wl, as_cd(te), as_cd(tm),
+                       mkc(epsi_host[0], epsi_host[1]), mkc(epse_host[0], epse_host[1]), batch, R, T, Rord,
+                       Tord);
+}
+
+int khx_joint_subspace(const double* in, int Nb, int blocks, int kind, int items, double* out, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(Nb > 0 && (blocks == 1 || blocks == 4) && items >= 0, "khx_joint_subspace: bad arguments");
+  return khx::joint_subspace(as_stream(stream), as_cd(in), Nb, Nb, blocks, kind, items, as_cd(out));
+}
+
+int khx_joint_subspace_diag(const double* in, int Nb, int kind, int items, double* out, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(Nb > 0 && items >= 0, "khx_joint_subspace_diag: bad arguments");
+  return khx::joint_subspace_vec(as_stream(stream), as_cd(in), Nb, Nb, kind, items, as_cd(out));
+}
+
+}  // extern "C"

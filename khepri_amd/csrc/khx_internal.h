@@ -1,0 +1,34 @@
+// Internal (non-exported) launchers shared by the khx translation units.
+#pragma once
+#include "khx_common.h"
+
+namespace khx {
+
+// Strided batch of row-major complex matrices: element (i, j) of item b is
+// p[b * stride + i * ld + j].  stride 0 broadcasts one matrix to the batch.
+struct Mat {
+  cd* p;
+  int ld;
+  long long stride;
+};
+inline Mat mat(cd* p, int ld, long long stride) { return Mat{p, ld, stride}; }
+inline Mat sub(Mat m, int r, int c) { return Mat{m.p + (long long)r * m.ld + c, m.ld, m.stride}; }
+
+// C = alpha * A (M x K) * B (K x N) + beta * C
+int zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch);
+// Solve A X = B (A n x n destroyed, B n x m overwritten); info[b] = first zero pivot (1-based) or 0
+int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info);
+
+// elementwise helpers (all batched, n_rows x n_cols views)
+int zset_identity(hipStream_t s, Mat A, int rows, int cols, cd diag, int batch);   // A = diag * I
+int zcopy(hipStream_t s, Mat dst, Mat src, int rows, int cols, int batch);
+// D = a*A + b*B
+int zaxpby(hipStream_t s, Mat D, cd a, Mat A, cd b, Mat B, int rows, int cols, int batch);
+// D[i][j] = d[i] * A[i][j]   (d: batch x rows vector with stride dstride)
+int zrow_scale(hipStream_t s, Mat D, const cd* d, long long dstride, Mat A, int rows, int cols, int batch);
+
+// eigen-decomposition: A destroyed; lam (n per item), W (n x n, unit columns)
+size_t zeig_workspace_bytes(int n, int batch);
+int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws);
+
+}  // namespace khx

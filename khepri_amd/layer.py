@@ -1,0 +1,201 @@
+"""Layers of a stack and the star-product chain (mirrors khepri/layer.py).
+
+A layer solves for a whole batch of (k_parallel, wavelength) pairs at once;
+its S-matrix is a device tensor of shape (B, 2, 2, n, n).  The reference's
+single-pair protocol ``solve(k_parallel, wavelength)`` is kept (B = 1).
+"""
+from __future__ import annotations
+
+from enum import IntEnum
+
+import numpy as np
+import torch
+
+from . import ops
+from .device import default_device
+
+
+class Formulation(IntEnum):
+    UNIFORM = 0
+    FFT = 1
+    ANALYTICAL = 2
+    HALF_SPACE_INC = 3
+    HALF_SPACE_TRN = 4
+
+
+class Field(IntEnum):
+    X = 0
+    Y = 1
+    Z = 2
+    NORM = 3
+    POYNTING = 4
+
+
+def as_source_batch(k_parallel, wavelength, device):
+    """(B, 2) complex kp and (B,) float wavelength tensors on `device`."""
+    if isinstance(k_parallel, torch.Tensor):
+        kp = k_parallel.to(device=device, dtype=torch.complex128)
+    else:
+        kp = torch.as_tensor(np.asarray(k_parallel, dtype=np.complex128), device=device)
+    if isinstance(wavelength, torch.Tensor):
+        wl = wavelength.to(device=device, dtype=torch.float64)
+    else:
+        wl = torch.as_tensor(np.asarray(wavelength, dtype=np.float64), device=device)
+    kp = kp.reshape(-1, 2)
+    wl = wl.reshape(-1)
+    B = max(kp.shape[0], wl.shape[0])
+    if kp.shape[0] == 1 and B > 1:
+        kp = kp.expand(B, 2)
+    if wl.shape[0] == 1 and B > 1:
+        wl = wl.expand(B)
+    if kp.shape[0] != B or wl.shape[0] != B:
+        raise ValueError("k_parallel and wavelength batch sizes differ")
+    return kp.contiguous(), wl.contiguous()
+
+
+class Layer:
+    """One layer of the stack (khepri/layer.py:63-193)."""
+
+    def __init__(self) -> None:
+        self.formulation = None
+        self.expansion = None
+        self.W = self.V = self.L = None
+        self.S = None
+        self.IC = None
+        self.fields = False
+        self.device = None
+        self._cm_key = None
+        self._C = self._IC = None
+        self.info = None
+
+    # ------------------------------------------------------------- builders
+    @classmethod
+    def pixmap_or_uniform(cls, expansion, pixmap, depth):
+        eps0 = pixmap.flatten()[0]
+        if np.all(pixmap == eps0):
+            return Layer.uniform(expansion, eps0, depth)
+        return Layer.pixmap(expansion, pixmap, depth)
+
+    @classmethod
+    def pixmap(cls, expansion, pixmap, depth):
+        layer = cls()
+        layer.expansion = expansion
+        layer.formulation = Formulation.FFT
+        layer.epsilon = pixmap
+        layer.depth = depth
+        return layer
+
+    @classmethod
+    def uniform(cls, expansion, epsilon, depth):
+        layer = cls()
+        layer.expansion = expansion
+        layer.formulation = Formulation.UNIFORM
+        layer.epsilon = epsilon
+        layer.depth = depth
+        return layer
+
+    @classmethod
+    def half_infinite(cls, expansion, type, epsilon):
+        layer = cls()
+        if type == "reflexion":
+            layer.formulation = Formulation.HALF_SPACE_INC
+        elif type == "transmission":
+            layer.formulation = Formulation.HALF_SPACE_TRN
+        else:
+            raise ValueError(f"half space type must be 'reflexion' or 'transmission', got {type!r}")
+        layer.expansion = expansion
+        layer.depth = 0
+        layer.epsilon = epsilon
+        return layer
+
+    @classmethod
+    def analytical(cls, expansion, islands_description, eps_host, depth):
+        raise NotImplementedError("analytical island layers are not part of the GPU path; "
+                                  "rasterise the islands into a pixmap (Layer.pixmap)")
+
+    # ------------------------------------------------------------- solve
+    def convolution_matrices(self, device):
+        """C and C^-1 of a pixmap layer, computed once per pixmap content."""
+        eps = np.asarray(self.epsilon)
+        key = (str(device), eps.shape, eps.dtype.str, hash(eps.tobytes()))
+        if self._cm_key != key:
+            t = torch.as_tensor(np.ascontiguousarray(eps), device=device)
+            if not t.is_complex():
+                t = t.to(torch.float64)
+            C = ops.convmat(t, self.expansion.pw)
+            IC, info = ops.inverse(C)
+            ops.raise_on_info(info, "inverse of the convolution matrix")
+            self._C, self._IC, self._cm_key = C[0], IC[0], key
+        return self._C, self._IC
+
+    def solve(self, k_parallel, wavelength):
+        """Reference protocol (khepri/layer.py:145-193); batched when the
+        arguments are arrays."""
+        dev = self.device or default_device()
+        kp, wl = as_source_batch(k_parallel, wavelength, dev)
+        self.solve_batch(kp, wl)
+
+    def solve_batch(self, kp, wl, ws=None):
+        dev = kp.device
+        self.device = dev
+        kx, ky = ops.kvectors(self.expansion.g_device(dev), kp, wl)
+        k0 = 2 * np.pi / wl
+        modes = bool(self.fields)
+        self.info = None
+        if self.formulation == Formulation.FFT:
+            C, IC = self.convolution_matrices(dev)
+            k0d = (k0 * self.depth).contiguous()
+            self.S, (W, V, L), self.info = ops.patterned_layer(C, IC, kx, ky, k0d, modes=modes, ws=ws)
+            self.C = C
+            self.IC = IC if modes else None
+        elif self.formulation == Formulation.UNIFORM:
+            k0d = (k0 * self.depth).contiguous()
+            self.S, (W, V, L) = ops.analytic_layer("uniform", kx, ky, self.epsilon, k0d, modes=modes)
+            self.IC = 1 / self.epsilon if modes else None
+        elif self.formulation in (Formulation.HALF_SPACE_INC, Formulation.HALF_SPACE_TRN):
+            side = "reflexion" if self.formulation == Formulation.HALF_SPACE_INC else "transmission"
+            self.S, (W, V, L) = ops.analytic_layer(side, kx, ky, self.epsilon, modes=modes)
+            self.IC = 1 / self.epsilon if modes else None
+        else:
+            raise NotImplementedError(f"formulation {self.formulation}")
+        self.W, self.V, self.L = (W, V, L) if modes else (None, None, None)
+        self.kx, self.ky = (kx, ky) if modes else (None, None)
+
+
+def scattering_identity(B, n, device):
+    """[[0, I], [I, 0]] per batch item (khepri/alternative.py:220-232)."""
+    S = torch.zeros((B, 2, 2, n, n), dtype=ops.CDT, device=device)
+    eye = torch.eye(n, dtype=ops.CDT, device=device)
+    S[:, 0, 1] = eye
+    S[:, 1, 0] = eye
+    return S
+
+
+def stack_layers(pw, layers, mask, reverse=True, ws=None):
+    """Forward chain S_I * S_0 * S_1 * ... (copies kept where mask) and the
+    reverse chain (khepri/layer.py:35-60).
+
+    S_I * S = S * S_I = S exactly, so the products with the identity are not
+    formed.  The reverse chain is only needed for fields (``reverse=False``
+    skips it and returns None for it).
+    """
+    Stot = layers[0].S
+    Sls = [Stot if mask[0] else None]
+    for keep, layer in zip(mask[1:], layers[1:]):
+        Stot, info = ops.redheffer(Stot, layer.S, ws=ws)
+        Sls.append(Stot if keep else None)
+    if not reverse:
+        return Sls, None, Stot
+    rmask = list(reversed(mask[1:]))
+    Srs = []
+    Srev = None
+    for keep, layer in zip(rmask, reversed(layers[1:])):
+        if keep:
+            Srs.append(Srev if Srev is not None else scattering_identity(layer.S.shape[0], layer.S.shape[-1],
+                                                                         layer.S.device))
+        else:
+            Srs.append(None)
+        Srev = layer.S if Srev is None else ops.redheffer(layer.S, Srev, ws=ws)[0]
+    Srs.append(Srev)
+    Srs = list(reversed(Srs))
+    return Sls, Srs, Stot

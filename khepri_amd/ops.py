@@ -1,0 +1,221 @@
+"""Device ops of the RCWA path: thin wrappers that allocate outputs and
+workspaces as torch tensors and launch the libkhx kernels on the current
+stream.  Tensors are complex128 / float64 on a HIP device.
+
+Numerical status words (eigen-solver convergence, exact zero pivots) are
+returned as device int tensors; ``raise_on_info`` turns them into the
+exceptions the reference's numpy calls raise.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+CDT = torch.complex128
+RDT = torch.float64
+
+
+def _p(t):
+    return None if t is None else t.data_ptr()
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ws(nbytes, device):
+    return torch.empty(max(int(nbytes), 16), dtype=torch.uint8, device=device)
+
+
+def _require(t, dtype, name):
+    if t.dtype != dtype or not t.is_contiguous() or t.device.type != "cuda":
+        raise TypeError(f"{name}: expected a contiguous {dtype} tensor on a HIP device")
+
+
+def raise_on_info(info, what):
+    """Map per-item status words to numpy-style exceptions (one host sync)."""
+    if info is None:
+        return
+    v = info.cpu().numpy()
+    if (v > 0).any():
+        raise np.linalg.LinAlgError(f"{what}: eigenvalues did not converge (items {np.nonzero(v > 0)[0][:8]})")
+    if (v < 0).any():
+        raise np.linalg.LinAlgError(f"{what}: singular matrix (items {np.nonzero(v < 0)[0][:8]})")
+
+
+# --------------------------------------------------------------------------
+def kvectors(g, kp, wl):
+    """g (2, N) float64; kp (B, 2) complex; wl (B,) float64 -> kx, ky (B, N)."""
+    lib = _lib.load()
+    B, N = kp.shape[0], g.shape[1]
+    kx = torch.empty((B, N), dtype=CDT, device=kp.device)
+    ky = torch.empty_like(kx)
+    check(lib.khx_kvectors(_p(g), N, _p(kp), _p(wl), B, _p(kx), _p(ky), _stream()), "khx_kvectors")
+    return kx, ky
+
+
+def convmat(eps, pw):
+    """eps (count, nx, ny) float64 or complex128 -> C (count, N, N)."""
+    lib = _lib.load()
+    P, Q = pw
+    if eps.dim() == 2:
+        eps = eps.unsqueeze(0)
+    count, nx, ny = eps.shape
+    if nx // 2 + P - 1 >= nx or ny // 2 + Q - 1 >= ny:
+        raise IndexError("pixmap too small for the requested number of plane waves")
+    cplx = 1 if eps.is_complex() else 0
+    eps = eps.contiguous()
+    C = torch.empty((count, P * Q, P * Q), dtype=CDT, device=eps.device)
+    ws = _ws(lib.khx_convmat_ws_bytes(P, Q, count), eps.device)
+    check(lib.khx_convmat(_p(eps), cplx, nx, ny, P, Q, count, _p(C), _p(ws), _stream()), "khx_convmat")
+    return C
+
+
+def solve(A, B):
+    """X = A^-1 B for batches (A (b, n, n), B (b, n, m)); inputs untouched."""
+    lib = _lib.load()
+    Af = A.clone().contiguous()
+    X = B.clone().contiguous()
+    b, n, m = X.shape
+    info = torch.empty(b, dtype=torch.int32, device=A.device)
+    check(lib.khx_zgesv(n, m, _p(Af), _p(X), b, _p(info), _stream()), "khx_zgesv")
+    return X, info
+
+
+def inverse(A):
+    b, n, _ = A.shape
+    I = torch.eye(n, dtype=CDT, device=A.device).expand(b, n, n).contiguous()
+    return solve(A, I)
+
+
+def matmul(A, B, alpha=1.0, beta=0.0, C=None):
+    lib = _lib.load()
+    ab = A.dim() == 2
+    bb = B.dim() == 2
+    M, K = A.shape[-2:]
+    N = B.shape[-1]
+    batch = max(1 if ab else A.shape[0], 1 if bb else B.shape[0])
+    if C is None:
+        C = torch.zeros((batch, M, N), dtype=CDT, device=A.device)
+    al = np.array([complex(alpha).real, complex(alpha).imag])
+    be = np.array([complex(beta).real, complex(beta).imag])
+    check(lib.khx_zgemm(M, N, K, al.ctypes.data, _p(A.contiguous()), int(ab), _p(B.contiguous()), int(bb),
+                        be.ctypes.data, _p(C), batch, _stream()), "khx_zgemm")
+    return C
+
+
+def eig(A):
+    """Eigen-decomposition of a batch of general complex matrices."""
+    lib = _lib.load()
+    Af = A.clone().contiguous()
+    b, n, _ = Af.shape
+    lam = torch.empty((b, n), dtype=CDT, device=A.device)
+    W = torch.empty((b, n, n), dtype=CDT, device=A.device)
+    info = torch.empty(b, dtype=torch.int32, device=A.device)
+    ws = _ws(lib.khx_eig_ws_bytes(n, b), A.device)
+    check(lib.khx_eig(n, _p(Af), b, _p(lam), _p(W), _p(info), _p(ws), _stream()), "khx_eig")
+    return lam, W, info
+
+
+def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None):
+    """S (B, 2, 2, n, n) of a patterned layer; (W, V, lam) too when modes."""
+    lib = _lib.load()
+    B, N = kx.shape
+    n = 2 * N
+    dev = kx.device
+    S = torch.empty((B, 2, 2, n, n), dtype=CDT, device=dev)
+    W = V = lam = None
+    if modes:
+        W = torch.empty((B, n, n), dtype=CDT, device=dev)
+        V = torch.empty_like(W)
+        lam = torch.empty((B, n), dtype=CDT, device=dev)
+    info = torch.empty(B, dtype=torch.int32, device=dev)
+    need = lib.khx_patterned_layer_ws_bytes(n, B)
+    if ws is None or ws.numel() < need:
+        ws = _ws(need, dev)
+    check(lib.khx_patterned_layer(_p(C), _p(IC), _p(kx), _p(ky), N, B, _p(k0d), _p(S), _p(W), _p(V), _p(lam),
+                                  _p(info), _p(ws), _stream()), "khx_patterned_layer")
+    return S, (W, V, lam), info
+
+
+def analytic_layer(kind, kx, ky, eps, k0d=None, modes=False):
+    """kind 'uniform' | 'reflexion' | 'transmission'."""
+    lib = _lib.load()
+    B, N = kx.shape
+    n = 2 * N
+    dev = kx.device
+    S = torch.empty((B, 2, 2, n, n), dtype=CDT, device=dev)
+    W = V = lam = None
+    if modes:
+        W = torch.empty((B, n, n), dtype=CDT, device=dev)
+        V = torch.empty_like(W)
+        lam = torch.empty((B, n), dtype=CDT, device=dev)
+    e = np.array([complex(eps).real, complex(eps).imag])
+    if kind == "uniform":
+        check(lib.khx_uniform_layer(_p(kx), _p(ky), N, B, e.ctypes.data, _p(k0d), _p(S), _p(W), _p(V), _p(lam),
+                                    _stream()), "khx_uniform_layer")
+    else:
+        side = 1 if kind == "reflexion" else 2
+        check(lib.khx_halfspace(side, _p(kx), _p(ky), N, B, e.ctypes.data, _p(S), _p(W), _p(V), _p(lam),
+                                _stream()), "khx_halfspace")
+    return S, (W, V, lam)
+
+
+def redheffer(SA, SB, out=None, ws=None):
+    lib = _lib.load()
+    ab = SA.dim() == 4
+    bb = SB.dim() == 4
+    n = SA.shape[-1]
+    B = max(1 if ab else SA.shape[0], 1 if bb else SB.shape[0])
+    dev = SA.device
+    S = out if out is not None else torch.empty((B, 2, 2, n, n), dtype=CDT, device=dev)
+    info = torch.empty(B, dtype=torch.int32, device=dev)
+    need = lib.khx_redheffer_ws_bytes(n, B)
+    if ws is None or ws.numel() < need:
+        ws = _ws(need, dev)
+    check(lib.khx_redheffer(_p(SA.contiguous()), int(ab), _p(SB.contiguous()), int(bb), n, B, _p(S), _p(info),
+                            _p(ws), _stream()), "khx_redheffer")
+    return S, info
+
+
+def poynting(Stot, g, kp, wl, te, tm, epsi, epse, orders=False):
+    lib = _lib.load()
+    B = Stot.shape[0]
+    N = g.shape[1]
+    dev = Stot.device
+    R = torch.empty(B, dtype=RDT, device=dev)
+    T = torch.empty(B, dtype=RDT, device=dev)
+    Ro = To = None
+    if orders:
+        Ro = torch.empty((B, N), dtype=RDT, device=dev)
+        To = torch.empty((B, N), dtype=RDT, device=dev)
+    ei = np.array([complex(epsi).real, complex(epsi).imag])
+    ee = np.array([complex(epse).real, complex(epse).imag])
+    check(lib.khx_poynting(_p(Stot), N, _p(g), _p(kp), _p(wl), _p(te), _p(tm), ei.ctypes.data, ee.ctypes.data, B,
+                           _p(R), _p(T), _p(Ro), _p(To), _stream()), "khx_poynting")
+    return R, T, Ro, To
+
+
+def joint_subspace(x, Nb, kind, blocks):
+    """x (items*Nb, blocks..., 2Nb, 2Nb) -> (items, blocks..., 2Nb^2, 2Nb^2)."""
+    lib = _lib.load()
+    x = x.contiguous()
+    items = x.shape[0] // Nb
+    ne = 2 * Nb * Nb
+    shape = (items, 2, 2, ne, ne) if blocks == 4 else (items, ne, ne)
+    out = torch.empty(shape, dtype=CDT, device=x.device)
+    check(lib.khx_joint_subspace(_p(x), Nb, blocks, kind, items, _p(out), _stream()), "khx_joint_subspace")
+    return out
+
+
+def joint_subspace_diag(v, Nb, kind):
+    """v (items*Nb, 2Nb) -> (items, 2Nb^2)."""
+    lib = _lib.load()
+    v = v.contiguous()
+    items = v.shape[0] // Nb
+    out = torch.empty((items, 2 * Nb * Nb), dtype=CDT, device=v.device)
+    check(lib.khx_joint_subspace_diag(_p(v), Nb, kind, items, _p(out), _stream()), "khx_joint_subspace_diag")
+    return out

@@ -1,0 +1,37 @@
+"""The C-ABI library loads and exports every entry point include/khx.h declares
+(no kernel is launched: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "khx.h")).read()
+    return sorted(set(re.findall(r"\b(khx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    assert "khx_eig" in names and "khx_redheffer" in names and len(names) >= 15
+
+
+def test_library_exports_all_declared_symbols():
+    from khepri_amd import _lib
+    lib = _lib.load()
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+    # every declared symbol has a ctypes signature in the binding
+    assert set(declared_symbols()) <= set(_lib.SIGNATURES)
+
+
+def test_host_only_entry_points():
+    from khepri_amd import _lib
+    lib = _lib.load()
+    assert lib.khx_abi_version() == 1
+    assert lib.khx_eig_ws_bytes(50, 4) >= 2 * 50 * 50 * 16 * 4
+    assert lib.khx_redheffer_ws_bytes(162, 2) >= 8 * 162 * 162 * 16 * 2
+    # argument validation happens on the host, before any launch
+    assert lib.khx_halfspace(3, None, None, 25, 1, None, None, None, None, None, None) == -1
+    assert b"side" in lib.khx_last_error()

@@ -1,0 +1,146 @@
+"""End-to-end parity of khepri_amd.Crystal on the GPU against the reference's
+golden vectors (tests/golden) and its own known-answer test.
+
+Tolerance: max |dR|, |dT| <= 1e-9 (FP64 path); NaN patterns of evanescent
+incidence must match exactly."""
+import numpy as np
+import pytest
+
+from conftest import golden
+from khepri_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-9
+
+
+def disc_crystal(pw, pixmap, fields=False, epsi=1, epse=1):
+    from khepri_amd import Crystal
+    cl = Crystal(pw, epsi=epsi, epse=epse)
+    cl.add_layer_uniform("S1", 1, 1.1)
+    cl.add_layer_pixmap("Scyl", pixmap, 0.55)
+    cl.set_device(["Scyl", "S1", "Scyl"], [fields] * 3)
+    return cl
+
+
+def test_disc_sweep_batched(gpu):
+    g = golden("disc_sweep.npz")
+    cfg = W.disc_stack_sweep()
+    cl = disc_crystal((5, 5), cfg["pixmap"])
+    cl.set_source(1 / cfg["freqs"], 1.0, 0.0, 0.0, 0.0)
+    cl.solve()
+    R, T = cl.poynting_flux_end()
+    assert R.shape == (151,)
+    assert np.abs(R - g["R"]).max() <= TOL and np.abs(T - g["T"]).max() <= TOL
+
+
+def test_disc_sweep_reference_protocol(gpu):
+    """One scalar set_source / solve / poynting_flux_end at a time, as the
+    reference README loop does."""
+    g = golden("disc_sweep.npz")
+    cfg = W.disc_stack_sweep()
+    cl = disc_crystal((5, 5), cfg["pixmap"], fields=True)
+    for i in [0, 40, 150]:
+        cl.set_source(1 / cfg["freqs"][i], 1.0, 0, 0.0, 0.0)
+        cl.solve()
+        R, T = cl.poynting_flux_end()
+        assert isinstance(R, (float, np.floating))
+        assert abs(R - g["R"][i]) <= TOL and abs(T - g["T"][i]) <= TOL
+    cl.set_source(2.0, 1.0, 0, 0.0, 0.0)
+    cl.solve()
+    np.testing.assert_allclose(cl.layers["Scyl"].S.cpu().numpy()[0], g["S_layer"], atol=1e-10)
+    np.testing.assert_allclose(cl.Stot.cpu().numpy()[0], g["Stot"], atol=1e-10)
+
+
+def test_disc_bz_batched_with_nan_pattern(gpu):
+    g = golden("disc_bz.npz")
+    cfg = W.disc_stack_bz()
+    idx = g["index"]
+    cl = disc_crystal((9, 9), cfg["pixmap"])
+    cl.set_source(1 / cfg["freqs"][idx], 1.0, 0.0, kp=cfg["kp"][idx])
+    cl.solve()
+    R, T = cl.poynting_flux_end()
+    np.testing.assert_array_equal(np.isnan(R), np.isnan(g["R"]))
+    np.testing.assert_array_equal(np.isnan(T), np.isnan(g["T"]))
+    np.testing.assert_array_equal(np.isinf(T), np.isinf(g["T"]))
+    fin = np.isfinite(g["T"])
+    assert fin.sum() >= 20
+    assert np.abs(R[fin] - g["R"][fin]).max() <= TOL and np.abs(T[fin] - g["T"][fin]).max() <= TOL
+
+
+def test_fresnel_thin_film(gpu):
+    """Reference known-answer test test/integration/test_complex_eps.py:13-42,
+    run batched over its 50 wavelengths."""
+    from khepri_amd import Crystal
+    c, e0, sigma, h = 299792458, 8.85418782e-12, 0.01e6, 1.2
+    covera = c / 1e-6
+    wls = np.linspace(0.7, 2.0)
+    R = []
+    for wl in wls:   # eps depends on the wavelength: one crystal per point, like the reference test
+        omega = np.pi * 2 * covera / wl
+        eps = 1.6 ** 2 - 1j * sigma / omega / e0
+        cl = Crystal((1, 1))
+        cl.add_layer_uniform("1", eps, h)
+        cl.set_device(["1"])
+        cl.set_source(wl, 1, 1)
+        cl.solve()
+        R.append(cl.poynting_flux_end()[0])
+    omega = np.pi * 2 * covera / wls
+    eps = 1.6 ** 2 + 1j * sigma / omega / e0
+    n2 = np.conj(np.sqrt(eps))
+    r12 = (1 - n2) / (1 + n2)
+    r23 = (n2 - 1) / (n2 + 1)
+    ph = np.exp(-2j * 2 * np.pi / wls * n2 * h)
+    rf = np.abs((r12 + r23 * ph) / (1 + r12 * r23 * ph)) ** 2
+    np.testing.assert_allclose(np.array(R), rf, rtol=1e-7)
+
+
+def test_random_devices(gpu):
+    from khepri_amd import Crystal
+    g = golden("random_devices.npz")
+    devs = W.random_device_params()
+    for k, i in enumerate(g["index"]):
+        dv = devs[i]
+        cl = Crystal(dv["pw"])
+        names = []
+        for j, ly in enumerate(dv["layers"]):
+            cl.add_layer_pixmap(f"L{j}", W.render_layer(ly), ly["depth"])
+            names.append(f"L{j}")
+        cl.set_device(names, False)
+        cl.set_source(1 / dv["freq"], 1.0, 1.0, dv["theta"], dv["phi"])
+        cl.solve()
+        r, t = cl.poynting_flux_end()
+        assert abs(r - g["R"][k]) <= TOL and abs(t - g["T"][k]) <= TOL, (i, r, t)
+        if i == int(g["orders_index"]):
+            (_, ro), (_, to) = cl.poynting_flux_end(only_total=False)
+            np.testing.assert_allclose(ro, g["R_orders"], atol=1e-10)
+            np.testing.assert_allclose(to, g["T_orders"], atol=1e-10)
+
+
+def test_grad_grid_forward(gpu):
+    from khepri_amd import Crystal
+    g = golden("grad_grid.npz")
+    cfg = W.grad_grid()
+    cl = Crystal((11, 11))
+    names = []
+    for j, pix in enumerate(cfg["pixmaps"]):
+        cl.add_layer_pixmap(f"G{j}", pix, cfg["depth"])
+        names.append(f"G{j}")
+    cl.set_device(names, False)
+    cl.set_source(1 / cfg["freqs"][g["findex"]], 1.0, 0.0)
+    cl.solve()
+    R, T = cl.poynting_flux_end()
+    assert np.abs(R - g["R"]).max() <= TOL and np.abs(T - g["T"]).max() <= TOL
+
+
+def test_chunked_batch_equals_resident(gpu, monkeypatch):
+    cfg = W.disc_stack_sweep()
+    cl = disc_crystal((5, 5), cfg["pixmap"])
+    f = cfg["freqs"][:20]
+    cl.set_source(1 / f, 1.0, 0.0)
+    cl.solve()
+    R1, T1 = cl.poynting_flux_end()
+    monkeypatch.setenv("KHX_CHUNK", "7")
+    cl.solve()
+    R2, T2 = cl.poynting_flux_end()
+    np.testing.assert_array_equal(R1, R2)
+    np.testing.assert_array_equal(T1, T2)

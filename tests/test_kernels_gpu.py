@@ -1,0 +1,182 @@
+"""Kernel-level parity on the GPU: each libkhx entry point against numpy /
+the oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import rcwa
+from khepri_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def crand(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+def dev(a, gpu):
+    return torch.as_tensor(np.ascontiguousarray(a), device=gpu)
+
+
+@pytest.mark.parametrize("M,N,K,batch", [(50, 50, 50, 3), (162, 162, 162, 2), (13, 27, 9, 4), (162, 81, 81, 2)])
+def test_zgemm(gpu, M, N, K, batch):
+    from khepri_amd import ops
+    rng = np.random.default_rng(M + N + K)
+    A, B, C0 = crand(rng, batch, M, K), crand(rng, batch, K, N), crand(rng, batch, M, N)
+    out = ops.matmul(dev(A, gpu), dev(B, gpu), alpha=1 - 0.5j, beta=0.25 + 1j, C=dev(C0, gpu).clone())
+    ref = (1 - 0.5j) * A @ B + (0.25 + 1j) * C0
+    np.testing.assert_allclose(out.cpu().numpy(), ref, atol=1e-12 * np.abs(ref).max())
+    # broadcast B
+    out = ops.matmul(dev(A, gpu), dev(B[0], gpu))
+    np.testing.assert_allclose(out.cpu().numpy(), A @ B[0], atol=1e-12 * np.abs(A @ B[0]).max())
+
+
+@pytest.mark.parametrize("n,m,batch", [(1, 1, 3), (50, 50, 4), (162, 324, 2), (97, 3, 3)])
+def test_zgesv(gpu, n, m, batch):
+    from khepri_amd import ops
+    rng = np.random.default_rng(n * 7 + m)
+    A, B = crand(rng, batch, n, n), crand(rng, batch, n, m)
+    X, info = ops.solve(dev(A, gpu), dev(B, gpu))
+    assert (info.cpu().numpy() == 0).all()
+    ref = np.linalg.solve(A, B)
+    np.testing.assert_allclose(X.cpu().numpy(), ref, atol=1e-10 * np.abs(ref).max())
+
+
+def test_zgesv_singular_reports_info(gpu):
+    from khepri_amd import ops
+    A = np.zeros((2, 4, 4), dtype=complex)
+    A[0] = np.eye(4)
+    A[1, :, :2] = 1.0
+    _, info = ops.solve(dev(A, gpu), dev(np.ones((2, 4, 1), dtype=complex), gpu))
+    info = info.cpu().numpy()
+    assert info[0] == 0 and info[1] > 0
+
+
+@pytest.mark.parametrize("n,batch", [(1, 2), (2, 3), (5, 4), (50, 6), (162, 2)])
+def test_eig_random(gpu, n, batch):
+    from khepri_amd import ops
+    rng = np.random.default_rng(n)
+    A = crand(rng, batch, n, n)
+    lam, Wm, info = ops.eig(dev(A, gpu))
+    assert (info.cpu().numpy() == 0).all()
+    lam, Wm = lam.cpu().numpy(), Wm.cpu().numpy()
+    for b in range(batch):
+        res = np.abs(A[b] @ Wm[b] - Wm[b] * lam[b]).max() / np.abs(A[b]).max()
+        assert res < 1e-12 * max(1, n / 10)
+        np.testing.assert_allclose(np.linalg.norm(Wm[b], axis=0), 1.0, atol=1e-13)
+        ref = np.linalg.eigvals(A[b])
+        d = np.abs(lam[b][:, None] - ref[None, :])
+        assert d.min(axis=1).max() < 1e-10 * np.abs(ref).max()
+
+
+def test_eig_structured_degenerate(gpu):
+    """P*Q of the disc layer at normal incidence has degenerate eigenvalue pairs."""
+    from khepri_amd import ops
+    pix = W.disc_pixmap()
+    C = rcwa.convolution_matrix(pix, (5, 5))
+    g = rcwa.g_vectors((5, 5), np.eye(2))
+    kx, ky, _ = rcwa.k_vectors(g, (0, 0), 2.0)
+    P, Q = rcwa.structured_pq(kx, ky, C)
+    M = P @ Q
+    lam, Wm, info = ops.eig(dev(M[None], gpu))
+    lam, Wm = lam.cpu().numpy()[0], Wm.cpu().numpy()[0]
+    assert np.abs(M @ Wm - Wm * lam).max() / np.abs(M).max() < 1e-12
+    assert np.linalg.cond(Wm) < 1e6
+
+
+@pytest.mark.parametrize("pw,cplx", [((5, 5), False), ((9, 9), False), ((3, 7), True)])
+def test_convmat(gpu, pw, cplx):
+    from khepri_amd import ops
+    rng = np.random.default_rng(5)
+    pix = W.disc_pixmap()
+    if cplx:
+        pix = pix + 1j * rng.uniform(0, 0.1, pix.shape)
+    C = ops.convmat(dev(pix, gpu), pw)[0].cpu().numpy()
+    ref = rcwa.convolution_matrix(pix, pw)
+    np.testing.assert_allclose(C, ref, atol=1e-14 * np.abs(ref).max())
+    # gather is exact: entries with equal (dp, dq) are bit-identical
+    P = pw[0]
+    assert C[0, 0] == C[P + 1, P + 1]
+
+
+def _layer_inputs(pw, kp, wl):
+    g = rcwa.g_vectors(pw, np.eye(2))
+    kx, ky, _ = rcwa.k_vectors(g, kp, wl)
+    return g, kx, ky
+
+
+@pytest.mark.parametrize("pw,kp,wl", [((5, 5), (0.0, 0.0), 2.0), ((5, 5), (0.4, -0.3), 1.7),
+                                      ((9, 9), (0.2, 0.1), 1.9)])
+def test_patterned_layer(gpu, pw, kp, wl):
+    from khepri_amd import ops
+    pix = W.disc_pixmap()
+    C = rcwa.convolution_matrix(pix, pw)
+    g, kx, ky = _layer_inputs(pw, kp, wl)
+    k0 = 2 * np.pi / wl
+    Wr, Vr, Lr = rcwa.structured_layer_modes(kx, ky, C)
+    W0, V0 = rcwa.free_space_eigenmodes(kx, ky)
+    Sref = rcwa.build_scatmat(Wr, Vr, W0, V0, Lr, 0.55, k0)
+    Cd = dev(C, gpu)
+    IC, _ = ops.inverse(Cd[None])
+    k0d = dev(np.array([k0 * 0.55]), gpu)
+    S, (Wm, Vm, L), info = ops.patterned_layer(Cd, IC[0], dev(kx[None], gpu), dev(ky[None], gpu), k0d, modes=True)
+    assert (info.cpu().numpy() == 0).all()
+    np.testing.assert_allclose(S.cpu().numpy()[0], Sref, atol=1e-10)
+    # modes are a valid eigen-decomposition with the principal branch
+    P, Q = rcwa.structured_pq(kx, ky, C)
+    Wm, L = Wm.cpu().numpy()[0], L.cpu().numpy()[0]
+    assert np.abs(P @ Q @ Wm - Wm * L ** 2).max() / np.abs(P @ Q).max() < 1e-11
+    assert (L.real >= 0).all()
+
+
+@pytest.mark.parametrize("kind,eps", [("uniform", 4.0 - 0.2j), ("uniform", 1.0), ("reflexion", 2.25),
+                                      ("transmission", 1.44)])
+def test_analytic_layers(gpu, kind, eps):
+    from khepri_amd import ops
+    pw, kp, wl, d = (3, 3), (0.5, 0.3), 1.3, 0.7
+    g, kx, ky = _layer_inputs(pw, kp, wl)
+    W0, V0 = rcwa.free_space_eigenmodes(kx, ky)
+    k0 = 2 * np.pi / wl
+    if kind == "uniform":
+        Wr, Vr, Lr = rcwa.uniform_layer_modes(kx, ky, eps)
+        Sref = rcwa.build_scatmat(Wr, Vr, W0, V0, Lr, d, k0)
+    elif kind == "reflexion":
+        Sref, Wr, Vr, Lr = rcwa.halfspace_reflection(kx, ky, W0, V0, eps)
+    else:
+        Sref, Wr, Vr, Lr = rcwa.halfspace_transmission(kx, ky, W0, V0, eps)
+    S, (Wm, Vm, L) = ops.analytic_layer(kind, dev(kx[None], gpu), dev(ky[None], gpu), eps,
+                                        dev(np.array([k0 * d]), gpu), modes=True)
+    np.testing.assert_allclose(S.cpu().numpy()[0], Sref, atol=1e-12)
+    np.testing.assert_allclose(Vm.cpu().numpy()[0], Vr, atol=1e-12)
+    np.testing.assert_allclose(L.cpu().numpy()[0], Lr, atol=1e-12)
+
+
+def test_redheffer(gpu):
+    from khepri_amd import ops
+    rng = np.random.default_rng(11)
+    n, B = 50, 3
+    SA = 0.2 * crand(rng, B, 2, 2, n, n)
+    SB = 0.2 * crand(rng, B, 2, 2, n, n)
+    S, info = ops.redheffer(dev(SA, gpu), dev(SB, gpu))
+    ref = np.array([rcwa.redheffer(SA[b], SB[b]) for b in range(B)])
+    np.testing.assert_allclose(S.cpu().numpy(), ref, atol=1e-12)
+
+
+def test_poynting_matches_oracle(gpu):
+    from khepri_amd import ops
+    from oracle import crystal as OC
+    pix = W.disc_pixmap(64, 6.0, 0.3, 1.0)
+    cl = OC.Crystal((3, 3), epsi=1.0, epse=2.25)
+    cl.add_layer_pixmap("P", pix, 0.3)
+    cl.set_device(["P"])
+    cl.set_source(1 / 0.6, 1.0, 0.4, 25.0, 70.0)
+    cl.solve()
+    (R, Ro), (T, To) = cl.poynting_flux_end(only_total=False)
+    kp = np.array([cl.kp], dtype=complex)
+    out = ops.poynting(dev(cl.Stot[None], gpu), dev(cl.expansion.g, gpu), dev(kp, gpu),
+                       dev(np.array([cl.wavelength]), gpu), dev(np.array([1.0 + 0j]), gpu),
+                       dev(np.array([0.4 + 0j]), gpu), 1.0, 2.25, orders=True)
+    Rg, Tg, Rog, Tog = (x.cpu().numpy()[0] for x in out)
+    assert abs(Rg - R.real) < 1e-13 and abs(Tg - T.real) < 1e-13
+    np.testing.assert_allclose(Rog, Ro.real, atol=1e-14)
+    np.testing.assert_allclose(Tog, To.real, atol=1e-14)
