@@ -1,0 +1,173 @@
+#!/usr/bin/env python
+"""Throughput of the batched RCWA solve path on one MI355X.
+
+Workload (one "step"): the Bloch-sweep workload at 81 harmonics -- pw (9,9),
+3-layer stack [disc, gap, disc] (128x128 pixmap eps 12 with a centred disc
+r = 0.4 of eps 1, depth 0.55; uniform eps 1 gap, depth 1.1), 64 frequencies
+linspace(0.45, 0.65) x a 32x32 cell-centred Bloch grid = 65 536 solves, each
+(device, lambda, k_par) -> (R, T).  Inputs are built on the device before
+the timed region; a step is Crystal.solve() + poynting_flux_end() over all
+solves (R/T copied back to the host).
+
+Prints one JSON line (the driver's contract) with a roofline entry for the
+dominant kernel (HIP events on its launch stream) and a CPU baseline (the
+oracle on a bounded sample of the same workload, 1 BLAS thread).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s @81 harmonics, plain FP64 vs mixed+batched; grad evals/s; max|ΔT| vs FP64"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def eig_flops(n):
+    # Hessenberg + complex Schur + triangular eigenvectors: 110 n^3 real flops
+    # per matrix, minus the 4 n^3 back-transform W = Z X that runs as a GEMM.
+    return 106.0 * n ** 3
+
+
+def build_crystal(nsolves):
+    from khepri_amd import Crystal, workloads
+    cfg = workloads.disc_stack_bz()
+    idx = np.arange(len(cfg["freqs"]))
+    if nsolves < len(idx):
+        idx = np.linspace(0, len(idx) - 1, nsolves).astype(int)
+    cl = Crystal(cfg["pw"])
+    cl.add_layer_uniform("S1", cfg["eps_gap"], cfg["depth_gap"])
+    cl.add_layer_pixmap("Scyl", cfg["pixmap"], cfg["depth_disc"])
+    cl.set_device(["Scyl", "S1", "Scyl"], False)
+    cl.set_source(1 / cfg["freqs"][idx], 1.0, 0.0, kp=cfg["kp"][idx])
+    return cl, cfg, idx
+
+
+def cpu_baseline(cfg, gpu_R, gpu_T, idx_all, sample):
+    """The oracle (numpy restatement, same op sequence as the reference) on a
+    bounded, evenly spread sample of the workload; 1 BLAS thread."""
+    from threadpoolctl import threadpool_limits
+    from oracle import crystal as OC
+    pos = np.linspace(0, len(idx_all) - 1, sample).astype(int)
+    cl = OC.Crystal(cfg["pw"])
+    cl.add_layer_uniform("S1", cfg["eps_gap"], cfg["depth_gap"])
+    cl.add_layer_pixmap("Scyl", cfg["pixmap"], cfg["depth_disc"])
+    cl.set_device(["Scyl", "S1", "Scyl"], False)
+    dR = dT = 0.0
+    with threadpool_limits(limits=1), np.errstate(all="ignore"):
+        t0 = time.perf_counter()
+        for p in pos:
+            s = idx_all[p]
+            cl.set_source(1 / cfg["freqs"][s], 1.0, 0.0, kp=tuple(cfg["kp"][s]))
+            cl.solve()
+            r, t = cl.poynting_flux_end()
+            if np.isfinite(t):
+                dR = max(dR, abs(r - gpu_R[p]))
+                dT = max(dT, abs(t - gpu_T[p]))
+            elif not (np.isnan(t) == np.isnan(gpu_T[p])):
+                dT = float("inf")
+        el = time.perf_counter() - t0
+    return dict(value=round(sample / el, 3), unit="solves/s", cores=1, kind="port",
+                sample=f"{sample} solves evenly spread over the 65536-solve workload, oracle (numpy complex128, "
+                       f"reference op sequence incl. reverse star chain), 1 BLAS thread, {el:.1f} s"), dR, dT
+
+
+def load_traffic(kernel, workload):
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("workload") != workload:
+        return None
+    return d.get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--solves", type=int, default=65536, help="subset of the workload (default: all)")
+    ap.add_argument("--cpu-sample", type=int, default=40, help="oracle solves timed for cpu_baseline (0: skip)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}): every solve is independent, "
+              "so N GPUs would each run the same per-GPU workload; multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from khepri_amd import _lib
+
+    lib = _lib.load()
+    cl, cfg, idx = build_crystal(args.solves)
+    nsol = len(idx)
+    n = 2 * 81
+    print(f"[bench] {nsol} solves per step, warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        cl.solve()
+        R, T = cl.poynting_flux_end()
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    lib.khx_prof_enable(1)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        cl.solve()
+        R, T = cl.poynting_flux_end()
+        print(f"[bench] step {s} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kern = {}
+    import ctypes
+    for name in ("zeig_schur_kernel", "zgesv_kernel", "zgemm_kernel"):
+        tot, cnt = ctypes.c_double(), ctypes.c_longlong()
+        _lib.check(lib.khx_prof_collect(name.encode(), ctypes.byref(tot), ctypes.byref(cnt)), "prof")
+        kern[name] = (tot.value, cnt.value)
+    lib.khx_prof_enable(0)
+    value = nsol * args.steps / el
+
+    # dominant kernel roofline: the eigen-solver (FP64, one matrix per workgroup)
+    eig_ms, eig_n = kern["zeig_schur_kernel"]
+    avg_ms = eig_ms / max(eig_n, 1)
+    matrices_per_launch = nsol * args.steps / max(eig_n, 1)
+    flops_per_launch = eig_flops(n) * matrices_per_launch
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    workload = "disc_stack_bz pw(9,9) 65536 solves"
+    roofline = dict(bound="mfma", kernel="zeig_schur_kernel", achieved=round(achieved, 4), peak=PEAK_FP64_TFLOPS,
+                    unit="TFLOP/s", frac=round(achieved / PEAK_FP64_TFLOPS, 6),
+                    traffic=load_traffic("zeig_schur_kernel", workload),
+                    avg_launch_ms=round(avg_ms, 4), matrices_per_launch=matrices_per_launch,
+                    flops_per_matrix=eig_flops(n))
+    share = {k: round(v[0] / (el * 1e3 * 1.0), 4) for k, v in kern.items()}
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload if nsol == 65536 else f"disc_stack_bz pw(9,9) {nsol}-solve subset",
+                   "pw": [9, 9], "harmonics": 81, "solves_per_step": nsol, "layers": ["Scyl", "S1", "Scyl"],
+                   "precision": "fp64 (plain)", "chunk": cl._chunk(nsol)},
+        "roofline": roofline,
+        "kernel_time_share": share,
+    }
+    if args.cpu_sample > 0:
+        Rn, Tn = np.asarray(R), np.asarray(T)
+        cb, dR, dT = cpu_baseline(cfg, Rn, Tn, idx, args.cpu_sample)
+        out["cpu_baseline"] = cb
+        out["max_abs_dR"] = dR
+        out["max_abs_dT"] = dT
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -58,11 +58,13 @@ int khx_convmat(const double* eps, int is_complex, int nx, int ny, int P, int Q,
  * info[b] = k+1 when the k-th pivot is exactly zero (numpy: LinAlgError). */
 int khx_zgesv(int n, int nrhs, double* A, double* B, int batch, int* info, void* stream);
 
-/* Batched complex GEMM C = alpha A B + beta C, A (batch,M,K), B (batch,K,N),
- * C (batch,M,N); a_bcast / b_bcast != 0 shares one A / B across the batch.
- * (numpy matmul as used throughout khepri/alternative.py)                  */
-int khx_zgemm(int M, int N, int K, const double* alpha, const double* A, int a_bcast, const double* B,
-              int b_bcast, const double* beta, double* C, int batch, void* stream);
+/* Strided-batched complex GEMM C = alpha A B + beta C (numpy matmul as used
+ * throughout khepri/alternative.py).  Item b of A is the M x K row-major
+ * matrix at A + b*strideA with row stride lda (complex elements); a batch
+ * stride of 0 shares one operand.  alpha, beta: host complex scalars.      */
+int khx_zgemm(int M, int N, int K, const double* alpha, const double* A, int lda, long long strideA,
+              const double* B, int ldb, long long strideB, const double* beta, double* C, int ldc,
+              long long strideC, int batch, void* stream);
 
 /* Batched eigen-decomposition of general complex matrices
  * (numpy.linalg.eig / LAPACK zgeev as called at khepri/alternative.py:172).
@@ -110,9 +112,42 @@ int khx_redheffer(const double* SA, int a_bcast, const double* SB, int b_bcast, 
  * kp complex (batch, 2); wl (batch); te, tm complex (batch);
  * epsi, epse: complex scalars in host memory.  R, T (batch); Rord, Tord
  * (batch, N) per diffraction order when non-NULL.                          */
+size_t khx_poynting_ws_bytes(int N, int batch);
 int khx_poynting(const double* Stot, int N, const double* g, const double* kp, const double* wl,
                  const double* te, const double* tm, const double* epsi_host, const double* epse_host, int batch,
-                 double* R, double* T, double* Rord, double* Tord, void* stream);
+                 double* R, double* T, double* Rord, double* Tord, void* ws, void* stream);
+
+/* The same split in its two halves, for the R/T-only chain:
+ * khx_incident writes the incident vector c1p (batch, n) -- zero except the
+ * x / y parts of the central harmonic (khepri/alternative.py:101-128);
+ * khx_poynting_vec takes the output amplitudes sR = Stot11 c1p and
+ * sT = Stot21 c1p, each (batch, n) (khepri/alternative.py:235-245).       */
+int khx_incident(int N, const double* kp, const double* wl, const double* te, const double* tm,
+                 const double* epsi_host, int batch, double* c, void* stream);
+int khx_poynting_vec(const double* sR, const double* sT, int N, const double* g, const double* kp,
+                     const double* wl, const double* epsi_host, const double* epse_host, int batch, double* R,
+                     double* T, double* Rord, double* Tord, void* stream);
+
+/* LU factorisation with stored pivots (LAPACK zgetrf layout, 0-based ipiv)
+ * and the matching solve (zgetrs) -- numpy.linalg.solve split in two, for a
+ * factor reused across several solves.  A, LU (batch, n, n); ipiv (batch, n)
+ * device ints; B (batch, n, nrhs) overwritten by X.                        */
+int khx_lu_factor(int n, double* A, int batch, int* ipiv, int* info, void* stream);
+int khx_lu_solve(int n, int nrhs, const double* LU, const int* ipiv, double* B, int batch, void* stream);
+
+/* Elementwise helpers of the S-matrix algebra: out = diag(dl) A diag(dr)
+ * (strided batches like khx_zgemm; dl (batch, rows), dr (batch, cols),
+ * either may be NULL; out may alias A)
+ * and out = I - A (out may alias A).                                       */
+int khx_diag_scale(const double* A, int lda, long long strideA, const double* dl, const double* dr, int rows,
+                   int cols, int batch, double* out, int ldo, long long strideO, void* stream);
+int khx_identity_minus(const double* A, int n, int batch, double* out, void* stream);
+
+/* Propagation factors X (batch, n) of a uniform layer of the gap medium
+ * (eps = 1), whose S-matrix is exactly [[0, diag X], [diag X, 0]]
+ * (khepri/alternative.py:130-156 + 181-195 with W = W0, V = V0).          */
+int khx_propagation(const double* kx, const double* ky, int N, int batch, const double* k0d, double* X,
+                    void* stream);
 
 /* Extended (twisted-bilayer) basis: scatter the base-layer matrices solved at
  * kp + g_m (m = 0..Nb-1, one per harmonic of the other lattice) into the
@@ -124,6 +159,16 @@ int khx_poynting(const double* Stot, int N, const double* g, const double* kp, c
  * (items, 2Nb^2) (khepri/extension.py:110).                                */
 int khx_joint_subspace(const double* in, int Nb, int blocks, int kind, int items, double* out, void* stream);
 int khx_joint_subspace_diag(const double* in, int Nb, int kind, int items, double* out, void* stream);
+
+/* Measurement support (no counterpart in the reference).
+ * khx_prof_enable(1) clears the records and brackets every launch of the
+ * instrumented kernels (zeig_schur_kernel, zgesv_kernel, zgemm_kernel) with
+ * a HIP event pair on its stream; khx_prof_collect sums the durations of one
+ * kernel (synchronising on its events).  khx_copy is a plain device copy of
+ * `bytes` (multiple of 16) used to calibrate HBM byte counters.            */
+int khx_prof_enable(int on);
+int khx_prof_collect(const char* kernel, double* total_ms, long long* launches);
+int khx_copy(const void* src, void* dst, size_t bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,8 @@ SIGNATURES = {
     "khx_convmat_ws_bytes": (_sz, [_i, _i, _i]),
     "khx_convmat": (_i, [_dp, _i, _i, _i, _i, _i, _i, _dp, _vp, _vp]),
     "khx_zgesv": (_i, [_i, _i, _dp, _dp, _i, _ip, _vp]),
-    "khx_zgemm": (_i, [_i, _i, _i, _dp, _dp, _i, _dp, _i, _dp, _dp, _i, _vp]),
+    "khx_zgemm": (_i, [_i, _i, _i, _dp, _dp, _i, C.c_longlong, _dp, _i, C.c_longlong, _dp, _dp, _i, C.c_longlong,
+                       _i, _vp]),
     "khx_eig_ws_bytes": (_sz, [_i, _i]),
     "khx_eig": (_i, [_i, _dp, _i, _dp, _dp, _ip, _vp, _vp]),
     "khx_patterned_layer_ws_bytes": (_sz, [_i, _i]),
@@ -30,9 +31,20 @@ SIGNATURES = {
     "khx_halfspace": (_i, [_i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _vp]),
     "khx_redheffer_ws_bytes": (_sz, [_i, _i]),
     "khx_redheffer": (_i, [_dp, _i, _dp, _i, _i, _i, _dp, _ip, _vp, _vp]),
-    "khx_poynting": (_i, [_dp, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _dp, _dp, _dp, _dp, _vp]),
+    "khx_poynting_ws_bytes": (_sz, [_i, _i]),
+    "khx_poynting": (_i, [_dp, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _dp, _dp, _dp, _dp, _vp, _vp]),
+    "khx_incident": (_i, [_i, _dp, _dp, _dp, _dp, _dp, _i, _dp, _vp]),
+    "khx_poynting_vec": (_i, [_dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _i, _dp, _dp, _dp, _dp, _vp]),
+    "khx_lu_factor": (_i, [_i, _dp, _i, _ip, _ip, _vp]),
+    "khx_lu_solve": (_i, [_i, _i, _dp, _ip, _dp, _i, _vp]),
+    "khx_diag_scale": (_i, [_dp, _i, C.c_longlong, _dp, _dp, _i, _i, _i, _dp, _i, C.c_longlong, _vp]),
+    "khx_identity_minus": (_i, [_dp, _i, _i, _dp, _vp]),
+    "khx_propagation": (_i, [_dp, _dp, _i, _i, _dp, _dp, _vp]),
     "khx_joint_subspace": (_i, [_dp, _i, _i, _i, _i, _dp, _vp]),
     "khx_joint_subspace_diag": (_i, [_dp, _i, _i, _i, _dp, _vp]),
+    "khx_prof_enable": (_i, [_i]),
+    "khx_prof_collect": (_i, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "khx_copy": (_i, [_vp, _vp, _sz, _vp]),
 }
 
 _lib = None

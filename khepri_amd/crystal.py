@@ -15,7 +15,7 @@ from copy import copy
 import numpy as np
 import torch
 
-from . import ops
+from . import chain, ops
 from .device import default_device
 from .expansion import Expansion, compute_kplanar
 from .extension import ExtendedLayer
@@ -168,9 +168,14 @@ class Crystal:
         per = _bytes_per_item(n, len(set(self.global_stacking)), len(self.global_stacking))
         return int(max(1, min(B, budget // per)))
 
-    def solve(self):
-        """khepri/crystal.py:180-206.  Batches larger than one memory chunk
-        are streamed: each chunk's R/T is computed and its S-matrices freed."""
+    def solve(self, rt_only=None):
+        """khepri/crystal.py:180-206.
+
+        rt_only: compute only what poynting_flux_end needs (Stot11 c1p and
+        Stot21 c1p, see khepri_amd.chain) instead of every S-matrix of the
+        chain.  Default: on when no fields are requested and the batch does
+        not fit one memory chunk -- then the batch is streamed chunk by chunk
+        and only R/T (total and per order) are kept."""
         assert self.source_defined, "Call set_source before solving."
         dev = self._dev()
         names = list(dict.fromkeys(self.global_stacking))   # unique, in stack order
@@ -188,7 +193,13 @@ class Crystal:
         tm_all = torch.as_tensor(self._tm, device=dev)
         self._sources_dev = (kp_all, wl_all, te_all, tm_all)
         want_reverse = getattr(self, "_fields_requested", False)
-        if chunk >= B:
+        if rt_only is None:
+            rt_only = (not want_reverse) and chunk < B
+        if rt_only and want_reverse:
+            raise ValueError("fields need the full S-matrix chain (rt_only=False)")
+        self.S = None
+        self._RT = None
+        if not rt_only and chunk >= B:
             for name in names:
                 self.layers[name].solve_batch(kp_all, wl_all)
             self._check_layers(names)
@@ -198,37 +209,47 @@ class Crystal:
             return
         if want_reverse:
             raise ValueError("fields need the whole batch resident; lower the batch size (KHX_CHUNK)")
+        g = self.expansion.g_device(dev)
+        N = self.expansion.N
         R = torch.empty(B, dtype=torch.float64, device=dev)
         T = torch.empty_like(R)
+        Ro = torch.empty((B, N), dtype=torch.float64, device=dev)
+        To = torch.empty_like(Ro)
         for c0 in range(0, B, chunk):
-            c1 = min(B, c0 + chunk)
+            sl = slice(c0, min(B, c0 + chunk))
+            kp, wl = kp_all[sl], wl_all[sl]
             for name in names:
-                self.layers[name].solve_batch(kp_all[c0:c1], wl_all[c0:c1])
+                self.layers[name].solve_batch(kp, wl, rt_only=rt_only)
             self._check_layers(names)
-            _, _, Stot = stack_layers(self.expansion.pw, stacked, self.stack_retain_mask, reverse=False)
-            r, t, _, _ = ops.poynting(Stot, self.expansion.g_device(dev), kp_all[c0:c1], wl_all[c0:c1],
-                                      te_all[c0:c1], tm_all[c0:c1], self.epsi, self.epse)
-            R[c0:c1] = r
-            T[c0:c1] = t
+            if rt_only:
+                c = ops.incident(N, kp, wl, te_all[sl], tm_all[sl], self.epsi)
+                sR, sT, infos = chain.rt_amplitudes([layer.chain_item() for layer in stacked], c)
+                for info in infos:
+                    self._check_info(info, "star-product junction")
+                r, t, ro, to = ops.poynting_vec(sR, sT, g, kp, wl, self.epsi, self.epse, orders=True)
+            else:
+                _, _, Stot = stack_layers(self.expansion.pw, stacked, self.stack_retain_mask, reverse=False)
+                r, t, ro, to = ops.poynting(Stot, g, kp, wl, te_all[sl], tm_all[sl], self.epsi, self.epse,
+                                            orders=True)
+            R[sl], T[sl], Ro[sl], To[sl] = r, t, ro, to
             for name in names:
                 self.layers[name].S = None
-        self._RT = (R, T)
-        self.S = None
+        self._RT = (R, T, Ro, To)
+
+    def _check_info(self, info, what):
+        if info is not None and os.environ.get("KHX_CHECK_INFO", "1") == "1":
+            ops.raise_on_info(info, what)
 
     def _check_layers(self, names):
         for name in names:
-            info = getattr(self.layers[name], "info", None)
-            if info is not None and os.environ.get("KHX_CHECK_INFO", "1") == "1":
-                ops.raise_on_info(info, f"layer {name}")
+            self._check_info(getattr(self.layers[name], "info", None), f"layer {name}")
 
     # ------------------------------------------------------------------ outputs
     def poynting_flux_end(self, only_total=True):
         """(R, T) -- khepri/crystal.py:363-396."""
         assert self.solved, "Call solve first"
         if self._RT is not None:
-            if not only_total:
-                raise ValueError("per-order fluxes need the whole batch resident (lower KHX_CHUNK)")
-            R, T = (x.cpu().numpy() for x in self._RT)
+            R, T, Ro, To = (x.cpu().numpy() for x in self._RT)
         else:
             kp, wl, te, tm = self._sources_dev
             R, T, Ro, To = ops.poynting(self.Stot, self.expansion.g_device(self._dev()), kp, wl, te, tm,
@@ -236,9 +257,10 @@ class Crystal:
             R, T = R.cpu().numpy(), T.cpu().numpy()
             if not only_total:
                 Ro, To = Ro.cpu().numpy(), To.cpu().numpy()
-                if self._scalar:
-                    return (R[0], Ro[0]), (T[0], To[0])
-                return (R, Ro), (T, To)
+        if not only_total:
+            if self._scalar:
+                return (R[0], Ro[0]), (T[0], To[0])
+            return (R, Ro), (T, To)
         if self._scalar:
             return R[0], T[0]
         return R, T

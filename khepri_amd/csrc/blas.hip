@@ -4,6 +4,7 @@
 // stride of 0 broadcasts one operand to every item (e.g. the convolution
 // matrix of a layer, shared by all frequencies and Bloch vectors).
 #include "khx_internal.h"
+#include "khx_prof.h"
 
 namespace khx {
 
@@ -95,6 +96,7 @@ static int launch_zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat
   const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
   const long long blocks = (long long)tilesM * tilesN * batch;
   KHX_REQUIRE(blocks < (1ll << 31), "zgemm: grid too large");
+  ProfScope ps("zgemm_kernel", s);
   zgemm_kernel<BM, BN, BK, TM, TN><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
       M, N, K, alpha, A.p, A.ld, A.stride, B.p, B.ld, B.stride, beta, C.p, C.ld, C.stride, tilesN,
       tilesM * tilesN);
@@ -251,6 +253,7 @@ int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info) {
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_set = true;
   }
+  ProfScope ps("zgesv_kernel", s);
   zgesv_kernel<<<dim3(batch), dim3(GESV_THREADS), lds, s>>>(n, m, A.p, A.ld, A.stride, B.p, B.ld,
                                                            B.stride, info);
   KHX_LAUNCH_CHECK();
@@ -329,6 +332,62 @@ int zrow_scale(hipStream_t s, Mat D, const cd* d, long long dstride, Mat A, int 
   if (batch <= 0) return 0;
   zrow_scale_kernel<<<ew_grid((long long)rows * cols, batch), 256, 0, s>>>(
       D.p, D.ld, D.stride, d, dstride, A.p, A.ld, A.stride, rows, cols, batch);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace khx
+
+namespace khx {
+
+// D[b][i][j] = dl[b][i] * A[b][i][j] * dr[b][j]  (dl / dr may be null = 1)
+__global__ void zdiag_scale_kernel(const cd* A, int lda, long long sA, const cd* __restrict__ dl,
+                                   const cd* __restrict__ dr, int rows, int cols, int batch, cd* D, int ldd,
+                                   long long sD) {
+  const long long per = (long long)rows * cols;
+  const long long total = per * batch;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long b = e / per;
+    const int rc = (int)(e % per);
+    const int i = rc / cols, j = rc % cols;
+    cd v = A[b * sA + (long long)i * lda + j];
+    if (dl) v = dl[b * rows + i] * v;
+    if (dr) v = v * dr[b * cols + j];
+    D[b * sD + (long long)i * ldd + j] = v;
+  }
+}
+
+// D = I - A
+__global__ void zidentity_minus_kernel(const cd* A, int n, int batch, cd* D) {
+  const long long per = (long long)n * n;
+  const long long total = per * batch;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int rc = (int)(e % per);
+    const cd v = A[e];
+    D[e] = (rc / n == rc % n) ? mkc(1.0 - v.x, -v.y) : mkc(-v.x, -v.y);
+  }
+}
+
+static dim3 grid_big(long long total) {
+  long long g = (total + 255) / 256;
+  if (g > 131072) g = 131072;
+  if (g < 1) g = 1;
+  return dim3((unsigned)g);
+}
+
+int zdiag_scale(hipStream_t s, Mat A, const cd* dl, const cd* dr, int rows, int cols, int batch, Mat D) {
+  if (batch <= 0) return 0;
+  zdiag_scale_kernel<<<grid_big((long long)rows * cols * batch), 256, 0, s>>>(A.p, A.ld, A.stride, dl, dr, rows,
+                                                                              cols, batch, D.p, D.ld, D.stride);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+int zidentity_minus(hipStream_t s, const cd* A, int n, int batch, cd* D) {
+  if (batch <= 0) return 0;
+  zidentity_minus_kernel<<<grid_big((long long)n * n * batch), 256, 0, s>>>(A, n, batch, D);
   KHX_LAUNCH_CHECK();
   return 0;
 }

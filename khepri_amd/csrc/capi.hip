@@ -24,7 +24,16 @@ int star(hipStream_t s, const cd* SA, long long sA, const cd* SB, long long sB, 
          int* info, void* ws);
 int poynting(hipStream_t s, const cd* S, long long sS, int N, const double* g, const cd* kp, const double* wl,
              const cd* te, const cd* tm, cd epsi, cd epse, int batch, double* R, double* T, double* Rord,
-             double* Tord);
+             double* Tord, cd* ws);
+int poynting_vec(hipStream_t s, const cd* sR, const cd* sT, int N, const double* g, const cd* kp, const double* wl,
+                 cd epsi, cd epse, int batch, double* R, double* T, double* Rord, double* Tord);
+int incident(hipStream_t s, int N, const cd* kp, const double* wl, const cd* te, const cd* tm, cd epsi, int batch,
+             cd* c);
+int zgetrf(hipStream_t s, int n, Mat A, int batch, int* ipiv, int* info);
+int zgetrs(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch);
+int zdiag_scale(hipStream_t s, Mat A, const cd* dl, const cd* dr, int rows, int cols, int batch, Mat D);
+int zidentity_minus(hipStream_t s, const cd* A, int n, int batch, cd* D);
+int propagation(hipStream_t s, const cd* kx, const cd* ky, int N, int batch, const double* k0d, cd* X);
 
 int joint_subspace(hipStream_t s, const cd* in, int Nb, int M, int blocks, int kind, int items, cd* out);
 int joint_subspace_vec(hipStream_t s, const cd* in, int Nb, int M, int kind, int items, cd* out);
@@ -68,15 +77,15 @@ int khx_zgesv(int n, int nrhs, double* A, double* B, int batch, int* info, void*
                     batch, info);
 }
 
-int khx_zgemm(int M, int N, int K, const double* alpha, const double* A, int a_bcast, const double* B,
-              int b_bcast, const double* beta, double* C, int batch, void* stream) {
+int khx_zgemm(int M, int N, int K, const double* alpha, const double* A, int lda, long long strideA,
+              const double* B, int ldb, long long strideB, const double* beta, double* C, int ldc,
+              long long strideC, int batch, void* stream) {
   KHX_ENTRY_BEGIN
   KHX_REQUIRE(M > 0 && N > 0 && K > 0 && batch >= 0, "khx_zgemm: bad sizes");
+  KHX_REQUIRE(lda >= K && ldb >= N && ldc >= N, "khx_zgemm: leading dimensions too small");
   const cd al = mkc(alpha[0], alpha[1]), be = mkc(beta[0], beta[1]);
-  return khx::zgemm(as_stream(stream), M, N, K, al,
-                    khx::Mat{const_cast<cd*>(as_cd(A)), K, a_bcast ? 0 : (long long)M * K},
-                    khx::Mat{const_cast<cd*>(as_cd(B)), N, b_bcast ? 0 : (long long)K * N}, be,
-                    khx::Mat{as_cd(C), N, (long long)M * N}, batch);
+  return khx::zgemm(as_stream(stream), M, N, K, al, khx::Mat{const_cast<cd*>(as_cd(A)), lda, strideA},
+                    khx::Mat{const_cast<cd*>(as_cd(B)), ldb, strideB}, be, khx::Mat{as_cd(C), ldc, strideC}, batch);
 }
 
 size_t khx_eig_ws_bytes(int n, int batch) { return khx::zeig_workspace_bytes(n, batch) + 256; }
@@ -129,15 +138,68 @@ int khx_redheffer(const double* SA, int a_bcast, const double* SB, int b_bcast, 
                    info, ws);
 }
 
+size_t khx_poynting_ws_bytes(int N, int batch) { return (size_t)3 * 2 * N * sizeof(cd) * batch + 256; }
+
 int khx_poynting(const double* Stot, int N, const double* g, const double* kp, const double* wl, const double* te,
                  const double* tm, const double* epsi_host, const double* epse_host, int batch, double* R, double* T,
-                 double* Rord, double* Tord, void* stream) {
+                 double* Rord, double* Tord, void* ws, void* stream) {
   KHX_ENTRY_BEGIN
-  KHX_REQUIRE(N > 0 && batch >= 0 && epsi_host && epse_host, "khx_poynting: bad arguments");
+  KHX_REQUIRE(N > 0 && batch >= 0 && epsi_host && epse_host && ws, "khx_poynting: bad arguments");
   const long long s4 = 4ll * (2 * N) * (2 * N);
   return khx::poynting(as_stream(stream), as_cd(Stot), s4, N, g, as_cd(kp), wl, as_cd(te), as_cd(tm),
                        mkc(epsi_host[0], epsi_host[1]), mkc(epse_host[0], epse_host[1]), batch, R, T, Rord,
-                       Tord);
+                       Tord, static_cast<cd*>(ws));
+}
+
+int khx_incident(int N, const double* kp, const double* wl, const double* te, const double* tm,
+                 const double* epsi_host, int batch, double* c, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0 && epsi_host, "khx_incident: bad arguments");
+  return khx::incident(as_stream(stream), N, as_cd(kp), wl, as_cd(te), as_cd(tm), mkc(epsi_host[0], epsi_host[1]),
+                       batch, as_cd(c));
+}
+
+int khx_poynting_vec(const double* sR, const double* sT, int N, const double* g, const double* kp, const double* wl,
+                     const double* epsi_host, const double* epse_host, int batch, double* R, double* T, double* Rord,
+                     double* Tord, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0 && epsi_host && epse_host, "khx_poynting_vec: bad arguments");
+  return khx::poynting_vec(as_stream(stream), as_cd(sR), as_cd(sT), N, g, as_cd(kp), wl,
+                           mkc(epsi_host[0], epsi_host[1]), mkc(epse_host[0], epse_host[1]), batch, R, T, Rord, Tord);
+}
+
+int khx_lu_factor(int n, double* A, int batch, int* ipiv, int* info, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0, "khx_lu_factor: bad sizes");
+  return khx::zgetrf(as_stream(stream), n, khx::Mat{as_cd(A), n, (long long)n * n}, batch, ipiv, info);
+}
+
+int khx_lu_solve(int n, int nrhs, const double* LU, const int* ipiv, double* B, int batch, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && nrhs > 0 && batch >= 0, "khx_lu_solve: bad sizes");
+  return khx::zgetrs(as_stream(stream), n, nrhs, khx::Mat{const_cast<cd*>(as_cd(LU)), n, (long long)n * n}, ipiv,
+                     khx::Mat{as_cd(B), nrhs, (long long)n * nrhs}, batch);
+}
+
+int khx_diag_scale(const double* A, int lda, long long strideA, const double* dl, const double* dr, int rows,
+                   int cols, int batch, double* out, int ldo, long long strideO, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(rows > 0 && cols > 0 && batch >= 0 && lda >= cols && ldo >= cols, "khx_diag_scale: bad sizes");
+  return khx::zdiag_scale(as_stream(stream), khx::Mat{const_cast<cd*>(as_cd(A)), lda, strideA}, as_cd(dl),
+                          as_cd(dr), rows, cols, batch, khx::Mat{as_cd(out), ldo, strideO});
+}
+
+int khx_propagation(const double* kx, const double* ky, int N, int batch, const double* k0d, double* X,
+                    void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0, "khx_propagation: bad sizes");
+  return khx::propagation(as_stream(stream), as_cd(kx), as_cd(ky), N, batch, k0d, as_cd(X));
+}
+
+int khx_identity_minus(const double* A, int n, int batch, double* out, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0, "khx_identity_minus: bad sizes");
+  return khx::zidentity_minus(as_stream(stream), as_cd(A), n, batch, as_cd(out));
 }
 
 int khx_joint_subspace(const double* in, int Nb, int blocks, int kind, int items, double* out, void* stream) {

@@ -14,6 +14,7 @@
 // Each QR rotation touches disjoint row/column/Z element sets, so a rotation
 // is one barrier; the inputs of the next rotation travel through LDS.
 #include "khx_internal.h"
+#include "khx_prof.h"
 
 namespace khx {
 
@@ -366,9 +367,12 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   const size_t lds = (size_t)(4 + EIG_WAVES) * n * sizeof(cd);
   KHX_REQUIRE(lds <= 64 * 1024, "zeig: n too large for the LDS vectors");
   // Z and X use the W stride layout (n*n per item) in the workspace
+  {
+  ProfScope ps("zeig_schur_kernel", s);
   zeig_schur_kernel<<<dim3(batch), dim3(EIG_THREADS), lds, s>>>(n, A.p, A.stride, Z, X, sZ, lam,
                                                                lstride, info);
   KHX_LAUNCH_CHECK();
+  }
   // W = Z X
   KHX_TRY(zgemm(s, n, n, n, mkc(1.0, 0.0), Mat{Z, n, sZ}, Mat{X, n, sZ}, mkc(0.0, 0.0), W, batch));
   znormalize_cols_kernel<<<dim3(batch), dim3(256), 0, s>>>(n, W.p, W.stride);

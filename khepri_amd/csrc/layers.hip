@@ -616,3 +616,33 @@ int joint_subspace_vec(hipStream_t s, const cd* in, int Nb, int M, int kind, int
 }
 
 }  // namespace khx
+
+namespace khx {
+
+// Propagation factors of a uniform layer whose medium equals the gap medium
+// (eps = 1): its S-matrix is exactly [[0, X], [X, 0]] with
+// X = exp(-lam k0 d), lam = i kz (khepri/alternative.py:130-156 with
+// W = W0, V = V0, so A = 2I and B = 0 in build_scatmat).
+__global__ void propagation_kernel(const cd* __restrict__ kx, const cd* __restrict__ ky, int N, int batch,
+                                   const double* __restrict__ k0d, cd* X) {
+  const long long total = (long long)batch * N;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long b = e / N;
+    const int h = (int)(e % N);
+    const cd x = kx[e], y = ky[e];
+    const cd lam = mul_i(kz_masked(mkc(1.0, 0.0) - x * x - y * y));
+    const cd v = expc(-(k0d[b] * lam));
+    X[b * 2 * N + h] = v;
+    X[b * 2 * N + N + h] = v;
+  }
+}
+
+int propagation(hipStream_t s, const cd* kx, const cd* ky, int N, int batch, const double* k0d, cd* X) {
+  if (batch <= 0) return 0;
+  propagation_kernel<<<grid_for((long long)batch * N), 256, 0, s>>>(kx, ky, N, batch, k0d, X);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace khx

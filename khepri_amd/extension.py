@@ -48,7 +48,10 @@ class ExtendedLayer:
         kp, wl = as_source_batch(k_parallel, wavelength, default_device())
         self.solve_batch(kp, wl)
 
-    def solve_batch(self, kp, wl, ws=None):
+    def chain_item(self):
+        return ("dense", self.S)
+
+    def solve_batch(self, kp, wl, ws=None, rt_only=False):
         if isinstance(self.base, Layer):
             self.base.fields = self.fields
         kpb, wlb, Nb = self._base_sources(kp, wl)

@@ -135,13 +135,32 @@ class Layer:
         kp, wl = as_source_batch(k_parallel, wavelength, dev)
         self.solve_batch(kp, wl)
 
-    def solve_batch(self, kp, wl, ws=None):
+    def chain_item(self):
+        return (self.skind, self.X if self.skind == "prop" else self.S)
+
+    def solve_batch(self, kp, wl, ws=None, rt_only=False):
+        """S-matrix for a batch.  With rt_only (R/T wanted, no fields), layers of
+        the gap medium are kept as their exact structure instead of a dense S:
+        a uniform eps = 1 layer as its propagation factors X ("prop"), an
+        eps = 1 half space as the identity S-matrix ("identity")."""
         dev = kp.device
         self.device = dev
         kx, ky = ops.kvectors(self.expansion.g_device(dev), kp, wl)
         k0 = 2 * np.pi / wl
         modes = bool(self.fields)
         self.info = None
+        self.skind, self.X = "dense", None
+        gap_medium = np.isscalar(self.epsilon) and complex(self.epsilon) == 1
+        if rt_only and not modes and gap_medium:
+            if self.formulation == Formulation.UNIFORM:
+                self.skind, self.S = "prop", None
+                self.X = ops.propagation(kx, ky, (k0 * self.depth).contiguous())
+                self.W = self.V = self.L = self.IC = None
+                return
+            if self.formulation in (Formulation.HALF_SPACE_INC, Formulation.HALF_SPACE_TRN):
+                self.skind, self.S = "identity", None
+                self.W = self.V = self.L = self.IC = None
+                return
         if self.formulation == Formulation.FFT:
             C, IC = self.convolution_matrices(dev)
             k0d = (k0 * self.depth).contiguous()

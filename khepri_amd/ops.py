@@ -91,19 +91,38 @@ def inverse(A):
     return solve(A, I)
 
 
+def _mat_view(t, name):
+    """(ld, batch stride) in complex elements of a (b, rows, cols) or (rows, cols) view."""
+    if t.stride(-1) != 1:
+        raise ValueError(f"{name}: last dimension must be contiguous")
+    if t.dim() == 2:
+        return t.stride(0), 0, 1
+    if t.dim() != 3:
+        raise ValueError(f"{name}: expected a (batch, rows, cols) view")
+    sb = t.stride(0) if t.shape[0] > 1 else 0
+    return t.stride(1), sb, t.shape[0]
+
+
 def matmul(A, B, alpha=1.0, beta=0.0, C=None):
+    """alpha A B + beta C for strided batches (views of S-matrix blocks welcome);
+    a 2-D operand is shared by the batch.  C, if given, is written in place."""
     lib = _lib.load()
-    ab = A.dim() == 2
-    bb = B.dim() == 2
     M, K = A.shape[-2:]
     N = B.shape[-1]
-    batch = max(1 if ab else A.shape[0], 1 if bb else B.shape[0])
+    lda, sa, ba = _mat_view(A, "A")
+    ldb, sb, bb = _mat_view(B, "B")
+    batch = max(ba, bb)
     if C is None:
-        C = torch.zeros((batch, M, N), dtype=CDT, device=A.device)
+        C = torch.empty((batch, M, N), dtype=CDT, device=A.device)
+        if complex(beta) != 0:
+            raise ValueError("beta != 0 needs C")
+    ldc, sc, bc = _mat_view(C, "C")
+    if bc != batch or (sc == 0 and batch > 1):
+        raise ValueError("C must hold one matrix per batch item")
     al = np.array([complex(alpha).real, complex(alpha).imag])
     be = np.array([complex(beta).real, complex(beta).imag])
-    check(lib.khx_zgemm(M, N, K, al.ctypes.data, _p(A.contiguous()), int(ab), _p(B.contiguous()), int(bb),
-                        be.ctypes.data, _p(C), batch, _stream()), "khx_zgemm")
+    check(lib.khx_zgemm(M, N, K, al.ctypes.data, _p(A), lda, sa, _p(B), ldb, sb, be.ctypes.data, _p(C), ldc, sc,
+                        batch, _stream()), "khx_zgemm")
     return C
 
 
@@ -186,6 +205,7 @@ def poynting(Stot, g, kp, wl, te, tm, epsi, epse, orders=False):
     B = Stot.shape[0]
     N = g.shape[1]
     dev = Stot.device
+    ws = _ws(lib.khx_poynting_ws_bytes(N, B), dev)
     R = torch.empty(B, dtype=RDT, device=dev)
     T = torch.empty(B, dtype=RDT, device=dev)
     Ro = To = None
@@ -195,8 +215,89 @@ def poynting(Stot, g, kp, wl, te, tm, epsi, epse, orders=False):
     ei = np.array([complex(epsi).real, complex(epsi).imag])
     ee = np.array([complex(epse).real, complex(epse).imag])
     check(lib.khx_poynting(_p(Stot), N, _p(g), _p(kp), _p(wl), _p(te), _p(tm), ei.ctypes.data, ee.ctypes.data, B,
-                           _p(R), _p(T), _p(Ro), _p(To), _stream()), "khx_poynting")
+                           _p(R), _p(T), _p(Ro), _p(To), _p(ws), _stream()), "khx_poynting")
     return R, T, Ro, To
+
+
+def incident(N, kp, wl, te, tm, epsi):
+    """Incident vectors c1p (B, n) on the device."""
+    lib = _lib.load()
+    B = kp.shape[0]
+    c = torch.empty((B, 2 * N), dtype=CDT, device=kp.device)
+    ei = np.array([complex(epsi).real, complex(epsi).imag])
+    check(lib.khx_incident(N, _p(kp), _p(wl), _p(te), _p(tm), ei.ctypes.data, B, _p(c), _stream()), "khx_incident")
+    return c
+
+
+def poynting_vec(sR, sT, g, kp, wl, epsi, epse, orders=False):
+    lib = _lib.load()
+    B, n = sR.shape
+    N = n // 2
+    dev = sR.device
+    R = torch.empty(B, dtype=RDT, device=dev)
+    T = torch.empty(B, dtype=RDT, device=dev)
+    Ro = To = None
+    if orders:
+        Ro = torch.empty((B, N), dtype=RDT, device=dev)
+        To = torch.empty((B, N), dtype=RDT, device=dev)
+    ei = np.array([complex(epsi).real, complex(epsi).imag])
+    ee = np.array([complex(epse).real, complex(epse).imag])
+    sR, sT = sR.contiguous(), sT.contiguous()
+    check(lib.khx_poynting_vec(_p(sR), _p(sT), N, _p(g), _p(kp), _p(wl), ei.ctypes.data, ee.ctypes.data, B, _p(R),
+                               _p(T), _p(Ro), _p(To), _stream()), "khx_poynting_vec")
+    return R, T, Ro, To
+
+
+def lu_factor(A):
+    """(LU, ipiv, info) of a batch (A untouched)."""
+    lib = _lib.load()
+    LU = A.clone().contiguous()
+    b, n, _ = LU.shape
+    piv = torch.empty((b, n), dtype=torch.int32, device=A.device)
+    info = torch.empty(b, dtype=torch.int32, device=A.device)
+    check(lib.khx_lu_factor(n, _p(LU), b, _p(piv), _p(info), _stream()), "khx_lu_factor")
+    return LU, piv, info
+
+
+def lu_solve(LU, piv, B):
+    lib = _lib.load()
+    X = B.clone().contiguous()
+    if X.dim() == 2:
+        X = X.unsqueeze(-1)
+    b, n, m = X.shape
+    check(lib.khx_lu_solve(n, m, _p(LU), _p(piv), _p(X), b, _stream()), "khx_lu_solve")
+    return X
+
+
+def diag_scale(A, dl=None, dr=None, out=None):
+    """diag(dl) A diag(dr) for a strided batch (A (b, rows, cols), dl (b, rows), dr (b, cols))."""
+    lib = _lib.load()
+    lda, sa, b = _mat_view(A, "A")
+    rows, cols = A.shape[-2:]
+    out = torch.empty((b, rows, cols), dtype=CDT, device=A.device) if out is None else out
+    ldo, so, _ = _mat_view(out, "out")
+    dl = None if dl is None else dl.contiguous()
+    dr = None if dr is None else dr.contiguous()
+    check(lib.khx_diag_scale(_p(A), lda, sa, _p(dl), _p(dr), rows, cols, b, _p(out), ldo, so, _stream()),
+          "khx_diag_scale")
+    return out
+
+
+def identity_minus(A, out=None):
+    lib = _lib.load()
+    A = A.contiguous()
+    b, n, _ = A.shape
+    out = torch.empty_like(A) if out is None else out
+    check(lib.khx_identity_minus(_p(A), n, b, _p(out), _stream()), "khx_identity_minus")
+    return out
+
+
+def propagation(kx, ky, k0d):
+    lib = _lib.load()
+    B, N = kx.shape
+    X = torch.empty((B, 2 * N), dtype=CDT, device=kx.device)
+    check(lib.khx_propagation(_p(kx), _p(ky), N, B, _p(k0d), _p(X), _stream()), "khx_propagation")
+    return X
 
 
 def joint_subspace(x, Nb, kind, blocks):

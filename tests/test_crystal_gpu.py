@@ -137,10 +137,65 @@ def test_chunked_batch_equals_resident(gpu, monkeypatch):
     cl = disc_crystal((5, 5), cfg["pixmap"])
     f = cfg["freqs"][:20]
     cl.set_source(1 / f, 1.0, 0.0)
-    cl.solve()
+    cl.solve(rt_only=False)
     R1, T1 = cl.poynting_flux_end()
     monkeypatch.setenv("KHX_CHUNK", "7")
-    cl.solve()
+    cl.solve(rt_only=False)
     R2, T2 = cl.poynting_flux_end()
     np.testing.assert_array_equal(R1, R2)
     np.testing.assert_array_equal(T1, T2)
+    cl.solve()   # streamed chunks default to the R/T-only chain
+    R3, T3 = cl.poynting_flux_end()
+    assert np.abs(R3 - R1).max() < 1e-11 and np.abs(T3 - T1).max() < 1e-11
+
+
+@pytest.mark.parametrize("which", ["disc_sweep", "disc_bz", "grad_grid"])
+def test_rt_only_chain_golden(gpu, which):
+    from khepri_amd import Crystal
+    if which == "disc_sweep":
+        g = golden("disc_sweep.npz")
+        cfg = W.disc_stack_sweep()
+        cl = disc_crystal((5, 5), cfg["pixmap"])
+        cl.set_source(1 / cfg["freqs"], 1.0, 0.0)
+    elif which == "disc_bz":
+        g = golden("disc_bz.npz")
+        cfg = W.disc_stack_bz()
+        cl = disc_crystal((9, 9), cfg["pixmap"])
+        cl.set_source(1 / cfg["freqs"][g["index"]], 1.0, 0.0, kp=cfg["kp"][g["index"]])
+    else:
+        g = golden("grad_grid.npz")
+        cfg = W.grad_grid()
+        cl = Crystal((11, 11))
+        names = []
+        for j, pix in enumerate(cfg["pixmaps"]):
+            cl.add_layer_pixmap(f"G{j}", pix, cfg["depth"])
+            names.append(f"G{j}")
+        cl.set_device(names, False)
+        cl.set_source(1 / cfg["freqs"][g["findex"]], 1.0, 0.0)
+    cl.solve(rt_only=True)
+    assert cl.S is None
+    R, T = cl.poynting_flux_end()
+    np.testing.assert_array_equal(np.isnan(T), np.isnan(g["T"]))
+    fin = np.isfinite(g["T"])
+    assert np.abs(R[fin] - g["R"][fin]).max() <= TOL and np.abs(T[fin] - g["T"][fin]).max() <= TOL
+
+
+def test_rt_only_random_devices_orders(gpu):
+    from khepri_amd import Crystal
+    g = golden("random_devices.npz")
+    devs = W.random_device_params()
+    i = int(g["orders_index"])
+    k = list(g["index"]).index(i)
+    dv = devs[i]
+    cl = Crystal(dv["pw"])
+    names = []
+    for j, ly in enumerate(dv["layers"]):
+        cl.add_layer_pixmap(f"L{j}", W.render_layer(ly), ly["depth"])
+        names.append(f"L{j}")
+    cl.set_device(names, False)
+    cl.set_source(1 / dv["freq"], 1.0, 1.0, dv["theta"], dv["phi"])
+    cl.solve(rt_only=True)
+    (r, ro), (t, to) = cl.poynting_flux_end(only_total=False)
+    assert abs(r - g["R"][k]) <= TOL and abs(t - g["T"][k]) <= TOL
+    np.testing.assert_allclose(ro, g["R_orders"], atol=1e-10)
+    np.testing.assert_allclose(to, g["T_orders"], atol=1e-10)

@@ -27,7 +27,7 @@ def test_zgemm(gpu, M, N, K, batch):
     ref = (1 - 0.5j) * A @ B + (0.25 + 1j) * C0
     np.testing.assert_allclose(out.cpu().numpy(), ref, atol=1e-12 * np.abs(ref).max())
     # broadcast B
-    out = ops.matmul(dev(A, gpu), dev(B[0], gpu))
+    out = ops.matmul(dev(A, gpu), dev(np.ascontiguousarray(B[0]), gpu))
     np.testing.assert_allclose(out.cpu().numpy(), A @ B[0], atol=1e-12 * np.abs(A @ B[0]).max())
 
 
@@ -180,3 +180,50 @@ def test_poynting_matches_oracle(gpu):
     assert abs(Rg - R.real) < 1e-13 and abs(Tg - T.real) < 1e-13
     np.testing.assert_allclose(Rog, Ro.real, atol=1e-14)
     np.testing.assert_allclose(Tog, To.real, atol=1e-14)
+
+
+@pytest.mark.parametrize("n,m,batch", [(1, 1, 2), (50, 1, 3), (162, 7, 2)])
+def test_lu_factor_solve(gpu, n, m, batch):
+    from khepri_amd import ops
+    rng = np.random.default_rng(n + 100 * m)
+    A, B = crand(rng, batch, n, n), crand(rng, batch, n, m)
+    lu, piv, info = ops.lu_factor(dev(A, gpu))
+    assert (info.cpu().numpy() == 0).all()
+    X = ops.lu_solve(lu, piv, dev(B, gpu)).cpu().numpy()
+    ref = np.linalg.solve(A, B)
+    np.testing.assert_allclose(X, ref, atol=1e-10 * np.abs(ref).max())
+    # the factors reproduce P A = L U
+    lu, piv = lu.cpu().numpy(), piv.cpu().numpy()
+    for b in range(batch):
+        L = np.tril(lu[b], -1) + np.eye(n)
+        U = np.triu(lu[b])
+        PA = A[b].copy()
+        for k in range(n):
+            PA[[k, piv[b, k]]] = PA[[piv[b, k], k]]
+        np.testing.assert_allclose(L @ U, PA, atol=1e-12 * np.abs(A[b]).max())
+
+
+def test_strided_views_and_elementwise(gpu):
+    from khepri_amd import ops
+    rng = np.random.default_rng(21)
+    S = crand(rng, 3, 2, 2, 9, 9)
+    St = dev(S, gpu)
+    out = ops.matmul(St[:, 0, 1], St[:, 1, 0])
+    np.testing.assert_allclose(out.cpu().numpy(), S[:, 0, 1] @ S[:, 1, 0], atol=1e-12)
+    dl, dr = crand(rng, 3, 9), crand(rng, 3, 9)
+    d = ops.diag_scale(St[:, 1, 1], dev(dl, gpu), dev(dr, gpu)).cpu().numpy()
+    np.testing.assert_allclose(d, dl[:, :, None] * S[:, 1, 1] * dr[:, None, :], atol=1e-13)
+    im = ops.identity_minus(dev(S[:, 0, 0].copy(), gpu)).cpu().numpy()
+    np.testing.assert_allclose(im, np.eye(9) - S[:, 0, 0], atol=0)
+
+
+def test_propagation_layer_is_exact_free_space(gpu):
+    """A uniform eps = 1 layer: S = [[0, X], [X, 0]] (the dense analytic S agrees)."""
+    from khepri_amd import ops
+    g, kx, ky = _layer_inputs((3, 3), (0.7, 0.2), 1.1)
+    k0d = dev(np.array([2 * np.pi / 1.1 * 0.9]), gpu)
+    X = ops.propagation(dev(kx[None], gpu), dev(ky[None], gpu), k0d).cpu().numpy()[0]
+    S, _ = ops.analytic_layer("uniform", dev(kx[None], gpu), dev(ky[None], gpu), 1.0, k0d)
+    S = S.cpu().numpy()[0]
+    np.testing.assert_allclose(S[0, 1], np.diag(X), atol=1e-14)
+    np.testing.assert_allclose(S[0, 0], 0, atol=1e-14)
