@@ -6,6 +6,8 @@
 #include "khx_internal.h"
 #include "khx_prof.h"
 
+#include <cstdlib>
+
 namespace khx {
 
 // ======================================================================== GEMM
@@ -13,7 +15,7 @@ namespace khx {
 // (BM/TM) x (BN/TN), each owning a TM x TN register tile.  Operand tiles are
 // staged in LDS, A transposed so a wave reads a few broadcast rows of A and
 // consecutive columns of B.
-template <int BM, int BN, int BK, int TM, int TN>
+template <int BM, int BN, int BK, int TM, int TN, bool TA>
 __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, cd alpha,
                                                     const cd* __restrict__ A, int lda, long long sA,
                                                     const cd* __restrict__ B, int ldb, long long sB,
@@ -43,12 +45,13 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, cd alph
     for (int j = 0; j < TN; ++j) acc[i][j] = mkc(0.0, 0.0);
 
   for (int k0 = 0; k0 < K; k0 += BK) {
-    // stage A tile (BM x BK) -> As[k][m]
+    // stage A tile (BM x BK) -> As[k][m]; op(A) = A^T reads A's rows along m
 #pragma unroll
     for (int e = tid; e < BM * BK; e += 256) {
-      const int r = e / BK, kk = e % BK;
+      const int r = TA ? e % BM : e / BK, kk = TA ? e / BM : e % BK;
       const int gr = row0 + r, gk = k0 + kk;
-      As[kk][r] = (gr < M && gk < K) ? a[(long long)gr * lda + gk] : mkc(0.0, 0.0);
+      const long long off = TA ? (long long)gk * lda + gr : (long long)gr * lda + gk;
+      As[kk][r] = (gr < M && gk < K) ? a[off] : mkc(0.0, 0.0);
     }
     // stage B tile (BK x BN)
 #pragma unroll
@@ -90,14 +93,14 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, cd alph
   }
 }
 
-template <int BM, int BN, int BK, int TM, int TN>
+template <int BM, int BN, int BK, int TM, int TN, bool TA>
 static int launch_zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C,
                         int batch) {
   const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
   const long long blocks = (long long)tilesM * tilesN * batch;
   KHX_REQUIRE(blocks < (1ll << 31), "zgemm: grid too large");
   ProfScope ps("zgemm_kernel", s);
-  zgemm_kernel<BM, BN, BK, TM, TN><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+  zgemm_kernel<BM, BN, BK, TM, TN, TA><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
       M, N, K, alpha, A.p, A.ld, A.stride, B.p, B.ld, B.stride, beta, C.p, C.ld, C.stride, tilesN,
       tilesM * tilesN);
   KHX_LAUNCH_CHECK();
@@ -106,8 +109,15 @@ static int launch_zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat
 
 int zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
   if (batch <= 0 || M <= 0 || N <= 0) return 0;
-  if (M <= 40 || N <= 40) return launch_zgemm<32, 32, 16, 2, 2>(s, M, N, K, alpha, A, B, beta, C, batch);
-  return launch_zgemm<64, 64, 16, 4, 4>(s, M, N, K, alpha, A, B, beta, C, batch);
+  if (M <= 40 || N <= 40) return launch_zgemm<32, 32, 16, 2, 2, false>(s, M, N, K, alpha, A, B, beta, C, batch);
+  return launch_zgemm<64, 64, 16, 4, 4, false>(s, M, N, K, alpha, A, B, beta, C, batch);
+}
+
+// C = alpha A^T B + beta C (A stored K x M)
+int zgemm_ta(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
+  if (batch <= 0 || M <= 0 || N <= 0) return 0;
+  if (M <= 40 || N <= 40) return launch_zgemm<32, 32, 16, 2, 2, true>(s, M, N, K, alpha, A, B, beta, C, batch);
+  return launch_zgemm<64, 64, 16, 4, 4, true>(s, M, N, K, alpha, A, B, beta, C, batch);
 }
 
 // ======================================================================== LU solve
@@ -244,15 +254,13 @@ __global__ __launch_bounds__(GESV_THREADS) void zgesv_kernel(int n, int m, cd* A
 
 int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info) {
   if (batch <= 0 || n <= 0) return 0;
+  if (n >= 24 && lu_blocked_fits(n) && getenv("KHX_LU_UNBLOCKED") == nullptr)
+    return zlu_blocked(s, n, m, A, B, batch, nullptr, info, false);
   KHX_REQUIRE(A.stride != 0 && B.stride != 0 || batch == 1, "zgesv: operands cannot be broadcast");
   const size_t lds = (size_t)(2 * n + m) * sizeof(cd);
-  KHX_REQUIRE(lds <= 160 * 1024, "zgesv: system too large for the LDS staging buffers");
-  static bool attr_set = false;
-  if (!attr_set && lds > 64 * 1024) {
-    KHX_CHECK_HIP(hipFuncSetAttribute((const void*)zgesv_kernel,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr_set = true;
-  }
+  KHX_REQUIRE(lds <= 150 * 1024, "zgesv: system too large for the LDS staging buffers");
+  static size_t lds_limit = 0;
+  KHX_TRY(ensure_dynamic_lds((const void*)zgesv_kernel, lds, lds_limit));
   ProfScope ps("zgesv_kernel", s);
   zgesv_kernel<<<dim3(batch), dim3(GESV_THREADS), lds, s>>>(n, m, A.p, A.ld, A.stride, B.p, B.ld,
                                                            B.stride, info);

@@ -98,6 +98,15 @@ int khx_eig(int n, double* A, int batch, double* lam, double* W, int* info, void
                    batch, info, ws);
 }
 
+int khx_eig_timed(int n, double* A, int batch, double* lam, double* W, int* info, void* ws,
+                  unsigned long long* tphase, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0 && tphase, "khx_eig_timed: bad arguments");
+  const long long nn = (long long)n * n;
+  return khx::zeig(as_stream(stream), n, khx::Mat{as_cd(A), n, nn}, as_cd(lam), n, khx::Mat{as_cd(W), n, nn},
+                   batch, info, ws, tphase);
+}
+
 size_t khx_patterned_layer_ws_bytes(int n, int batch) { return khx::patterned_layer_ws_bytes(n, batch); }
 
 int khx_patterned_layer(const double* C, const double* IC, const double* kx, const double* ky, int N, int batch,

@@ -1,29 +1,38 @@
 // Batched eigen-decomposition of general complex FP64 matrices (the P*Q
 // eigenproblem of a patterned layer, khepri/alternative.py:172).
 //
-// One workgroup per matrix, four stages:
-//   1. Householder reduction to upper Hessenberg form H = Z^H A Z (Z explicit);
+// One workgroup per matrix:
+//   1. Householder reduction to upper Hessenberg form H = Z^H A Z;
 //   2. implicit single-shift complex QR (Wilkinson shift, exceptional shifts
-//      every 10 sweeps, Ahues-Tisseur deflation test) to the Schur form
-//      T = Z^H A Z, accumulating Z;
-//   3. eigenvectors of the triangular T by back substitution (column-parallel
-//      over eigenvectors, row-sequential), tiny pivots replaced by
-//      max(ulp*|lambda|_1, smallnum) as LAPACK does;
-//   4. (separate launches) W = Z X, columns normalised to unit 2-norm with the
-//      largest component real -- the zgeev convention.
-// Each QR rotation touches disjoint row/column/Z element sets, so a rotation
-// is one barrier; the inputs of the next rotation travel through LDS.
+//      every 10 sweeps, Ahues-Tisseur deflation test) to the Schur form T;
+//   3. eigenvectors of the triangular T by back substitution, tiny pivots
+//      replaced by max(ulp*|lambda|_1, smallnum) as LAPACK ztrevc does;
+//   4. (separate launches) W = Z X, columns normalised to unit 2-norm with
+//      the largest component real -- the zgeev convention.
+//
+// Memory layout: Z is kept transposed (row k of Zt = Schur vector k) so that
+// both the reflector and the rotation updates of Z are contiguous rows.  H
+// lives in LDS when it fits (n <= KHX_EIG_LDS_N), in global memory otherwise.
+// In the QR phase a rotation updates only the active window [l, ihi]
+// immediately (one barrier per rotation; the inputs of the next rotation
+// travel through LDS); the sweep's rotations are recorded in LDS and applied
+// once per sweep to the rows above the window, the columns right of it and
+// to Z, each thread sliding along its own row/column.
 #include "khx_internal.h"
 #include "khx_prof.h"
 
+#include <cstdlib>
+
 namespace khx {
 
-constexpr int EIG_THREADS = 256;
-constexpr int EIG_WAVES = EIG_THREADS / 64;
+constexpr int EIG_THREADS_DEFAULT = 256;
 constexpr double ULP = 2.220446049250313e-16;
 constexpr double SAFMIN = 2.2250738585072014e-308;
+constexpr int KHX_EIG_LDS_N = 64;  // H resident in LDS up to this order
 
+template <int EIG_THREADS>
 __device__ __forceinline__ double block_sum(double v, double* red) {
+  constexpr int EIG_WAVES = EIG_THREADS / 64;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   v = wave_sum(v);
   if (lane == 0) red[wid] = v;
@@ -35,7 +44,9 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return t;
 }
 
+template <int EIG_THREADS>
 __device__ __forceinline__ int block_max_int(int v, int* red) {
+  constexpr int EIG_WAVES = EIG_THREADS / 64;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
@@ -49,7 +60,19 @@ __device__ __forceinline__ int block_max_int(int v, int* red) {
 }
 
 // Complex Givens rotation G = [[c, s], [-conj(s), c]] with G [x; y] = [r; 0].
+// Fast path: two independent reciprocal square roots instead of the chain of
+// hypot calls and divisions; squares stay far from over/underflow there.
 __device__ __forceinline__ void givens(cd x, cd y, double& c, cd& s, cd& r) {
+  const double ax2 = abs2c(x), ay2 = abs2c(y);
+  const double nr2 = ax2 + ay2;
+  if (ax2 > 1e-280 && nr2 < 1e280 && ay2 > 0.0) {
+    const double iax = rsqrt(ax2), inr = rsqrt(nr2);
+    const double ax = ax2 * iax, nr = nr2 * inr;
+    c = ax * inr;
+    s = (iax * inr) * (x * conjc(y));
+    r = (nr * iax) * x;
+    return;
+  }
   const double ax = absc(x), ay = absc(y);
   if (ay == 0.0) {
     c = 1.0;
@@ -72,15 +95,15 @@ __device__ __forceinline__ void givens(cd x, cd y, double& c, cd& s, cd& r) {
 
 // Negligible sub-diagonal H[k][k-1] (conventional test + Ahues & Tisseur).
 __device__ __forceinline__ bool negligible(const cd* H, int n, int k, double smlnum) {
-  const cd hkk1 = H[(long long)k * n + k - 1];
+  const cd hkk1 = H[k * n + k - 1];
   const double h = cabs1(hkk1);
   if (h <= smlnum) return true;
-  const cd a = H[(long long)(k - 1) * n + k - 1];
-  const cd d = H[(long long)k * n + k];
-  double tst = cabs1(a) + cabs1(d);
+  const cd a = H[(k - 1) * n + k - 1];
+  const cd d = H[k * n + k];
+  const double tst = cabs1(a) + cabs1(d);
   if (tst == 0.0) return false;
   if (h > ULP * tst) return false;
-  const double up = cabs1(H[(long long)(k - 1) * n + k]);
+  const double up = cabs1(H[(k - 1) * n + k]);
   const double ab = fmax(h, up), ba = fmin(h, up);
   const double dd = cabs1(a - d), ad = cabs1(d);
   const double aa = fmax(ad, dd), bb = fmin(ad, dd);
@@ -88,104 +111,167 @@ __device__ __forceinline__ bool negligible(const cd* H, int n, int k, double sml
   return ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)));
 }
 
-// Stages 1-3.  A (n x n) becomes T; Z gets the Schur vectors; X the
-// eigenvectors of T (upper triangular, unit diagonal); lam the eigenvalues.
-__global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, long long sA, cd* Zg,
-                                                                 cd* Xg, long long sW, cd* lam,
-                                                                 long long sL, int* info) {
+// column rotation (H G^H on columns k, k+1): (x, y) <- (c x + conj(s) y, c y - s x)
+__device__ __forceinline__ void rot_cols(cd& x, cd& y, double c, cd s) {
+  const cd nx = fmac(c * x, conjc(s), y);
+  y = fmsc(c * y, s, x);
+  x = nx;
+}
+// row rotation (G H on rows k, k+1): (x, y) <- (c x + s y, c y - conj(s) x)
+__device__ __forceinline__ void rot_rows(cd& x, cd& y, double c, cd s) {
+  const cd nx = fmac(c * x, s, y);
+  y = fmsc(c * y, conjc(s), x);
+  x = nx;
+}
+
+// Apply the recorded rotations k = l..ihi-1 to the line v[l*step], ...,
+// v[ihi*step] (a row of H above the window: step 1, column rotations; a
+// column right of it or of Zt: step n).  Loads go out eight at a time so the
+// sequential chain does not wait on each one.
+template <bool ROWS>
+__device__ __forceinline__ void slide(cd* v, int step, int l, int ihi, const double* crot, const cd* srot) {
+  cd xv = v[l * step];
+  int k = l;
+  for (; k + 8 <= ihi; k += 8) {
+    cd yv[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) yv[m] = v[(k + 1 + m) * step];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      if (ROWS)
+        rot_rows(xv, yv[m], crot[k + m], srot[k + m]);
+      else
+        rot_cols(xv, yv[m], crot[k + m], srot[k + m]);
+      v[(k + m) * step] = xv;
+      xv = yv[m];
+    }
+  }
+  for (; k < ihi; ++k) {
+    cd yv = v[(k + 1) * step];
+    if (ROWS)
+      rot_rows(xv, yv, crot[k], srot[k]);
+    else
+      rot_cols(xv, yv, crot[k], srot[k]);
+    v[k * step] = xv;
+    xv = yv;
+  }
+  v[ihi * step] = xv;
+}
+
+// Stages 1-3.  A (n x n) becomes T (upper triangle); Zt gets the transposed
+// Schur vectors; X the eigenvectors of T (upper triangular, unit diagonal);
+// lam the eigenvalues.
+template <int EIG_THREADS>
+__global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, long long sA, cd* Ztg, cd* Xg,
+                                                                 long long sW, cd* lam, long long sL, int* info,
+                                                                 unsigned long long* tphase) {
+  constexpr int EIG_WAVES = EIG_THREADS / 64;
   extern __shared__ cd sm[];
-  cd* u = sm;            // n   Householder vector
-  cd* wv = u + n;        // n   u^H H  (left product)
-  cd* vh = wv + n;       // n   H u    (right product)
-  cd* vz = vh + n;       // n   Z u
-  cd* part = vz + n;     // EIG_WAVES * n partial sums
+  cd* u = sm;                         // n     Householder vector / solved row
+  cd* wv = u + n;                     // 2n    u^H H (columns of H) and u^T Zt
+  cd* vh = wv + 2 * n;                // n     H u
+  cd* part = vh + n;                  // waves*2n  partial sums
+  cd* srot = part + EIG_WAVES * 2 * n;  // n     rotation sines of the current sweep
+  double* crot = reinterpret_cast<double*>(srot + n);  // n cosines
+  cd* hl = srot + n + (n + 1) / 2;                      // n*n LDS copy of H (small n)
   __shared__ double red[EIG_WAVES];
   __shared__ int redi[EIG_WAVES];
   __shared__ cd s_xy[2][2];  // double-buffered inputs of the next rotation
 
   const long long item = blockIdx.x;
-  cd* H = A + item * sA;
-  cd* Z = Zg + item * sW;
+  cd* Hg = A + item * sA;
+  cd* Zt = Ztg + item * sW;
   cd* X = Xg + item * sW;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const double smlnum = SAFMIN * ((double)n / ULP);
+  const bool in_lds = n <= KHX_EIG_LDS_N;
+  cd* H = in_lds ? hl : Hg;
+  // optional phase timestamps (wall clock): start, Hessenberg, Schur, eigenvectors, sweeps
+  unsigned long long* tp = tphase ? tphase + item * 5 : nullptr;
+  if (tp && tid == 0) tp[0] = wall_clock64();
 
-  // Z = I
   for (int i = wid; i < n; i += EIG_WAVES)
-    for (int j = lane; j < n; j += 64) Z[(long long)i * n + j] = mkc(i == j ? 1.0 : 0.0, 0.0);
+    for (int j = lane; j < n; j += 64) {
+      Zt[i * n + j] = mkc(i == j ? 1.0 : 0.0, 0.0);
+      if (in_lds) hl[i * n + j] = Hg[(long long)i * n + j];
+    }
   __syncthreads();
 
   // ------------------------------------------------------------ 1. Hessenberg
   for (int k = 0; k + 2 < n; ++k) {
-    const int L = n - k - 1;  // reflector length (rows k+1..n-1)
+    const int L = n - k - 1;  // reflector acts on rows/cols k+1..n-1
     double part2 = 0.0;
-    for (int i = k + 2 + tid; i < n; i += EIG_THREADS) part2 += abs2c(H[(long long)i * n + k]);
-    const double rest2 = block_sum(part2, red);
+    for (int i = k + 2 + tid; i < n; i += EIG_THREADS) part2 += abs2c(H[i * n + k]);
+    const double rest2 = block_sum<EIG_THREADS>(part2, red);
     if (rest2 == 0.0) continue;  // uniform: every thread sees the same sum
-    const cd alpha = H[(long long)(k + 1) * n + k];
+    const cd alpha = H[(k + 1) * n + k];
     const double aa = absc(alpha);
     const double xn = sqrt(aa * aa + rest2);
     const cd ph = aa != 0.0 ? (1.0 / aa) * alpha : mkc(1.0, 0.0);
     const double sc = 1.0 / sqrt(xn * (xn + aa));
     for (int i = tid; i < L; i += EIG_THREADS) {
-      cd x = H[(long long)(k + 1 + i) * n + k];
+      cd x = H[(k + 1 + i) * n + k];
       if (i == 0) x = x + xn * ph;
       u[i] = sc * x;
     }
     __syncthreads();
-    // left: w_j = sum_i conj(u_i) H[k+1+i][j], j > k
-    for (int j = k + 1 + lane; j < n; j += 64) {
+    // partial sums: H columns j > k (conj(u)^T H) and Zt columns (u^T Zt)
+    for (int j = lane; j < 2 * n; j += 64) {
+      const bool hz = j < n;
+      const int col = hz ? j : j - n;
+      if (hz && col <= k) continue;
       cd acc = mkc(0.0, 0.0);
-      for (int i = wid; i < L; i += EIG_WAVES) acc = fmac_cj(acc, u[i], H[(long long)(k + 1 + i) * n + j]);
-      part[wid * n + j] = acc;
+      if (hz) {
+#pragma unroll 8
+        for (int i = wid; i < L; i += EIG_WAVES) acc = fmac_cj(acc, u[i], H[(k + 1 + i) * n + col]);
+      } else {
+#pragma unroll 8
+        for (int i = wid; i < L; i += EIG_WAVES) acc = fmac(acc, u[i], Zt[(k + 1 + i) * n + col]);
+      }
+      part[wid * 2 * n + j] = acc;
     }
     __syncthreads();
-    for (int j = k + 1 + tid; j < n; j += EIG_THREADS) {
+    for (int j = tid; j < 2 * n; j += EIG_THREADS) {
+      if (j < n && j <= k) continue;
       cd acc = part[j];
 #pragma unroll
-      for (int w = 1; w < EIG_WAVES; ++w) acc = acc + part[w * n + j];
+      for (int w = 1; w < EIG_WAVES; ++w) acc = acc + part[w * 2 * n + j];
       wv[j] = acc;
     }
     __syncthreads();
+    // H := P H (rows k+1..), Zt := conj(P) Zt (Z := Z P)
     for (int i = wid; i < L; i += EIG_WAVES) {
-      const cd ui = u[i];
-      cd* row = H + (long long)(k + 1 + i) * n;
+      const cd ui = u[i], cui = conjc(u[i]);
+      cd* row = H + (k + 1 + i) * n;
       for (int j = k + 1 + lane; j < n; j += 64) row[j] = fmsc(row[j], ui, wv[j]);
       if (lane == 0) row[k] = (i == 0) ? -(xn * ph) : mkc(0.0, 0.0);
+      cd* zr = Zt + (k + 1 + i) * n;
+      for (int j = lane; j < n; j += 64) zr[j] = fmsc(zr[j], cui, wv[n + j]);
     }
     __syncthreads();
-    // right: v_i = sum_j H[i][k+1+j] u_j, same for Z
-    for (int i = wid; i < n; i += EIG_WAVES) {
-      const cd* hr = H + (long long)i * n + k + 1;
-      const cd* zr = Z + (long long)i * n + k + 1;
-      cd ah = mkc(0.0, 0.0), az = mkc(0.0, 0.0);
-      for (int j = lane; j < L; j += 64) {
-        ah = fmac(ah, hr[j], u[j]);
-        az = fmac(az, zr[j], u[j]);
+    // H := H P (columns k+1..): v_i = sum_j H[i][k+1+j] u_j, one thread per row
+    for (int i = tid; i < n; i += EIG_THREADS) {
+      const cd* hr = H + i * n + k + 1;
+      cd a0 = mkc(0.0, 0.0), a1 = mkc(0.0, 0.0);
+      int j = 0;
+#pragma unroll 4
+      for (; j + 1 < L; j += 2) {
+        a0 = fmac(a0, hr[j], u[j]);
+        a1 = fmac(a1, hr[j + 1], u[j + 1]);
       }
-      ah.x = wave_sum(ah.x);
-      ah.y = wave_sum(ah.y);
-      az.x = wave_sum(az.x);
-      az.y = wave_sum(az.y);
-      if (lane == 0) {
-        vh[i] = ah;
-        vz[i] = az;
-      }
+      if (j < L) a0 = fmac(a0, hr[j], u[j]);
+      vh[i] = a0 + a1;
     }
     __syncthreads();
     for (int i = wid; i < n; i += EIG_WAVES) {
-      cd* hr = H + (long long)i * n + k + 1;
-      cd* zr = Z + (long long)i * n + k + 1;
-      const cd a = vh[i], b = vz[i];
-      for (int j = lane; j < L; j += 64) {
-        const cd cu = conjc(u[j]);
-        hr[j] = fmsc(hr[j], a, cu);
-        zr[j] = fmsc(zr[j], b, cu);
-      }
+      cd* hr = H + i * n + k + 1;
+      const cd a = vh[i];
+      for (int j = lane; j < L; j += 64) hr[j] = fmsc(hr[j], a, conjc(u[j]));
     }
     __syncthreads();
   }
 
+  if (tp && tid == 0) tp[1] = wall_clock64();
   // ------------------------------------------------------------ 2. Schur form
   int ihi = n - 1, its = 0, itot = 0, fail = 0;
   const int itmax = 30 * max(10, n);
@@ -193,11 +279,11 @@ __global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, l
     int lk = 0;
     for (int k = 1 + tid; k <= ihi; k += EIG_THREADS) {
       if (negligible(H, n, k, smlnum)) {
-        H[(long long)k * n + k - 1] = mkc(0.0, 0.0);
+        H[k * n + k - 1] = mkc(0.0, 0.0);
         lk = max(lk, k);
       }
     }
-    const int l = block_max_int(lk, redi);
+    const int l = block_max_int<EIG_THREADS>(lk, redi);
     if (l == ihi) {
       --ihi;
       its = 0;
@@ -208,15 +294,14 @@ __global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, l
       break;
     }
     ++its;
-    // shift
     cd shift;
     if (its % 20 == 10) {  // exceptional shifts break cycles
-      shift = H[(long long)l * n + l] + mkc(0.75 * fabs(H[(long long)(l + 1) * n + l].x), 0.0);
+      shift = H[l * n + l] + mkc(0.75 * fabs(H[(l + 1) * n + l].x), 0.0);
     } else if (its % 20 == 0) {
-      shift = H[(long long)ihi * n + ihi] + mkc(0.75 * fabs(H[(long long)ihi * n + ihi - 1].x), 0.0);
-    } else {
-      const cd a = H[(long long)(ihi - 1) * n + ihi - 1], b = H[(long long)(ihi - 1) * n + ihi];
-      const cd c = H[(long long)ihi * n + ihi - 1], d = H[(long long)ihi * n + ihi];
+      shift = H[ihi * n + ihi] + mkc(0.75 * fabs(H[ihi * n + ihi - 1].x), 0.0);
+    } else {  // Wilkinson: eigenvalue of the trailing 2x2 closest to H[ihi][ihi]
+      const cd a = H[(ihi - 1) * n + ihi - 1], b = H[(ihi - 1) * n + ihi];
+      const cd c = H[ihi * n + ihi - 1], d = H[ihi * n + ihi];
       const cd bc = b * c;
       if (bc.x == 0.0 && bc.y == 0.0) {
         shift = d;
@@ -228,8 +313,9 @@ __global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, l
         shift = (den.x == 0.0 && den.y == 0.0) ? d : d - divc(bc, den);
       }
     }
-    cd x = H[(long long)l * n + l] - shift;
-    cd y = H[(long long)(l + 1) * n + l];
+    cd x = H[l * n + l] - shift;
+    cd y = H[(l + 1) * n + l];
+    // bulge chase restricted to the window [l, ihi]
     for (int k = l; k < ihi; ++k) {
       cd* nxt = s_xy[(k + 1) & 1];
       if (k > l) {
@@ -239,77 +325,85 @@ __global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, l
       double c;
       cd s, r;
       givens(x, y, c, s, r);
-      const cd sc = conjc(s);
-      const int nrow = n - k - 2;                        // row part: columns k+2..n-1
-      const int ncol = k + ((k + 2 <= ihi) ? 1 : 0);     // col part: rows 0..k-1 (+ row k+2)
-      const int total = 1 + nrow + ncol + n;
+      if (tid == 0) {
+        crot[k] = c;
+        srot[k] = s;
+      }
+      const int nrow = ihi - k - 1;                         // row part: columns k+2..ihi
+      const int ncol = (k - l) + ((k + 2 <= ihi) ? 1 : 0);  // col part: rows l..k-1 (+ row k+2)
+      const int total = 1 + nrow + ncol;
       for (int e = tid; e < total; e += EIG_THREADS) {
         if (e == 0) {
-          // 2x2 diagonal block G B G^H, and the bulge column k-1
           if (k > l) {
-            H[(long long)k * n + k - 1] = r;
-            H[(long long)(k + 1) * n + k - 1] = mkc(0.0, 0.0);
+            H[k * n + k - 1] = r;
+            H[(k + 1) * n + k - 1] = mkc(0.0, 0.0);
           }
-          cd* p00 = H + (long long)k * n + k;
-          cd* p10 = H + (long long)(k + 1) * n + k;
-          const cd b00 = p00[0], b01 = p00[1], b10 = p10[0], b11 = p10[1];
-          // rows: G B
-          const cd r00 = fmac(c * b00, s, b10), r01 = fmac(c * b01, s, b11);
-          const cd r10 = fmsc(c * b10, sc, b00), r11 = fmsc(c * b11, sc, b01);
-          // columns: (G B) G^H
-          const cd n00 = fmac(c * r00, sc, r01), n01 = fmsc(c * r01, s, r00);
-          const cd n10 = fmac(c * r10, sc, r11), n11 = fmsc(c * r11, s, r10);
-          p00[0] = n00;
-          p00[1] = n01;
-          p10[0] = n10;
-          p10[1] = n11;
-          nxt[0] = n10;
+          cd* p0 = H + k * n + k;
+          cd* p1 = H + (k + 1) * n + k;
+          cd b00 = p0[0], b01 = p0[1], b10 = p1[0], b11 = p1[1];
+          rot_rows(b00, b10, c, s);
+          rot_rows(b01, b11, c, s);
+          rot_cols(b00, b01, c, s);
+          rot_cols(b10, b11, c, s);
+          p0[0] = b00;
+          p0[1] = b01;
+          p1[0] = b10;
+          p1[1] = b11;
+          nxt[0] = b10;
         } else if (e <= nrow) {
-          const int j = k + 1 + e;  // k+2 ..
-          cd* pk = H + (long long)k * n + j;
-          cd* pk1 = H + (long long)(k + 1) * n + j;
-          const cd hk = *pk, hk1 = *pk1;
-          *pk = fmac(c * hk, s, hk1);
-          *pk1 = fmsc(c * hk1, sc, hk);
-        } else if (e <= nrow + ncol) {
-          const int q = e - nrow - 1;
-          const int i = (q < k) ? q : k + 2;
-          cd* pr = H + (long long)i * n + k;
-          const cd hk = pr[0], hk1 = pr[1];
-          const cd nk = fmac(c * hk, sc, hk1);
-          pr[0] = nk;
-          pr[1] = fmsc(c * hk1, s, hk);
-          if (i == k + 2) nxt[1] = nk;
+          const int j = k + 1 + e;  // k+2 .. ihi
+          cd xk = H[k * n + j], yk = H[(k + 1) * n + j];
+          rot_rows(xk, yk, c, s);
+          H[k * n + j] = xk;
+          H[(k + 1) * n + j] = yk;
         } else {
-          const int i = e - nrow - ncol - 1;
-          cd* pr = Z + (long long)i * n + k;
-          const cd zk = pr[0], zk1 = pr[1];
-          pr[0] = fmac(c * zk, sc, zk1);
-          pr[1] = fmsc(c * zk1, s, zk);
+          const int q = e - nrow - 1;
+          const int i = (q < k - l) ? l + q : k + 2;
+          cd* pr = H + i * n + k;
+          cd xk = pr[0], yk = pr[1];
+          rot_cols(xk, yk, c, s);
+          pr[0] = xk;
+          pr[1] = yk;
+          if (i == k + 2) nxt[1] = xk;
         }
       }
       __syncthreads();
     }
+    // deferred: rows above the window, columns right of it, and Z
+    const int nabove = l, nright = n - 1 - ihi;
+    const int tot = nabove + nright + n;
+    for (int e = tid; e < tot; e += EIG_THREADS) {
+      if (e < nabove)  // row e of H, columns l..ihi
+        slide<false>(H + e * n, 1, l, ihi, crot, srot);
+      else if (e < nabove + nright)  // column j of H, rows l..ihi
+        slide<true>(H + ihi + 1 + (e - nabove), n, l, ihi, crot, srot);
+      else  // column i of Zt (row i of Z), rows l..ihi
+        slide<false>(Zt + (e - nabove - nright), n, l, ihi, crot, srot);
+    }
+    __syncthreads();
   }
   if (tid == 0 && info) info[item] = fail;
+  if (tp && tid == 0) {
+    tp[2] = wall_clock64();
+    tp[4] = (unsigned long long)itot;
+  }
 
-  // eigenvalues
   cd* lm = lam + item * sL;
-  for (int i = tid; i < n; i += EIG_THREADS) lm[i] = H[(long long)i * n + i];
+  for (int i = tid; i < n; i += EIG_THREADS) lm[i] = H[i * n + i];
 
   // ------------------------------------------------------------ 3. eigenvectors of T
   // X[i][k] = -T[i][k] (i < k), 1 (i == k), 0 (i > k)
   for (int i = wid; i < n; i += EIG_WAVES)
     for (int k = lane; k < n; k += 64) {
-      const cd t = H[(long long)i * n + k];
+      const cd t = H[i * n + k];
       X[(long long)i * n + k] = (k > i) ? -t : mkc(k == i ? 1.0 : 0.0, 0.0);
     }
   __syncthreads();
-  cd* xr = u;  // reuse: current row of solved components
+  cd* xr = u;
   for (int j = n - 1; j >= 0; --j) {
-    const cd tjj = H[(long long)j * n + j];
+    const cd tjj = H[j * n + j];
     for (int k = j + 1 + tid; k < n; k += EIG_THREADS) {
-      const cd lk = H[(long long)k * n + k];
+      const cd lk = H[k * n + k];
       cd d = tjj - lk;
       const double smin = fmax(ULP * cabs1(lk), smlnum);
       if (cabs1(d) < smin) d = mkc(smin, 0.0);
@@ -319,12 +413,16 @@ __global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, l
     }
     __syncthreads();
     for (int i = wid; i < j; i += EIG_WAVES) {
-      const cd tij = H[(long long)i * n + j];
+      const cd tij = H[i * n + j];
       cd* row = X + (long long)i * n;
       for (int k = j + 1 + lane; k < n; k += 64) row[k] = fmsc(row[k], tij, xr[k]);
     }
     __syncthreads();
   }
+  if (in_lds)  // the Schur form goes back to A (callers may inspect it)
+    for (int i = wid; i < n; i += EIG_WAVES)
+      for (int j = lane; j < n; j += 64) Hg[(long long)i * n + j] = hl[i * n + j];
+  if (tp && tid == 0) tp[3] = wall_clock64();
 }
 
 // Column normalisation of W: unit 2-norm, largest |component| made real.
@@ -353,28 +451,48 @@ __global__ __launch_bounds__(256) void znormalize_cols_kernel(int n, cd* W, long
 }
 
 size_t zeig_workspace_bytes(int n, int batch) {
-  return 2ull * (size_t)n * n * sizeof(cd) * (size_t)batch;  // Z and X
+  return 2ull * (size_t)n * n * sizeof(cd) * (size_t)batch;  // Zt and X
 }
 
-int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws) {
+static size_t eig_lds_bytes(int n, int threads) {
+  size_t b = (size_t)(1 + 2 + 1 + 2 * (threads / 64) + 1) * n * sizeof(cd) + (size_t)((n + 1) / 2) * sizeof(cd);
+  if (n <= KHX_EIG_LDS_N) b += (size_t)n * n * sizeof(cd);
+  return b;
+}
+
+int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws,
+         unsigned long long* tphase) {
   if (batch <= 0) return 0;
-  KHX_REQUIRE(n >= 1, "zeig: n must be positive");
+  KHX_REQUIRE(n >= 1 && n <= 8192, "zeig: bad order");
   KHX_REQUIRE(A.ld == n && W.ld == n && A.stride != 0 && W.stride != 0,
               "zeig: matrices must be dense (ld == n) and not broadcast");
-  cd* Z = static_cast<cd*>(ws);
-  cd* X = Z + (size_t)n * n * batch;
+  cd* Zt = static_cast<cd*>(ws);
+  cd* X = Zt + (size_t)n * n * batch;
   const long long sZ = (long long)n * n;
-  const size_t lds = (size_t)(4 + EIG_WAVES) * n * sizeof(cd);
-  KHX_REQUIRE(lds <= 64 * 1024, "zeig: n too large for the LDS vectors");
-  // Z and X use the W stride layout (n*n per item) in the workspace
+  const char* env = getenv("KHX_EIG_THREADS");
+  const int threads = env ? atoi(env) : EIG_THREADS_DEFAULT;
+  KHX_REQUIRE(threads == 64 || threads == 128 || threads == 256, "zeig: KHX_EIG_THREADS must be 64, 128 or 256");
+  const size_t lds = eig_lds_bytes(n, threads);
+  KHX_REQUIRE(lds <= 150 * 1024, "zeig: n too large for the LDS work vectors");
+  static size_t lds_limit[3] = {0, 0, 0};
   {
-  ProfScope ps("zeig_schur_kernel", s);
-  zeig_schur_kernel<<<dim3(batch), dim3(EIG_THREADS), lds, s>>>(n, A.p, A.stride, Z, X, sZ, lam,
-                                                               lstride, info);
-  KHX_LAUNCH_CHECK();
+    ProfScope ps("zeig_schur_kernel", s);
+    if (threads == 64) {
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<64>, lds, lds_limit[0]));
+      zeig_schur_kernel<64><<<dim3(batch), dim3(64), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase);
+    } else if (threads == 128) {
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<128>, lds, lds_limit[1]));
+      zeig_schur_kernel<128><<<dim3(batch), dim3(128), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
+                                                                 tphase);
+    } else {
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<256>, lds, lds_limit[2]));
+      zeig_schur_kernel<256><<<dim3(batch), dim3(256), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
+                                                                 tphase);
+    }
+    KHX_LAUNCH_CHECK();
   }
-  // W = Z X
-  KHX_TRY(zgemm(s, n, n, n, mkc(1.0, 0.0), Mat{Z, n, sZ}, Mat{X, n, sZ}, mkc(0.0, 0.0), W, batch));
+  // W = Z X = Zt^T X
+  KHX_TRY(zgemm_ta(s, n, n, n, mkc(1.0, 0.0), Mat{Zt, n, sZ}, Mat{X, n, sZ}, mkc(0.0, 0.0), W, batch));
   znormalize_cols_kernel<<<dim3(batch), dim3(256), 0, s>>>(n, W.p, W.stride);
   KHX_LAUNCH_CHECK();
   return 0;

@@ -16,8 +16,14 @@ inline Mat sub(Mat m, int r, int c) { return Mat{m.p + (long long)r * m.ld + c, 
 
 // C = alpha * A (M x K) * B (K x N) + beta * C
 int zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch);
+// C = alpha A^T (A stored K x M) * B + beta C
+int zgemm_ta(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch);
 // Solve A X = B (A n x n destroyed, B n x m overwritten); info[b] = first zero pivot (1-based) or 0
 int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info);
+
+// blocked LU (one workgroup per system, panel in LDS): keepL -> zgetrf layout
+bool lu_blocked_fits(int n);
+int zlu_blocked(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL);
 
 // elementwise helpers (all batched, n_rows x n_cols views)
 int zset_identity(hipStream_t s, Mat A, int rows, int cols, cd diag, int batch);   // A = diag * I
@@ -29,6 +35,20 @@ int zrow_scale(hipStream_t s, Mat D, const cd* d, long long dstride, Mat A, int 
 
 // eigen-decomposition: A destroyed; lam (n per item), W (n x n, unit columns)
 size_t zeig_workspace_bytes(int n, int batch);
-int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws);
+int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws,
+         unsigned long long* tphase = nullptr);
+
+// Raise a kernel's dynamic-LDS limit to `bytes` when it exceeds the 64 KiB
+// default (`current` caches the value already set for that kernel).
+inline int ensure_dynamic_lds(const void* fn, size_t bytes, size_t& current) {
+  if (bytes <= 64 * 1024 || bytes <= current) return 0;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) {
+    set_error(std::string("hipFuncSetAttribute(MaxDynamicSharedMemorySize): ") + hipGetErrorString(e));
+    return -3;
+  }
+  current = bytes;
+  return 0;
+}
 
 }  // namespace khx

@@ -8,6 +8,8 @@
 #include "khx_internal.h"
 #include "khx_prof.h"
 
+#include <cstdlib>
+
 namespace khx {
 
 constexpr int LU_THREADS = 256;
@@ -156,6 +158,8 @@ __global__ __launch_bounds__(LU_THREADS) void zgetrs_kernel(int n, int m, const 
 
 int zgetrf(hipStream_t s, int n, Mat A, int batch, int* ipiv, int* info) {
   if (batch <= 0 || n <= 0) return 0;
+  if (n >= 24 && lu_blocked_fits(n) && getenv("KHX_LU_UNBLOCKED") == nullptr)
+    return zlu_blocked(s, n, 0, A, Mat{nullptr, 0, 0}, batch, ipiv, info, true);
   const size_t lds = (size_t)2 * n * sizeof(cd);
   KHX_REQUIRE(lds <= 64 * 1024, "zgetrf: n too large for the LDS staging buffers");
   ProfScope ps("zgetrf_kernel", s);

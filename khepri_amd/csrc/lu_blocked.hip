@@ -1,0 +1,319 @@
+// Blocked LU solve / factorisation, one workgroup per system.
+//
+// Right-looking LU with partial pivoting in panels of NB columns:
+//   1. the panel A[j0:n, j0:j0+NB] is factorised in LDS (pivot search, row
+//      interchange, scaling and rank-1 updates never leave LDS);
+//   2. its interchanges are applied to the other columns (and to B);
+//   3. U12 = L11^-1 A12 and Y1 = L11^-1 B1 (one thread per column);
+//   4. the trailing block [A22 | B2] -= L21 [U12 | Y1] as an LDS-staged
+//      mini-GEMM (L21 from the panel, [U12 | Y1] in 64-column tiles, 4x4
+//      complex register tiles per thread).
+// With right-hand sides, the back substitution U X = Y is blocked the same
+// way (NB x NB diagonal solves, then mini-GEMM updates of the rows above).
+// Every element of the trailing matrix is read and written once per panel
+// instead of once per column.
+#include "khx_internal.h"
+#include "khx_prof.h"
+
+namespace khx {
+
+constexpr int LB_THREADS = 256;
+constexpr int LB_NB = 16;  // panel width
+constexpr int LB_TW = 64;  // column tile of the mini-GEMM
+
+// C[r][c] -= sum_k L[r - r0][k] * U[k][c - cbase] for rows r in [r0, r1) and
+// tile columns [0, tw); L rows live in LDS with stride LB_NB, the U tile in
+// LDS with stride LB_TW.  colptr(t) gives the global address of (row 0,
+// tile column t); rows are ld apart.
+template <typename ColPtr>
+__device__ __forceinline__ void mini_gemm_sub(const cd* Ls, int lrow0, const cd* Us, int nb, int r0, int r1, int tw,
+                                              ColPtr colptr, long long ld) {
+  const int tid = threadIdx.x;
+  const int cg = tid & 15;  // 16 column groups of 4
+  const int rg = tid >> 4;  // 16 row groups of 4
+  const int cbase = cg * 4;
+  for (int rb = r0 + rg * 4; rb < r1; rb += 64) {
+    cd acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = mkc(0.0, 0.0);
+    for (int kk = 0; kk < nb; ++kk) {
+      cd lv[4], uv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) lv[i] = (rb + i < r1) ? Ls[(rb + i - lrow0) * LB_NB + kk] : mkc(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) uv[j] = Us[kk * LB_TW + cbase + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmac(acc[i][j], lv[i], uv[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (cbase + j >= tw) continue;
+      cd* col = colptr(cbase + j);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (rb + i >= r1) continue;
+        cd* p = col + (long long)(rb + i) * ld;
+        *p = *p - acc[i][j];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void argmax_abs1(double& best, int& bi, cd& bv, double* rv, int* ri, cd* rc) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(best, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    const double ox = __shfl_xor(bv.x, o, 64), oy = __shfl_xor(bv.y, o, 64);
+    if (ov > best || (ov == best && oi < bi)) {
+      best = ov;
+      bi = oi;
+      bv = mkc(ox, oy);
+    }
+  }
+  if (lane == 0) {
+    rv[wid] = best;
+    ri[wid] = bi;
+    rc[wid] = bv;
+  }
+  __syncthreads();
+  best = rv[0];
+  bi = ri[0];
+  bv = rc[0];
+#pragma unroll
+  for (int w = 1; w < LB_THREADS / 64; ++w)
+    if (rv[w] > best || (rv[w] == best && ri[w] < bi)) {
+      best = rv[w];
+      bi = ri[w];
+      bv = rc[w];
+    }
+}
+
+// keepL: LAPACK zgetrf layout (interchanges applied to the L part too, ipiv
+// stored).  m > 0: B (n x m) is transformed along and finally solved.
+__global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, cd* A, int lda, long long sA, cd* B,
+                                                                 int ldb, long long sB, int* ipiv, int* info,
+                                                                 int keepL) {
+  extern __shared__ cd smem[];
+  cd* pan = smem;                // n x LB_NB
+  cd* ut = pan + n * LB_NB;      // LB_NB x LB_TW
+  __shared__ double rv[LB_THREADS / 64];
+  __shared__ int ri[LB_THREADS / 64];
+  __shared__ cd rc[LB_THREADS / 64];
+  __shared__ int perm[LB_NB];
+  const long long item = blockIdx.x;
+  cd* a = A + item * sA;
+  cd* bm = m > 0 ? B + item * sB : nullptr;
+  const int tid = threadIdx.x;
+  int first_zero = 0;
+
+  for (int j0 = 0; j0 < n; j0 += LB_NB) {
+    const int nb = min(LB_NB, n - j0);
+    const int pr = n - j0;
+    for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
+      const int r = e / LB_NB, c = e % LB_NB;
+      pan[e] = (c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
+    }
+    __syncthreads();
+    // ---- factor the panel in LDS
+    for (int jj = 0; jj < nb; ++jj) {
+      double best = -1.0;
+      int bi = pr;
+      cd bv = mkc(0.0, 0.0);
+      for (int r = jj + tid; r < pr; r += LB_THREADS) {
+        const cd v = pan[r * LB_NB + jj];
+        const double av = cabs1(v);
+        if (av > best) {
+          best = av;
+          bi = r;
+          bv = v;
+        }
+      }
+      argmax_abs1(best, bi, bv, rv, ri, rc);
+      int p = bi;
+      cd piv = bv;
+      if (p >= pr) {  // NaN column: keep the diagonal
+        p = jj;
+        piv = pan[jj * LB_NB + jj];
+      }
+      const bool zero = (piv.x == 0.0 && piv.y == 0.0);
+      if (zero && first_zero == 0) first_zero = j0 + jj + 1;
+      const cd rp = zero ? mkc(0.0, 0.0) : recipc(piv);
+      if (tid == 0) {
+        perm[jj] = p;
+        if (ipiv) ipiv[item * n + j0 + jj] = j0 + p;
+      }
+      __syncthreads();  // every thread has read the pivot column before the swap
+      if (p != jj && tid < LB_NB) {
+        const cd t = pan[jj * LB_NB + tid];
+        pan[jj * LB_NB + tid] = pan[p * LB_NB + tid];
+        pan[p * LB_NB + tid] = t;
+      }
+      __syncthreads();
+      for (int r = jj + 1 + tid; r < pr; r += LB_THREADS) {
+        cd* row = pan + r * LB_NB;
+        const cd l = row[jj] * rp;
+        row[jj] = l;
+        for (int c = jj + 1; c < nb; ++c) row[c] = fmsc(row[c], l, pan[jj * LB_NB + c]);
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
+      const int r = e / LB_NB, c = e % LB_NB;
+      if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[e];
+    }
+    // ---- interchanges on the other columns (one thread per column)
+    const int nleft = keepL ? j0 : 0;
+    const int nright = n - j0 - nb;
+    for (int e = tid; e < nleft + nright + m; e += LB_THREADS) {
+      cd* col;
+      long long ld;
+      if (e < nleft) {
+        col = a + e;
+        ld = lda;
+      } else if (e < nleft + nright) {
+        col = a + j0 + nb + (e - nleft);
+        ld = lda;
+      } else {
+        col = bm + (e - nleft - nright);
+        ld = ldb;
+      }
+      for (int jj = 0; jj < nb; ++jj) {
+        const int p = perm[jj];
+        if (p != jj) {
+          cd* x = col + (long long)(j0 + jj) * ld;
+          cd* y = col + (long long)(j0 + p) * ld;
+          const cd t = *x;
+          *x = *y;
+          *y = t;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- U12 = L11^-1 A12, Y1 = L11^-1 B1 (unit lower, one thread per column)
+    for (int e = tid; e < nright + m; e += LB_THREADS) {
+      cd* col = (e < nright) ? a + j0 + nb + e : bm + (e - nright);
+      const long long ld = (e < nright) ? lda : ldb;
+      cd x[LB_NB];
+#pragma unroll
+      for (int i = 0; i < LB_NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ld] : mkc(0.0, 0.0);
+#pragma unroll
+      for (int i = 1; i < LB_NB; ++i)
+#pragma unroll
+        for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], pan[i * LB_NB + k], x[k]);
+#pragma unroll
+      for (int i = 0; i < LB_NB; ++i)
+        if (i < nb) col[(long long)(j0 + i) * ld] = x[i];
+    }
+    __syncthreads();
+    // ---- trailing update [A22 | B2] -= L21 [U12 | Y1]
+    if (j0 + nb < n) {
+      const int tot = nright + m;
+      for (int c0 = 0; c0 < tot; c0 += LB_TW) {
+        const int tw = min(LB_TW, tot - c0);
+        for (int e = tid; e < LB_NB * LB_TW; e += LB_THREADS) {
+          const int kk = e / LB_TW, t = e % LB_TW;
+          cd v = mkc(0.0, 0.0);
+          if (kk < nb && t < tw) {
+            const int c = c0 + t;
+            v = (c < nright) ? a[(long long)(j0 + kk) * lda + j0 + nb + c]
+                             : bm[(long long)(j0 + kk) * ldb + (c - nright)];
+          }
+          ut[e] = v;
+        }
+        __syncthreads();
+        // A and B columns have different leading dimensions: the tile is
+        // split at the A/B boundary.
+        const int ta = max(0, min(tw, nright - c0));
+        if (ta > 0)
+          mini_gemm_sub(pan, 0, ut, nb, nb, pr, ta,
+                        [&](int t) { return a + (long long)j0 * lda + j0 + nb + c0 + t; }, lda);
+        if (ta < tw) {
+          // shift the tile so that B columns start at tile column 0
+          __syncthreads();
+          for (int e = tid; e < LB_NB * LB_TW; e += LB_THREADS) {
+            const int kk = e / LB_TW, t = e % LB_TW;
+            cd v = mkc(0.0, 0.0);
+            if (kk < nb && ta + t < tw) v = bm[(long long)(j0 + kk) * ldb + (c0 + ta + t - nright)];
+            ut[e] = v;
+          }
+          __syncthreads();
+          mini_gemm_sub(pan, 0, ut, nb, nb, pr, tw - ta,
+                        [&](int t) { return bm + (long long)j0 * ldb + (c0 + ta + t - nright); }, ldb);
+        }
+        __syncthreads();
+      }
+    }
+  }
+  if (tid == 0 && info) info[item] = first_zero;
+  if (m <= 0) return;
+
+  // ---- back substitution U X = Y, NB rows at a time from the bottom
+  const int nblk = (n + LB_NB - 1) / LB_NB;
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int j0 = blk * LB_NB;
+    const int nb = min(LB_NB, n - j0);
+    // U[0:j0+nb, j0:j0+nb] -> pan (rows 0..j0+nb-1)
+    for (int e = tid; e < (j0 + nb) * LB_NB; e += LB_THREADS) {
+      const int r = e / LB_NB, c = e % LB_NB;
+      pan[e] = (c < nb) ? a[(long long)r * lda + j0 + c] : mkc(0.0, 0.0);
+    }
+    __syncthreads();
+    // X block = U11^-1 Y block (one thread per column of B)
+    for (int c = tid; c < m; c += LB_THREADS) {
+      cd* col = bm + c;
+      cd x[LB_NB];
+#pragma unroll
+      for (int i = 0; i < LB_NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ldb] : mkc(0.0, 0.0);
+#pragma unroll
+      for (int i = LB_NB - 1; i >= 0; --i) {
+        if (i >= nb) continue;
+#pragma unroll
+        for (int k = i + 1; k < LB_NB; ++k)
+          if (k < nb) x[i] = fmsc(x[i], pan[(j0 + i) * LB_NB + k], x[k]);
+        x[i] = divc(x[i], pan[(j0 + i) * LB_NB + i]);
+      }
+#pragma unroll
+      for (int i = 0; i < LB_NB; ++i)
+        if (i < nb) col[(long long)(j0 + i) * ldb] = x[i];
+    }
+    __syncthreads();
+    // Y[0:j0] -= U[0:j0, j0:j0+nb] X block
+    if (j0 > 0) {
+      for (int c0 = 0; c0 < m; c0 += LB_TW) {
+        const int tw = min(LB_TW, m - c0);
+        for (int e = tid; e < LB_NB * LB_TW; e += LB_THREADS) {
+          const int kk = e / LB_TW, t = e % LB_TW;
+          ut[e] = (kk < nb && t < tw) ? bm[(long long)(j0 + kk) * ldb + c0 + t] : mkc(0.0, 0.0);
+        }
+        __syncthreads();
+        mini_gemm_sub(pan, 0, ut, nb, 0, j0, tw, [&](int t) { return bm + c0 + t; }, ldb);
+        __syncthreads();
+      }
+    }
+  }
+}
+
+static size_t lu_blocked_lds(int n) { return ((size_t)n * LB_NB + (size_t)LB_NB * LB_TW) * sizeof(cd); }
+
+bool lu_blocked_fits(int n) { return lu_blocked_lds(n) <= 150 * 1024; }
+
+int zlu_blocked(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL) {
+  if (batch <= 0 || n <= 0) return 0;
+  const size_t lds = lu_blocked_lds(n);
+  KHX_REQUIRE(lds <= 150 * 1024, "zlu_blocked: n too large for the LDS panel");
+  static size_t lds_limit = 0;
+  KHX_TRY(ensure_dynamic_lds((const void*)zlu_blocked_kernel, lds, lds_limit));
+  ProfScope ps(m > 0 ? "zgesv_kernel" : "zgetrf_kernel", s);
+  zlu_blocked_kernel<<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, A.p, A.ld, A.stride, B.p, B.ld, B.stride, ipiv,
+                                                                info, keepL ? 1 : 0);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace khx
