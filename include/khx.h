@@ -166,6 +166,24 @@ int khx_propagation(const double* kx, const double* ky, int N, int batch, const 
 int khx_joint_subspace(const double* in, int Nb, int blocks, int kind, int items, double* out, void* stream);
 int khx_joint_subspace_diag(const double* in, int Nb, int kind, int items, double* out, void* stream);
 
+/* Field reconstruction (khepri/crystal.py:234-343, khepri/fields.py,
+ * khepri/fourier.py:136-142).
+ *   khx_eigenbasis:     R = [[W, W], [-V, V]]  (batch, 2n, 2n)   fields.py:46-51
+ *   khx_mode_propagate: M[i][j] = L_i(zbar_j) a_i, L = [e^{lam zbar}, e^{-lam zbar}]
+ *                       with |L| clamped to 1e14; lam (n), a (2n), M (2n, nz)
+ *                                                                fields.py:29-31, 53-62
+ *   khx_longitudinal:   F = [sx; sy; ux; uy] (4N, nz) -> uz = -i (kx sy - ky sx)
+ *                       and w = kx uy - ky ux (sz = -i IC w)     fields.py:68-76
+ *   khx_idft_phase:     E[p][g] = exp(i k0 (Kx_g x_p + Ky_g y_p)), (npts, N);
+ *                       Kx, Ky normalised (as from khx_kvectors)  fourier.py:136-142 */
+int khx_eigenbasis(const double* W, const double* V, int n, int batch, double* R, void* stream);
+int khx_mode_propagate(const double* lam, const double* a, const double* zbar, int n, int nz, double* M,
+                       void* stream);
+int khx_longitudinal(const double* kx, const double* ky, const double* F, int N, int nz, double* uz, double* w,
+                     void* stream);
+int khx_idft_phase(const double* kx, const double* ky, double k0, const double* x, const double* y, int N, int npts,
+                   double* E, void* stream);
+
 /* Measurement support (no counterpart in the reference).
  * khx_prof_enable(1) clears the records and brackets every launch of the
  * instrumented kernels (zeig_schur_kernel, zgesv_kernel, zgemm_kernel) with

@@ -41,6 +41,10 @@ SIGNATURES = {
     "khx_diag_scale": (_i, [_dp, _i, C.c_longlong, _dp, _dp, _i, _i, _i, _dp, _i, C.c_longlong, _vp]),
     "khx_identity_minus": (_i, [_dp, _i, _i, _dp, _vp]),
     "khx_propagation": (_i, [_dp, _dp, _i, _i, _dp, _dp, _vp]),
+    "khx_eigenbasis": (_i, [_dp, _dp, _i, _i, _dp, _vp]),
+    "khx_mode_propagate": (_i, [_dp, _dp, _dp, _i, _i, _dp, _vp]),
+    "khx_longitudinal": (_i, [_dp, _dp, _dp, _i, _i, _dp, _dp, _vp]),
+    "khx_idft_phase": (_i, [_dp, _dp, C.c_double, _dp, _dp, _i, _i, _dp, _vp]),
     "khx_joint_subspace": (_i, [_dp, _i, _i, _i, _i, _dp, _vp]),
     "khx_joint_subspace_diag": (_i, [_dp, _i, _i, _i, _dp, _vp]),
     "khx_prof_enable": (_i, [_i]),

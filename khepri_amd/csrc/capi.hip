@@ -34,6 +34,11 @@ int zgetrs(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch
 int zdiag_scale(hipStream_t s, Mat A, const cd* dl, const cd* dr, int rows, int cols, int batch, Mat D);
 int zidentity_minus(hipStream_t s, const cd* A, int n, int batch, cd* D);
 int propagation(hipStream_t s, const cd* kx, const cd* ky, int N, int batch, const double* k0d, cd* X);
+int eigenbasis(hipStream_t s, const cd* W, const cd* V, int n, int batch, cd* R);
+int mode_propagate(hipStream_t s, const cd* lam, const cd* a, const double* zbar, int n, int nz, cd* M);
+int longitudinal(hipStream_t s, const cd* kx, const cd* ky, const cd* F, int N, int nz, cd* uz, cd* w);
+int idft_phase(hipStream_t s, const cd* kx, const cd* ky, double k0, const double* x, const double* y, int N,
+               int npts, cd* E);
 
 int joint_subspace(hipStream_t s, const cd* in, int Nb, int M, int blocks, int kind, int items, cd* out);
 int joint_subspace_vec(hipStream_t s, const cd* in, int Nb, int M, int kind, int items, cd* out);
@@ -203,6 +208,33 @@ int khx_propagation(const double* kx, const double* ky, int N, int batch, const 
   KHX_ENTRY_BEGIN
   KHX_REQUIRE(N > 0 && batch >= 0, "khx_propagation: bad sizes");
   return khx::propagation(as_stream(stream), as_cd(kx), as_cd(ky), N, batch, k0d, as_cd(X));
+}
+
+int khx_eigenbasis(const double* W, const double* V, int n, int batch, double* R, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0, "khx_eigenbasis: bad sizes");
+  return khx::eigenbasis(as_stream(stream), as_cd(W), as_cd(V), n, batch, as_cd(R));
+}
+
+int khx_mode_propagate(const double* lam, const double* a, const double* zbar, int n, int nz, double* M,
+                       void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && nz >= 0, "khx_mode_propagate: bad sizes");
+  return khx::mode_propagate(as_stream(stream), as_cd(lam), as_cd(a), zbar, n, nz, as_cd(M));
+}
+
+int khx_longitudinal(const double* kx, const double* ky, const double* F, int N, int nz, double* uz, double* w,
+                     void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && nz >= 0, "khx_longitudinal: bad sizes");
+  return khx::longitudinal(as_stream(stream), as_cd(kx), as_cd(ky), as_cd(F), N, nz, as_cd(uz), as_cd(w));
+}
+
+int khx_idft_phase(const double* kx, const double* ky, double k0, const double* x, const double* y, int N, int npts,
+                   double* E, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && npts >= 0, "khx_idft_phase: bad sizes");
+  return khx::idft_phase(as_stream(stream), as_cd(kx), as_cd(ky), k0, x, y, N, npts, as_cd(E));
 }
 
 int khx_identity_minus(const double* A, int n, int batch, double* out, void* stream) {

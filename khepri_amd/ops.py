@@ -320,3 +320,45 @@ def joint_subspace_diag(v, Nb, kind):
     out = torch.empty((items, 2 * Nb * Nb), dtype=CDT, device=v.device)
     check(lib.khx_joint_subspace_diag(_p(v), Nb, kind, items, _p(out), _stream()), "khx_joint_subspace_diag")
     return out
+
+
+def eigenbasis(W, V):
+    """[[W, W], [-V, V]] for (b, n, n) batches."""
+    lib = _lib.load()
+    W, V = W.contiguous(), V.contiguous()
+    b, n, _ = W.shape
+    R = torch.empty((b, 2 * n, 2 * n), dtype=CDT, device=W.device)
+    check(lib.khx_eigenbasis(_p(W), _p(V), n, b, _p(R), _stream()), "khx_eigenbasis")
+    return R
+
+
+def mode_propagate(lam, a, zbar):
+    lib = _lib.load()
+    n = lam.shape[-1]
+    lam, a, zbar = lam.contiguous(), a.contiguous(), zbar.contiguous()
+    M = torch.empty((2 * n, zbar.shape[0]), dtype=CDT, device=lam.device)
+    check(lib.khx_mode_propagate(_p(lam), _p(a), _p(zbar), n, zbar.shape[0], _p(M), _stream()), "khx_mode_propagate")
+    return M
+
+
+def longitudinal(kx, ky, F):
+    lib = _lib.load()
+    N = kx.shape[-1]
+    nz = F.shape[-1]
+    F = F.contiguous()
+    uz = torch.empty((N, nz), dtype=CDT, device=F.device)
+    w = torch.empty_like(uz)
+    check(lib.khx_longitudinal(_p(kx.contiguous()), _p(ky.contiguous()), _p(F), N, nz, _p(uz), _p(w), _stream()),
+          "khx_longitudinal")
+    return uz, w
+
+
+def idft_phase(kx, ky, k0, x, y):
+    """exp(i k0 (Kx x + Ky y)) for normalised K, (npts, N)."""
+    lib = _lib.load()
+    N = kx.shape[-1]
+    x, y = x.contiguous(), y.contiguous()
+    E = torch.empty((x.shape[0], N), dtype=CDT, device=x.device)
+    check(lib.khx_idft_phase(_p(kx.contiguous()), _p(ky.contiguous()), float(k0), _p(x), _p(y), N, x.shape[0], _p(E),
+                             _stream()), "khx_idft_phase")
+    return E

@@ -1,0 +1,46 @@
+"""Extended-basis (twisted bilayer) R/T against the reference's golden vectors."""
+import numpy as np
+import pytest
+
+from conftest import golden
+from khepri_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-9
+
+
+def twisted(pw, angle, pix, fields=False):
+    from khepri_amd import Crystal, Expansion, Layer
+    e1, e2 = Expansion(pw), Expansion(pw)
+    e2.rotate(angle)
+    etw = e1 + e2
+    cl = Crystal.from_expansion(etw)
+    cl.add_layer("L1", Layer.pixmap(e1, pix, 0.4), extended=True)
+    cl.add_layer("gap", Layer.uniform(etw, 1.0, 0.2))
+    cl.add_layer("L2", Layer.pixmap(e2, pix, 0.4), extended=True)
+    cl.set_device(["L1", "gap", "L2"], fields)
+    return cl
+
+
+@pytest.mark.parametrize("rt_only", [False, True])
+def test_twisted_small_rt(gpu, rt_only):
+    g = golden("twisted_small.npz")
+    pix = W.disc_pixmap(128, 1.0, 0.3, 4.0)
+    for a, r0, t0 in zip(g["angles"], g["R"], g["T"]):
+        cl = twisted((3, 3), float(a), pix)
+        cl.set_source(1 / float(g["freq"]), 1.0, 0.0)
+        cl.solve(rt_only=rt_only)
+        r, t = cl.poynting_flux_end()
+        assert abs(r - r0) <= TOL and abs(t - t0) <= TOL, (a, r, r0, t, t0)
+
+
+def test_twisted_625_harmonics_rt(gpu):
+    """Base pw (5,5): 625 extended harmonics, n = 1250."""
+    g = golden("twisted.npz")
+    cfg = W.twisted_bilayer()
+    ai = int(g["angle_index"][0])
+    cl = twisted((5, 5), np.deg2rad(cfg["angles_deg"][ai]), cfg["pixmap"])
+    cl.set_source(1 / float(g["freq"]), 1.0, 0.0)
+    cl.solve(rt_only=True)
+    r, t = cl.poynting_flux_end()
+    assert abs(r - g["R"][0]) <= TOL and abs(t - g["T"][0]) <= TOL
