@@ -80,6 +80,73 @@ def cpu_baseline(cfg, gpu_R, gpu_T, idx_all, sample):
                        f"reference op sequence incl. reverse star chain), 1 BLAS thread, {el:.1f} s"), dR, dT
 
 
+def extra_workload(name, args):
+    """Secondary measurements (not the driver's line): returns a JSON dict."""
+    import torch
+    from khepri_amd import Crystal, Expansion, Layer, workloads
+    from khepri_amd.batch import CrystalBatch
+    if name == "disc_sweep":
+        cfg = workloads.disc_stack_sweep()
+        cl = Crystal(cfg["pw"])
+        cl.add_layer_uniform("S1", cfg["eps_gap"], cfg["depth_gap"])
+        cl.add_layer_pixmap("Scyl", cfg["pixmap"], cfg["depth_disc"])
+        cl.set_device(["Scyl", "S1", "Scyl"], False)
+        cl.set_source(1 / cfg["freqs"], 1.0, 0.0)
+
+        def step():
+            cl.solve(rt_only=True)
+            return cl.poynting_flux_end()
+        units, unit = len(cfg["freqs"]), "solves/s"
+    elif name == "random_devices":
+        devs = workloads.random_device_params()[:args.devices]
+        cls = []
+        for dv in devs:
+            c = Crystal(dv["pw"])
+            names = []
+            for j, ly in enumerate(dv["layers"]):
+                c.add_layer_pixmap(f"L{j}", workloads.render_layer(ly), ly["depth"])
+                names.append(f"L{j}")
+            c.set_device(names, False)
+            c.set_source(1 / dv["freq"], 1.0, 1.0, dv["theta"], dv["phi"])
+            cls.append(c)
+        batch = CrystalBatch(cls)
+
+        def step():
+            return batch.solve()
+        units, unit = len(cls), "devices/s"
+    elif name == "twisted":
+        cfg = workloads.twisted_bilayer()
+        e1, e2 = Expansion(cfg["pw"]), Expansion(cfg["pw"])
+        e2.rotate(np.deg2rad(cfg["angles_deg"][0]))
+        etw = e1 + e2
+        cl = Crystal.from_expansion(etw)
+        cl.add_layer("L1", Layer.pixmap(e1, cfg["pixmap"], cfg["depth"]), extended=True)
+        cl.add_layer("gap", Layer.uniform(etw, 1.0, cfg["gap"]))
+        cl.add_layer("L2", Layer.pixmap(e2, cfg["pixmap"], cfg["depth"]), extended=True)
+        cl.set_device(["L1", "gap", "L2"], False)
+        fr = cfg["freqs"][:args.freqs]
+        cl.set_source(1 / fr, 1.0, 0.0)
+
+        def step():
+            cl.solve(rt_only=True)
+            return cl.poynting_flux_end()
+        units, unit = len(fr), "solves/s"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"metric": f"{name} throughput", "value": round(units * args.steps / el, 3), "unit": unit,
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": name, "units_per_step": units}}
+
+
 def load_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
     if not os.path.exists(path):
@@ -98,6 +165,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--solves", type=int, default=65536, help="subset of the workload (default: all)")
     ap.add_argument("--cpu-sample", type=int, default=40, help="oracle solves timed for cpu_baseline (0: skip)")
+    ap.add_argument("--workload", default="disc_stack_bz",
+                    help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted")
+    ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
+    ap.add_argument("--freqs", type=int, default=32, help="twisted: number of frequencies")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -110,6 +181,9 @@ def main():
     from khepri_amd import _lib
 
     lib = _lib.load()
+    if args.workload != "disc_stack_bz":
+        print(json.dumps(extra_workload(args.workload, args)), flush=True)
+        return
     cl, cfg, idx = build_crystal(args.solves)
     nsol = len(idx)
     n = 2 * 81

@@ -80,8 +80,9 @@ int khx_eig(int n, double* A, int batch, double* lam, double* W, int* info, void
 int khx_eig_timed(int n, double* A, int batch, double* lam, double* W, int* info, void* ws,
                   unsigned long long* tphase, void* stream);
 
-/* Patterned layer: modes + S-matrix for a batch of (kx, ky, k0*depth) that
- * share one convolution matrix C and IC = C^-1.
+/* Patterned layer: modes + S-matrix for a batch of (kx, ky, k0*depth) with
+ * the convolution matrix C and IC = C^-1 shared by the batch (per_item = 0:
+ * C, IC are N x N) or one per item (per_item = 1: (batch, N, N)).
  * Replaces khepri/layer.py:157-160 (Layer.solve, FFT formulation):
  *   khepri/alternative.py:158-178 solve_structured_layer -> (W, V, lam)
  *   khepri/alternative.py:181-195 build_scatmat          -> S
@@ -89,8 +90,8 @@ int khx_eig_timed(int n, double* A, int batch, double* lam, double* W, int* info
  * lam (batch, n) are written when non-NULL (needed for fields).
  * info[b] > 0: eigen-solver did not converge.                              */
 size_t khx_patterned_layer_ws_bytes(int n, int batch);
-int khx_patterned_layer(const double* C, const double* IC, const double* kx, const double* ky, int N,
-                        int batch, const double* k0d, double* S, double* W, double* V, double* lam,
+int khx_patterned_layer(const double* C, const double* IC, int per_item, const double* kx, const double* ky,
+                        int N, int batch, const double* k0d, double* S, double* W, double* V, double* lam,
                         int* info, void* ws, void* stream);
 
 /* Uniform layer of permittivity eps (khepri/layer.py:166-168:

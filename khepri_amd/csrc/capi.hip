@@ -15,8 +15,8 @@ const char* last_error() { return g_last_error.c_str(); }
 int kvectors(hipStream_t s, const double* g, int N, const cd* kp, const double* wl, int batch, cd* kx, cd* ky);
 int convmat(hipStream_t s, const double* eps, int cplx, int nx, int ny, int P, int Q, int count, cd* C, cd* Fws);
 size_t patterned_layer_ws_bytes(int n, int batch);
-int patterned_layer(hipStream_t s, const cd* C, const cd* IC, const cd* kx, const cd* ky, int N, int batch,
-                    const double* k0d, cd* S, cd* Wout, cd* Vout, cd* lamout, int* info, void* ws);
+int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride, const cd* kx, const cd* ky, int N,
+                    int batch, const double* k0d, cd* S, cd* Wout, cd* Vout, cd* lamout, int* info, void* ws);
 int analytic_layer(hipStream_t s, int kind, const cd* kx, const cd* ky, int N, int batch, cd eps,
                    const double* k0d, cd* S, cd* W, cd* V, cd* lam);
 size_t star_ws_bytes(int n, int batch);
@@ -55,7 +55,7 @@ static inline const cd* as_cd(const double* p) { return reinterpret_cast<const c
 extern "C" {
 
 const char* khx_last_error(void) { return khx::last_error(); }
-int khx_abi_version(void) { return 1; }
+int khx_abi_version(void) { return 2; }
 
 int khx_kvectors(const double* g, int N, const double* kp, const double* wl, int batch, double* kx, double* ky,
                  void* stream) {
@@ -114,13 +114,13 @@ int khx_eig_timed(int n, double* A, int batch, double* lam, double* W, int* info
 
 size_t khx_patterned_layer_ws_bytes(int n, int batch) { return khx::patterned_layer_ws_bytes(n, batch); }
 
-int khx_patterned_layer(const double* C, const double* IC, const double* kx, const double* ky, int N, int batch,
-                        const double* k0d, double* S, double* W, double* V, double* lam, int* info, void* ws,
+int khx_patterned_layer(const double* C, const double* IC, int per_item, const double* kx, const double* ky, int N,
+                        int batch, const double* k0d, double* S, double* W, double* V, double* lam, int* info, void* ws,
                         void* stream) {
   KHX_ENTRY_BEGIN
   KHX_REQUIRE(N > 0 && batch >= 0, "khx_patterned_layer: bad sizes");
-  return khx::patterned_layer(as_stream(stream), as_cd(C), as_cd(IC), as_cd(kx), as_cd(ky), N, batch, k0d, as_cd(S),
-                              as_cd(W), as_cd(V), as_cd(lam), info, ws);
+  return khx::patterned_layer(as_stream(stream), as_cd(C), as_cd(IC), per_item ? (long long)N * N : 0, as_cd(kx),
+                              as_cd(ky), N, batch, k0d, as_cd(S), as_cd(W), as_cd(V), as_cd(lam), info, ws);
 }
 
 int khx_uniform_layer(const double* kx, const double* ky, int N, int batch, const double* eps_host,

@@ -180,7 +180,7 @@ __global__ void fill_v0_kernel(const cd* __restrict__ kx, const cd* __restrict__
 // and the diagonal part of P*Q (the C part is added by two GEMMs):
 //   PQ[:, c]   = P[:, c] kx_c ky_c + P[:, N+c] ky_c^2
 //   PQ[:, N+c] = -P[:, c] kx_c^2 - P[:, N+c] ky_c kx_c
-__global__ void build_p_kernel(const cd* __restrict__ IC, const cd* __restrict__ kx,
+__global__ void build_p_kernel(const cd* __restrict__ IC, long long icstride, const cd* __restrict__ kx,
                                const cd* __restrict__ ky, int N, int batch, cd* P, cd* PQ) {
   const int n = 2 * N;
   const long long nn = (long long)n * n;
@@ -190,7 +190,7 @@ __global__ void build_p_kernel(const cd* __restrict__ IC, const cd* __restrict__
     const long long b = e / ((long long)N * N);
     const int rc = (int)(e % ((long long)N * N));
     const int r = rc / N, c = rc % N;
-    const cd ic = IC[rc];
+    const cd ic = IC[b * icstride + rc];
     const cd xr = kx[b * N + r], yr = ky[b * N + r], xc = kx[b * N + c], yc = ky[b * N + c];
     const cd one = mkc(r == c ? 1.0 : 0.0, 0.0);
     const cd p11 = xr * ic * yc;
@@ -316,11 +316,11 @@ size_t patterned_layer_ws_bytes(int n, int batch) {
          (size_t)batch * sizeof(int) * 8 + 4096;  // 5 status words per item fit in 8
 }
 
-// Modes and S-matrix of a patterned layer for a batch of (kx, ky, k0*depth)
-// sharing one convolution matrix C (and its inverse IC).
+// Modes and S-matrix of a patterned layer for a batch of (kx, ky, k0*depth);
+// C and IC = C^-1 are shared (cstride 0) or per item (cstride N*N).
 //   khepri/alternative.py:158-178 (solve_structured_layer) followed by
 //   khepri/alternative.py:181-195 (build_scatmat) with W0 = I.
-int patterned_layer(hipStream_t s, const cd* C, const cd* IC, const cd* kx, const cd* ky, int N,
+int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride, const cd* kx, const cd* ky, int N,
                     int batch, const double* k0d, cd* S, cd* Wout, cd* Vout, cd* lamout, int* info,
                     void* wsv) {
   if (batch <= 0) return 0;
@@ -362,19 +362,19 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, const cd* kx, cons
   const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0), zero = mkc(0.0, 0.0);
   const long long tNN = (long long)batch * N * N;
   // P and diag(PQ)
-  build_p_kernel<<<grid_for(tNN), 256, 0, s>>>(IC, kx, ky, N, batch, P, PQ);
+  build_p_kernel<<<grid_for(tNN), 256, 0, s>>>(IC, cstride, kx, ky, N, batch, P, PQ);
   KHX_LAUNCH_CHECK();
   // PQ[:, 0:N] -= P[:, N:] C ;  PQ[:, N:] += P[:, 0:N] C
-  KHX_TRY(zgemm(s, n, N, N, mone, Mat{P + N, n, nn}, Mat{const_cast<cd*>(C), N, 0}, one, Mat{PQ, n, nn}, batch));
-  KHX_TRY(zgemm(s, n, N, N, one, Mat{P, n, nn}, Mat{const_cast<cd*>(C), N, 0}, one, Mat{PQ + N, n, nn}, batch));
+  KHX_TRY(zgemm(s, n, N, N, mone, Mat{P + N, n, nn}, Mat{const_cast<cd*>(C), N, cstride}, one, Mat{PQ, n, nn}, batch));
+  KHX_TRY(zgemm(s, n, N, N, one, Mat{P, n, nn}, Mat{const_cast<cd*>(C), N, cstride}, one, Mat{PQ + N, n, nn}, batch));
   // (lam^2, W) = eig(PQ)
   KHX_TRY(zeig(s, n, Mat{PQ, n, nn}, lam, n, Mat{W, n, nn}, batch, info_eig, eigws));
   // lam = sqrt(lam^2 + 0j); V = Q W / lam   (Q W: diagonal part + C blocks)
   qw_diag_kernel<<<grid_for((long long)batch * N * n), 256, 0, s>>>(W, kx, ky, N, batch, V, lam);
   KHX_LAUNCH_CHECK();
-  KHX_TRY(zgemm(s, N, n, N, one, Mat{const_cast<cd*>(C), N, 0}, Mat{W + (long long)N * n, n, nn}, one,
+  KHX_TRY(zgemm(s, N, n, N, one, Mat{const_cast<cd*>(C), N, cstride}, Mat{W + (long long)N * n, n, nn}, one,
                 Mat{V, n, nn}, batch));
-  KHX_TRY(zgemm(s, N, n, N, mone, Mat{const_cast<cd*>(C), N, 0}, Mat{W, n, nn}, one,
+  KHX_TRY(zgemm(s, N, n, N, mone, Mat{const_cast<cd*>(C), N, cstride}, Mat{W, n, nn}, one,
                 Mat{V + (long long)N * n, n, nn}, batch));
   col_div_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(V, V, lam, n, batch);
   KHX_LAUNCH_CHECK();

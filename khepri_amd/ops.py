@@ -140,7 +140,8 @@ def eig(A):
 
 
 def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None):
-    """S (B, 2, 2, n, n) of a patterned layer; (W, V, lam) too when modes."""
+    """S (B, 2, 2, n, n) of a patterned layer; (W, V, lam) too when modes.
+    C, IC: (N, N) shared by the batch or (B, N, N) one per item."""
     lib = _lib.load()
     B, N = kx.shape
     n = 2 * N
@@ -155,8 +156,12 @@ def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None):
     need = lib.khx_patterned_layer_ws_bytes(n, B)
     if ws is None or ws.numel() < need:
         ws = _ws(need, dev)
-    check(lib.khx_patterned_layer(_p(C), _p(IC), _p(kx), _p(ky), N, B, _p(k0d), _p(S), _p(W), _p(V), _p(lam),
-                                  _p(info), _p(ws), _stream()), "khx_patterned_layer")
+    per_item = 1 if C.dim() == 3 else 0
+    C, IC = C.contiguous(), IC.contiguous()
+    if per_item and (C.shape[0] != B or IC.shape[0] != B):
+        raise ValueError("per-item convolution matrices must match the batch")
+    check(lib.khx_patterned_layer(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(S), _p(W), _p(V),
+                                  _p(lam), _p(info), _p(ws), _stream()), "khx_patterned_layer")
     return S, (W, V, lam), info
 
 

@@ -29,7 +29,7 @@ def test_library_exports_all_declared_symbols():
 def test_host_only_entry_points():
     from khepri_amd import _lib
     lib = _lib.load()
-    assert lib.khx_abi_version() == 1
+    assert lib.khx_abi_version() == 2
     assert lib.khx_eig_ws_bytes(50, 4) >= 2 * 50 * 50 * 16 * 4
     assert lib.khx_redheffer_ws_bytes(162, 2) >= 8 * 162 * 162 * 16 * 2
     # argument validation happens on the host, before any launch
