@@ -10,7 +10,7 @@ CPU baseline.  The product package ``khepri_amd`` never imports it; the GPU
 path has no CPU fallback.
 
 Pinning status of each function (what checks it, besides retyping):
-  expansion_indices / reciproc / g_vectors   closed forms (square & hex), tests/test_oracle_closed_forms.py
+  expansion_indices / reciproc / g_vectors   closed forms (square & hex), tests/test_oracle.py
   compute_kplanar                            closed form
   convolution_matrix                         brute-force DFT, constant pixmap -> eps*I, 1-D grating sinc,
                                              golden vectors from the reference (tests/golden)
