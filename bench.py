@@ -131,6 +131,21 @@ def extra_workload(name, args):
             cl.solve(rt_only=True)
             return cl.poynting_flux_end()
         units, unit = len(fr), "solves/s"
+    elif name == "grad_grid":
+        from khepri_amd.adjoint import flux_gradient
+        cfg = workloads.grad_grid()
+        cl = Crystal(cfg["pw"])
+        names = []
+        for j, pix in enumerate(cfg["pixmaps"]):
+            cl.add_layer_pixmap(f"L{j}", pix, cfg["depth"])
+            names.append(f"L{j}")
+        cl.set_device(names, False)
+        cl.set_source(1 / cfg["freqs"], 1.0, 0.0)
+
+        def step():
+            # forward (R, T) + backward: one dT/d eps map per frequency for all 4 x 64 x 64 pixels
+            return flux_gradient(cl, wT=1.0)
+        units, unit = len(cfg["freqs"]), "grad evals/s"
     else:
         raise SystemExit(f"unknown workload {name}")
     for _ in range(args.warmup):
@@ -166,7 +181,7 @@ def main():
     ap.add_argument("--solves", type=int, default=65536, help="subset of the workload (default: all)")
     ap.add_argument("--cpu-sample", type=int, default=40, help="oracle solves timed for cpu_baseline (0: skip)")
     ap.add_argument("--workload", default="disc_stack_bz",
-                    help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted")
+                    help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted | grad_grid")
     ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
     ap.add_argument("--freqs", type=int, default=32, help="twisted: number of frequencies")
     args = ap.parse_args()

@@ -149,6 +149,11 @@ int khx_lu_solve(int n, int nrhs, const double* LU, const int* ipiv, double* B, 
 int khx_diag_scale(const double* A, int lda, long long strideA, const double* dl, const double* dr, int rows,
                    int cols, int batch, double* out, int ldo, long long strideO, void* stream);
 int khx_identity_minus(const double* A, int n, int batch, double* out, void* stream);
+/* D = a A + b B on strided batches of (rows x cols) views (B may be NULL
+ * when b = 0; D may alias A or B).  a, b: host complex scalars.            */
+int khx_zaxpby(double* D, int ldd, long long strideD, const double* a, const double* A, int lda, long long strideA,
+               const double* b, const double* B, int ldb, long long strideB, int rows, int cols, int batch,
+               void* stream);
 
 /* Propagation factors X (batch, n) of a uniform layer of the gap medium
  * (eps = 1), whose S-matrix is exactly [[0, diag X], [diag X, 0]]
@@ -184,6 +189,69 @@ int khx_longitudinal(const double* kx, const double* ky, const double* F, int N,
                      void* stream);
 int khx_idft_phase(const double* kx, const double* ky, double k0, const double* x, const double* y, int N, int npts,
                    double* E, void* stream);
+
+/* ---------------------------------------------------------------- gradient
+ * Reverse-mode (adjoint) building blocks for d(R, T)/d(pixel eps).  The
+ * reference has no gradient; these are the adjoints of the forward entry
+ * points above (khepri/tools.py:33-56 convolution_matrix,
+ * khepri/alternative.py:158-195 structured layer + build_scatmat,
+ * :19-30 redheffer_product, :235-245 poynting_fluxes).  Convention:
+ * Zbar = dL/dRe Z + i dL/dIm Z for a real objective L.
+ *
+ * khx_zgemm_op: C = alpha op(A) op(B) + beta C, op 0 = X, 1 = X^T, 2 = X^H
+ *   (a transposed operand is stored with its other dimension first).
+ * khx_conj_transpose: B = A^H for (rows x cols) items.
+ * khx_reduce_dot: out[b][i] (+)= sum_j A_ij conj(B_ij) (axis 1) or
+ *   out[b][j] (+)= sum_i A_ij conj(B_ij) (axis 0); out item stride ostride.
+ * khx_eig_adjoint_mix: G (batch, n, n) holds W^H Wbar on entry and
+ *   G_ij E_ij / (|E_ij|^2 + eta^2), E_ij = mu_j - mu_i, G_ii = mubar_i on exit
+ *   (eta = eta_rel max|mu|: Lorentzian broadening of degenerate pairs).
+ * khx_lambda_adjoint: mubar from lambar contributions of x = exp(-lam k0d)
+ *   (xbar) and 1/lam (ilbar), lam = sqrt(mu); either may be NULL.
+ * khx_build_pq: dense P and Q (batch, n, n) of a patterned layer.
+ * khx_pq_adjoint: ICbar, Cbar (batch, N, N) from Pbar, Qbar.
+ * khx_convmat_adjoint: pixel adjoint epsbar (batch, nx, ny) (real, or
+ *   interleaved complex when is_complex) from Cbar (batch, N, N).
+ * khx_poynting_vec_adjoint: sRbar, sTbar (batch, n) for the objective
+ *   sum_b wR[b] R[b] + wT[b] T[b] (wR or wT may be NULL).                   */
+int khx_zgemm_op(int opa, int opb, int M, int N, int K, const double* alpha, const double* A, int lda,
+                 long long strideA, const double* B, int ldb, long long strideB, const double* beta, double* C,
+                 int ldc, long long strideC, int batch, void* stream);
+int khx_conj_transpose(const double* A, int lda, long long strideA, int rows, int cols, double* B, int ldb,
+                       long long strideB, int batch, void* stream);
+int khx_reduce_dot(const double* A, int lda, long long strideA, const double* B, int ldb, long long strideB,
+                   int rows, int cols, int axis, int accumulate, double* out, long long ostride, int batch,
+                   void* stream);
+int khx_eig_adjoint_mix(double* G, const double* mu, const double* mubar, int n, int batch, double eta_rel,
+                        void* stream);
+int khx_lambda_adjoint(const double* lam, const double* k0d, const double* xbar, const double* ilbar, int n,
+                       int batch, double* mubar, void* stream);
+int khx_build_pq(const double* C, const double* IC, int per_item, const double* kx, const double* ky, int N,
+                 int batch, double* P, double* Q, void* stream);
+int khx_pq_adjoint(const double* Pbar, const double* Qbar, const double* kx, const double* ky, int N, int batch,
+                   double* ICbar, double* Cbar, void* stream);
+size_t khx_convmat_adjoint_ws_bytes(int P, int Q, int batch);
+int khx_convmat_adjoint(const double* Cbar, int P, int Q, int nx, int ny, int batch, int is_complex,
+                        double* epsbar, void* ws, void* stream);
+/* Backward twins of khx_patterned_layer and khx_redheffer.
+ * khx_patterned_layer_bwd: Cbar (batch, N, N) = adjoint of the convolution
+ *   matrix (IC = C^-1 is differentiated too) given Sbar (batch, 2, 2, n, n),
+ *   the modes W, V, lam written by khx_patterned_layer for the same inputs
+ *   and eta_rel, the Lorentzian broadening of the eigenvector adjoint
+ *   (1e-12 is a good default).  The S-matrix intermediates are recomputed.
+ * khx_redheffer_bwd: SAbar, SBbar (batch, 2, 2, n, n) for S = SA * SB given
+ *   Sbar (no broadcast operands).  info as in the forward calls.           */
+size_t khx_patterned_layer_bwd_ws_bytes(int n, int batch);
+int khx_patterned_layer_bwd(const double* C, const double* IC, int per_item, const double* kx, const double* ky,
+                            int N, int batch, const double* k0d, const double* W, const double* V,
+                            const double* lam, const double* Sbar, double eta_rel, double* Cbar, int* info, void* ws,
+                            void* stream);
+size_t khx_redheffer_bwd_ws_bytes(int n, int batch);
+int khx_redheffer_bwd(const double* SA, const double* SB, int n, int batch, const double* Sbar, double* SAbar,
+                      double* SBbar, int* info, void* ws, void* stream);
+int khx_poynting_vec_adjoint(const double* sR, const double* sT, int N, const double* g, const double* kp,
+                             const double* wl, const double* epsi_host, const double* epse_host, const double* wR,
+                             const double* wT, int batch, double* sRbar, double* sTbar, void* stream);
 
 /* Measurement support (no counterpart in the reference).
  * khx_prof_enable(1) clears the records and brackets every launch of the

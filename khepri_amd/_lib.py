@@ -40,6 +40,8 @@ SIGNATURES = {
     "khx_lu_solve": (_i, [_i, _i, _dp, _ip, _dp, _i, _vp]),
     "khx_diag_scale": (_i, [_dp, _i, C.c_longlong, _dp, _dp, _i, _i, _i, _dp, _i, C.c_longlong, _vp]),
     "khx_identity_minus": (_i, [_dp, _i, _i, _dp, _vp]),
+    "khx_zaxpby": (_i, [_dp, _i, C.c_longlong, _dp, _dp, _i, C.c_longlong, _dp, _dp, _i, C.c_longlong, _i, _i, _i,
+                        _vp]),
     "khx_propagation": (_i, [_dp, _dp, _i, _i, _dp, _dp, _vp]),
     "khx_eigenbasis": (_i, [_dp, _dp, _i, _i, _dp, _vp]),
     "khx_mode_propagate": (_i, [_dp, _dp, _dp, _i, _i, _dp, _vp]),
@@ -47,6 +49,23 @@ SIGNATURES = {
     "khx_idft_phase": (_i, [_dp, _dp, C.c_double, _dp, _dp, _i, _i, _dp, _vp]),
     "khx_joint_subspace": (_i, [_dp, _i, _i, _i, _i, _dp, _vp]),
     "khx_joint_subspace_diag": (_i, [_dp, _i, _i, _i, _dp, _vp]),
+    "khx_zgemm_op": (_i, [_i, _i, _i, _i, _i, _dp, _dp, _i, C.c_longlong, _dp, _i, C.c_longlong, _dp, _dp, _i,
+                          C.c_longlong, _i, _vp]),
+    "khx_conj_transpose": (_i, [_dp, _i, C.c_longlong, _i, _i, _dp, _i, C.c_longlong, _i, _vp]),
+    "khx_reduce_dot": (_i, [_dp, _i, C.c_longlong, _dp, _i, C.c_longlong, _i, _i, _i, _i, _dp, C.c_longlong, _i,
+                            _vp]),
+    "khx_eig_adjoint_mix": (_i, [_dp, _dp, _dp, _i, _i, C.c_double, _vp]),
+    "khx_lambda_adjoint": (_i, [_dp, _dp, _dp, _dp, _i, _i, _dp, _vp]),
+    "khx_build_pq": (_i, [_dp, _dp, _i, _dp, _dp, _i, _i, _dp, _dp, _vp]),
+    "khx_pq_adjoint": (_i, [_dp, _dp, _dp, _dp, _i, _i, _dp, _dp, _vp]),
+    "khx_convmat_adjoint_ws_bytes": (_sz, [_i, _i, _i]),
+    "khx_convmat_adjoint": (_i, [_dp, _i, _i, _i, _i, _i, _i, _dp, _vp, _vp]),
+    "khx_poynting_vec_adjoint": (_i, [_dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _dp, _dp, _vp]),
+    "khx_patterned_layer_bwd_ws_bytes": (_sz, [_i, _i]),
+    "khx_patterned_layer_bwd": (_i, [_dp, _dp, _i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _ip,
+                                     _vp, _vp]),
+    "khx_redheffer_bwd_ws_bytes": (_sz, [_i, _i]),
+    "khx_redheffer_bwd": (_i, [_dp, _dp, _i, _i, _dp, _dp, _dp, _ip, _vp, _vp]),
     "khx_prof_enable": (_i, [_i]),
     "khx_prof_collect": (_i, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "khx_copy": (_i, [_vp, _vp, _sz, _vp]),

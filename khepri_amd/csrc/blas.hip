@@ -14,8 +14,9 @@ namespace khx {
 // One workgroup = one BM x BN tile of one batch item; 256 threads arranged
 // (BM/TM) x (BN/TN), each owning a TM x TN register tile.  Operand tiles are
 // staged in LDS, A transposed so a wave reads a few broadcast rows of A and
-// consecutive columns of B.
-template <int BM, int BN, int BK, int TM, int TN, bool TA>
+// consecutive columns of B.  OA / OB select op(X) = X (0), X^T (1) or X^H (2);
+// a transposed operand is stored with the other dimension first.
+template <int BM, int BN, int BK, int TM, int TN, int OA, int OB>
 __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, cd alpha,
                                                     const cd* __restrict__ A, int lda, long long sA,
                                                     const cd* __restrict__ B, int ldb, long long sB,
@@ -48,17 +49,22 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, cd alph
     // stage A tile (BM x BK) -> As[k][m]; op(A) = A^T reads A's rows along m
 #pragma unroll
     for (int e = tid; e < BM * BK; e += 256) {
-      const int r = TA ? e % BM : e / BK, kk = TA ? e / BM : e % BK;
+      const int r = OA ? e % BM : e / BK, kk = OA ? e / BM : e % BK;
       const int gr = row0 + r, gk = k0 + kk;
-      const long long off = TA ? (long long)gk * lda + gr : (long long)gr * lda + gk;
-      As[kk][r] = (gr < M && gk < K) ? a[off] : mkc(0.0, 0.0);
+      const long long off = OA ? (long long)gk * lda + gr : (long long)gr * lda + gk;
+      cd v = (gr < M && gk < K) ? a[off] : mkc(0.0, 0.0);
+      if (OA == 2) v.y = -v.y;
+      As[kk][r] = v;
     }
     // stage B tile (BK x BN)
 #pragma unroll
     for (int e = tid; e < BK * BN; e += 256) {
-      const int kk = e / BN, cc = e % BN;
+      const int kk = OB ? e % BK : e / BN, cc = OB ? e / BK : e % BN;
       const int gk = k0 + kk, gc = col0 + cc;
-      Bs[kk][cc] = (gk < K && gc < N) ? b[(long long)gk * ldb + gc] : mkc(0.0, 0.0);
+      const long long off = OB ? (long long)gc * ldb + gk : (long long)gk * ldb + gc;
+      cd v = (gk < K && gc < N) ? b[off] : mkc(0.0, 0.0);
+      if (OB == 2) v.y = -v.y;
+      Bs[kk][cc] = v;
     }
     __syncthreads();
 #pragma unroll 4
@@ -93,31 +99,51 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, cd alph
   }
 }
 
-template <int BM, int BN, int BK, int TM, int TN, bool TA>
+template <int BM, int BN, int BK, int TM, int TN, int OA, int OB>
 static int launch_zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C,
                         int batch) {
   const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
   const long long blocks = (long long)tilesM * tilesN * batch;
   KHX_REQUIRE(blocks < (1ll << 31), "zgemm: grid too large");
   ProfScope ps("zgemm_kernel", s);
-  zgemm_kernel<BM, BN, BK, TM, TN, TA><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+  zgemm_kernel<BM, BN, BK, TM, TN, OA, OB><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
       M, N, K, alpha, A.p, A.ld, A.stride, B.p, B.ld, B.stride, beta, C.p, C.ld, C.stride, tilesN,
       tilesM * tilesN);
   KHX_LAUNCH_CHECK();
   return 0;
 }
 
-int zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
+template <int OA, int OB>
+static int zgemm_sized(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
   if (batch <= 0 || M <= 0 || N <= 0) return 0;
-  if (M <= 40 || N <= 40) return launch_zgemm<32, 32, 16, 2, 2, false>(s, M, N, K, alpha, A, B, beta, C, batch);
-  return launch_zgemm<64, 64, 16, 4, 4, false>(s, M, N, K, alpha, A, B, beta, C, batch);
+  if (M <= 40 || N <= 40) return launch_zgemm<32, 32, 16, 2, 2, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+  return launch_zgemm<64, 64, 16, 4, 4, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+}
+
+int zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
+  return zgemm_sized<0, 0>(s, M, N, K, alpha, A, B, beta, C, batch);
 }
 
 // C = alpha A^T B + beta C (A stored K x M)
 int zgemm_ta(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
-  if (batch <= 0 || M <= 0 || N <= 0) return 0;
-  if (M <= 40 || N <= 40) return launch_zgemm<32, 32, 16, 2, 2, true>(s, M, N, K, alpha, A, B, beta, C, batch);
-  return launch_zgemm<64, 64, 16, 4, 4, true>(s, M, N, K, alpha, A, B, beta, C, batch);
+  return zgemm_sized<1, 0>(s, M, N, K, alpha, A, B, beta, C, batch);
+}
+
+// C = alpha op(A) op(B) + beta C, op in {0: X, 1: X^T, 2: X^H}
+int zgemm_op(hipStream_t s, int opa, int opb, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C,
+             int batch) {
+  KHX_REQUIRE(opa >= 0 && opa <= 2 && opb >= 0 && opb <= 2, "zgemm_op: op must be 0 (N), 1 (T) or 2 (C)");
+  switch (opa * 3 + opb) {
+    case 0: return zgemm_sized<0, 0>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 1: return zgemm_sized<0, 1>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 2: return zgemm_sized<0, 2>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 3: return zgemm_sized<1, 0>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 4: return zgemm_sized<1, 1>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 5: return zgemm_sized<1, 2>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 6: return zgemm_sized<2, 0>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 7: return zgemm_sized<2, 1>(s, M, N, K, alpha, A, B, beta, C, batch);
+    default: return zgemm_sized<2, 2>(s, M, N, K, alpha, A, B, beta, C, batch);
+  }
 }
 
 // ======================================================================== LU solve

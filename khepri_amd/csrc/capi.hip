@@ -12,6 +12,19 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
 
 // defined in the other translation units
+size_t patterned_layer_bwd_ws_bytes(int n, int batch);
+int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstride, const cd* kx, const cd* ky,
+                        int N, int batch, const double* k0d, const cd* W, const cd* V, const cd* lam, const cd* Sbar,
+                        double eta_rel, cd* Cbar, int* info, void* ws);
+size_t star_bwd_ws_bytes(int n, int batch);
+int star_bwd(hipStream_t s, const cd* SA, long long sA, const cd* SB, long long sB, int n, int batch, const cd* Sbar,
+             cd* SAbar, cd* SBbar, int* info, void* ws);
+size_t convmat_adjoint_ws_bytes(int P, int Q, int batch);
+int convmat_adjoint(hipStream_t s, const cd* Cbar, int P, int Q, int nx, int ny, int batch, int cplx,
+                    double* epsbar, cd* Fws);
+int poynting_vec_adjoint(hipStream_t s, const cd* sR, const cd* sT, int N, const double* g, const cd* kp,
+                         const double* wl, cd epsi, cd epse, const double* wR, const double* wT, int batch,
+                         cd* sRbar, cd* sTbar);
 int kvectors(hipStream_t s, const double* g, int N, const cd* kp, const double* wl, int batch, cd* kx, cd* ky);
 int convmat(hipStream_t s, const double* eps, int cplx, int nx, int ny, int P, int Q, int count, cd* C, cd* Fws);
 size_t patterned_layer_ws_bytes(int n, int batch);
@@ -182,6 +195,115 @@ int khx_poynting_vec(const double* sR, const double* sT, int N, const double* g,
                            mkc(epsi_host[0], epsi_host[1]), mkc(epse_host[0], epse_host[1]), batch, R, T, Rord, Tord);
 }
 
+// ------------------------------------------------------------------ gradient
+int khx_zgemm_op(int opa, int opb, int M, int N, int K, const double* alpha, const double* A, int lda,
+                 long long strideA, const double* B, int ldb, long long strideB, const double* beta, double* C,
+                 int ldc, long long strideC, int batch, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(M > 0 && N > 0 && K > 0 && batch >= 0, "khx_zgemm_op: bad sizes");
+  KHX_REQUIRE(lda >= (opa ? M : K) && ldb >= (opb ? K : N) && ldc >= N, "khx_zgemm_op: leading dimensions too small");
+  const cd al = mkc(alpha[0], alpha[1]), be = mkc(beta[0], beta[1]);
+  return khx::zgemm_op(as_stream(stream), opa, opb, M, N, K, al, khx::Mat{const_cast<cd*>(as_cd(A)), lda, strideA},
+                       khx::Mat{const_cast<cd*>(as_cd(B)), ldb, strideB}, be, khx::Mat{as_cd(C), ldc, strideC},
+                       batch);
+}
+
+int khx_conj_transpose(const double* A, int lda, long long strideA, int rows, int cols, double* B, int ldb,
+                       long long strideB, int batch, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(rows >= 0 && cols >= 0 && batch >= 0 && lda >= cols && ldb >= rows, "khx_conj_transpose: bad sizes");
+  return khx::conj_transpose(as_stream(stream), khx::Mat{const_cast<cd*>(as_cd(A)), lda, strideA}, rows, cols,
+                             khx::Mat{as_cd(B), ldb, strideB}, batch);
+}
+
+int khx_reduce_dot(const double* A, int lda, long long strideA, const double* B, int ldb, long long strideB,
+                   int rows, int cols, int axis, int accumulate, double* out, long long ostride, int batch,
+                   void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(rows >= 0 && cols >= 0 && batch >= 0 && lda >= cols && ldb >= cols && (axis == 0 || axis == 1),
+              "khx_reduce_dot: bad arguments");
+  return khx::reduce_dot(as_stream(stream), khx::Mat{const_cast<cd*>(as_cd(A)), lda, strideA},
+                         khx::Mat{const_cast<cd*>(as_cd(B)), ldb, strideB}, rows, cols, axis, accumulate,
+                         as_cd(out), ostride, batch);
+}
+
+int khx_eig_adjoint_mix(double* G, const double* mu, const double* mubar, int n, int batch, double eta_rel,
+                        void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0 && eta_rel >= 0.0, "khx_eig_adjoint_mix: bad arguments");
+  return khx::eig_adjoint_mix(as_stream(stream), as_cd(G), as_cd(mu), as_cd(mubar), n, batch, eta_rel);
+}
+
+int khx_lambda_adjoint(const double* lam, const double* k0d, const double* xbar, const double* ilbar, int n,
+                       int batch, double* mubar, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0 && (xbar == nullptr || k0d != nullptr), "khx_lambda_adjoint: bad arguments");
+  return khx::lambda_adjoint(as_stream(stream), as_cd(lam), k0d, as_cd(xbar), as_cd(ilbar), n, batch, as_cd(mubar));
+}
+
+int khx_build_pq(const double* C, const double* IC, int per_item, const double* kx, const double* ky, int N,
+                 int batch, double* P, double* Q, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0, "khx_build_pq: bad sizes");
+  return khx::build_pq(as_stream(stream), as_cd(C), as_cd(IC), per_item ? (long long)N * N : 0, as_cd(kx), as_cd(ky),
+                       N, batch, as_cd(P), as_cd(Q));
+}
+
+int khx_pq_adjoint(const double* Pbar, const double* Qbar, const double* kx, const double* ky, int N, int batch,
+                   double* ICbar, double* Cbar, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0, "khx_pq_adjoint: bad sizes");
+  return khx::pq_adjoint(as_stream(stream), as_cd(Pbar), as_cd(Qbar), as_cd(kx), as_cd(ky), N, batch, as_cd(ICbar),
+                         as_cd(Cbar));
+}
+
+size_t khx_convmat_adjoint_ws_bytes(int P, int Q, int batch) { return khx::convmat_adjoint_ws_bytes(P, Q, batch); }
+
+int khx_convmat_adjoint(const double* Cbar, int P, int Q, int nx, int ny, int batch, int is_complex,
+                        double* epsbar, void* ws, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(batch >= 0 && ws != nullptr, "khx_convmat_adjoint: bad arguments");
+  return khx::convmat_adjoint(as_stream(stream), as_cd(Cbar), P, Q, nx, ny, batch, is_complex, epsbar,
+                              static_cast<cd*>(ws));
+}
+
+size_t khx_patterned_layer_bwd_ws_bytes(int n, int batch) { return khx::patterned_layer_bwd_ws_bytes(n, batch); }
+
+int khx_patterned_layer_bwd(const double* C, const double* IC, int per_item, const double* kx, const double* ky,
+                            int N, int batch, const double* k0d, const double* W, const double* V,
+                            const double* lam, const double* Sbar, double eta_rel, double* Cbar, int* info, void* ws,
+                            void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0 && ws && W && V && lam && Sbar && Cbar && eta_rel >= 0.0,
+              "khx_patterned_layer_bwd: bad arguments");
+  KHX_REQUIRE(batch <= 65535, "khx_patterned_layer_bwd: at most 65535 items per call");
+  return khx::patterned_layer_bwd(as_stream(stream), as_cd(C), as_cd(IC), per_item ? (long long)N * N : 0, as_cd(kx),
+                                  as_cd(ky), N, batch, k0d, as_cd(W), as_cd(V), as_cd(lam), as_cd(Sbar), eta_rel,
+                                  as_cd(Cbar), info, ws);
+}
+
+size_t khx_redheffer_bwd_ws_bytes(int n, int batch) { return khx::star_bwd_ws_bytes(n, batch); }
+
+int khx_redheffer_bwd(const double* SA, const double* SB, int n, int batch, const double* Sbar, double* SAbar,
+                      double* SBbar, int* info, void* ws, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0 && ws && SA && SB && Sbar && SAbar && SBbar, "khx_redheffer_bwd: bad arguments");
+  KHX_REQUIRE(batch <= 65535, "khx_redheffer_bwd: at most 65535 items per call");
+  const long long s4 = 4ll * n * n;
+  return khx::star_bwd(as_stream(stream), as_cd(SA), s4, as_cd(SB), s4, n, batch, as_cd(Sbar), as_cd(SAbar),
+                       as_cd(SBbar), info, ws);
+}
+
+int khx_poynting_vec_adjoint(const double* sR, const double* sT, int N, const double* g, const double* kp,
+                             const double* wl, const double* epsi_host, const double* epse_host, const double* wR,
+                             const double* wT, int batch, double* sRbar, double* sTbar, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0 && epsi_host && epse_host, "khx_poynting_vec_adjoint: bad arguments");
+  return khx::poynting_vec_adjoint(as_stream(stream), as_cd(sR), as_cd(sT), N, g, as_cd(kp), wl,
+                                   mkc(epsi_host[0], epsi_host[1]), mkc(epse_host[0], epse_host[1]), wR, wT, batch,
+                                   as_cd(sRbar), as_cd(sTbar));
+}
+
 int khx_lu_factor(int n, double* A, int batch, int* ipiv, int* info, void* stream) {
   KHX_ENTRY_BEGIN
   KHX_REQUIRE(n > 0 && batch >= 0, "khx_lu_factor: bad sizes");
@@ -201,6 +323,19 @@ int khx_diag_scale(const double* A, int lda, long long strideA, const double* dl
   KHX_REQUIRE(rows > 0 && cols > 0 && batch >= 0 && lda >= cols && ldo >= cols, "khx_diag_scale: bad sizes");
   return khx::zdiag_scale(as_stream(stream), khx::Mat{const_cast<cd*>(as_cd(A)), lda, strideA}, as_cd(dl),
                           as_cd(dr), rows, cols, batch, khx::Mat{as_cd(out), ldo, strideO});
+}
+
+int khx_zaxpby(double* D, int ldd, long long strideD, const double* a, const double* A, int lda, long long strideA,
+               const double* b, const double* B, int ldb, long long strideB, int rows, int cols, int batch,
+               void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(rows > 0 && cols > 0 && batch >= 0 && ldd >= cols && lda >= cols && (B == nullptr || ldb >= cols),
+              "khx_zaxpby: bad sizes");
+  const cd ca = mkc(a[0], a[1]), cb = mkc(b[0], b[1]);
+  KHX_REQUIRE(B != nullptr || (cb.x == 0.0 && cb.y == 0.0), "khx_zaxpby: B is NULL but b != 0");
+  return khx::zaxpby(as_stream(stream), khx::Mat{as_cd(D), ldd, strideD}, ca,
+                     khx::Mat{const_cast<cd*>(as_cd(A)), lda, strideA}, cb,
+                     khx::Mat{const_cast<cd*>(as_cd(B)), ldb, strideB}, rows, cols, batch);
 }
 
 int khx_propagation(const double* kx, const double* ky, int N, int batch, const double* k0d, double* X,

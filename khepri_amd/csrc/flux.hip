@@ -127,6 +127,49 @@ __global__ __launch_bounds__(256) void poynting_vec_kernel(const cd* __restrict_
   flux_reduce(accR, accT, R, T, b);
 }
 
+// Adjoint of order_flux with respect to s for an objective weight w:
+// t = a (|sx|^2 + |sy|^2 + |sz|^2), a = Re(kz) / Re(kzi), sz = -(kx sx + ky sy) / kz
+//   sxbar = 2 a w (sx - sz conj(kx / kz)),  sybar = 2 a w (sy - sz conj(ky / kz)).
+// Orders with Re(kz) = 0 carry no power and get a zero adjoint.
+__device__ __forceinline__ void order_flux_adjoint(const cd* s, int N, int h, double k0, cd kpx, cd kpy,
+                                                   const double* g, cd eps, cd kzin, double w, cd* sbar) {
+  const cd Kx = mkc(kpx.x + g[h], kpx.y), Ky = mkc(kpy.x + g[N + h], kpy.y);
+  const cd kx = mkc(Kx.x / k0, Kx.y / k0), ky = mkc(Ky.x / k0, Ky.y / k0);
+  const cd kza = conjc(sqrtc_p0(k0 * k0 * conjc(eps) - Kx * Kx - Ky * Ky));
+  const cd kz = mkc(kza.x / k0, kza.y / k0);
+  if (kz.x == 0.0 || w == 0.0) {
+    sbar[h] = mkc(0.0, 0.0);
+    sbar[N + h] = mkc(0.0, 0.0);
+    return;
+  }
+  const cd sx = s[h], sy = s[N + h];
+  const cd sz = -divc(kx * sx + ky * sy, kz);
+  const double a2 = 2.0 * w * kz.x / kzin.x;
+  sbar[h] = a2 * (sx - sz * conjc(divc(kx, kz)));
+  sbar[N + h] = a2 * (sy - sz * conjc(divc(ky, kz)));
+}
+
+// sRbar, sTbar (batch, n) for the objective sum_b wR[b] R[b] + wT[b] T[b].
+__global__ void poynting_vec_adjoint_kernel(const cd* __restrict__ sR, const cd* __restrict__ sT, int N,
+                                            const double* __restrict__ g, const cd* __restrict__ kp,
+                                            const double* __restrict__ wl, cd epsi, cd epse,
+                                            const double* __restrict__ wR, const double* __restrict__ wT, int batch,
+                                            cd* sRbar, cd* sTbar) {
+  const long long total = (long long)batch * N;
+  const int n = 2 * N;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long b = e / N;
+    const int h = (int)(e % N);
+    const double k0 = 2.0 * M_PI / wl[b];
+    const cd kpx = kp[2 * b], kpy = kp[2 * b + 1];
+    const cd kzi = kz_incident(k0, kpx, kpy, epsi);
+    const cd kzin = mkc(kzi.x / k0, kzi.y / k0);
+    order_flux_adjoint(sR + b * n, N, h, k0, kpx, kpy, g, epsi, kzin, wR ? wR[b] : 0.0, sRbar + b * n);
+    order_flux_adjoint(sT + b * n, N, h, k0, kpx, kpy, g, epse, kzin, wT ? wT[b] : 0.0, sTbar + b * n);
+  }
+}
+
 // Output amplitudes read from a resident Stot: S11 c1p and S21 c1p.
 __global__ void stot_amplitudes_kernel(const cd* __restrict__ S, long long sS, const cd* __restrict__ c, int N,
                                        int batch, cd* sR, cd* sT) {
@@ -165,6 +208,16 @@ int poynting_vec(hipStream_t s, const cd* sR, const cd* sT, int N, const double*
                  cd epsi, cd epse, int batch, double* R, double* T, double* Rord, double* Tord) {
   if (batch <= 0) return 0;
   poynting_vec_kernel<<<dim3(batch), dim3(256), 0, s>>>(sR, sT, N, g, kp, wl, epsi, epse, R, T, Rord, Tord);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+int poynting_vec_adjoint(hipStream_t s, const cd* sR, const cd* sT, int N, const double* g, const cd* kp,
+                         const double* wl, cd epsi, cd epse, const double* wR, const double* wT, int batch,
+                         cd* sRbar, cd* sTbar) {
+  if (batch <= 0) return 0;
+  poynting_vec_adjoint_kernel<<<grid_n((long long)batch * N), 256, 0, s>>>(sR, sT, N, g, kp, wl, epsi, epse, wR,
+                                                                          wT, batch, sRbar, sTbar);
   KHX_LAUNCH_CHECK();
   return 0;
 }

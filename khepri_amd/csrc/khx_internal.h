@@ -18,6 +18,9 @@ inline Mat sub(Mat m, int r, int c) { return Mat{m.p + (long long)r * m.ld + c, 
 int zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch);
 // C = alpha A^T (A stored K x M) * B + beta C
 int zgemm_ta(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch);
+// C = alpha op(A) op(B) + beta C with op 0 = X, 1 = X^T, 2 = X^H
+int zgemm_op(hipStream_t s, int opa, int opb, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C,
+             int batch);
 // Solve A X = B (A n x n destroyed, B n x m overwritten); info[b] = first zero pivot (1-based) or 0
 int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info);
 
@@ -32,6 +35,21 @@ int zcopy(hipStream_t s, Mat dst, Mat src, int rows, int cols, int batch);
 int zaxpby(hipStream_t s, Mat D, cd a, Mat A, cd b, Mat B, int rows, int cols, int batch);
 // D[i][j] = d[i] * A[i][j]   (d: batch x rows vector with stride dstride)
 int zrow_scale(hipStream_t s, Mat D, const cd* d, long long dstride, Mat A, int rows, int cols, int batch);
+
+// D = diag(dl) A diag(dr) (dl: batch x rows, dr: batch x cols; either may be null)
+int zdiag_scale(hipStream_t s, Mat A, const cd* dl, const cd* dr, int rows, int cols, int batch, Mat D);
+
+// adjoint helpers (adjoint.hip)
+int conj_transpose(hipStream_t s, Mat A, int rows, int cols, Mat B, int batch);
+int reduce_dot(hipStream_t s, Mat A, Mat B, int rows, int cols, int axis, int accumulate, cd* out,
+               long long ostride, int batch);
+int eig_adjoint_mix(hipStream_t s, cd* G, const cd* mu, const cd* mubar, int n, int batch, double eta_rel);
+int lambda_adjoint(hipStream_t s, const cd* lam, const double* k0d, const cd* xbar, const cd* ilbar, int n,
+                   int batch, cd* mubar);
+int build_pq(hipStream_t s, const cd* C, const cd* IC, long long cstride, const cd* kx, const cd* ky, int N,
+             int batch, cd* P, cd* Q);
+int pq_adjoint(hipStream_t s, const cd* Pbar, const cd* Qbar, const cd* kx, const cd* ky, int N, int batch,
+               cd* ICbar, cd* Cbar);
 
 // eigen-decomposition: A destroyed; lam (n per item), W (n x n, unit columns)
 size_t zeig_workspace_bytes(int n, int batch);

@@ -531,6 +531,229 @@ int analytic_layer(hipStream_t s, int kind, const cd* kx, const cd* ky, int N, i
   return 0;
 }
 
+
+// ================================================================= adjoint of a patterned layer
+// xc = conj(exp(-lam k0d)), ilc = conj(1 / lam), mu = lam^2
+__global__ void mode_vectors_kernel(const cd* __restrict__ lam, const double* __restrict__ k0d, int n, int batch,
+                                    cd* xc, cd* ilc, cd* mu) {
+  const long long total = (long long)batch * n;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const cd l = lam[e];
+    const double t = k0d[e / n];
+    xc[e] = conjc(expc(mkc(-l.x * t, -l.y * t)));
+    ilc[e] = conjc(recipc(l));
+    mu[e] = l * l;
+  }
+}
+
+// v[b][i] += J[b][i][i]  (J with row stride ld, item stride sJ)
+__global__ void diag_accum_kernel(const cd* __restrict__ J, int ld, long long sJ, int n, int batch, cd* v) {
+  const long long total = (long long)batch * n;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long b = e / n;
+    const int i = (int)(e % n);
+    v[e] = v[e] + J[b * sJ + (long long)i * ld + i];
+  }
+}
+
+// info[b] = -1 if any of the nwords LU status words of item b is nonzero
+__global__ void info_any_kernel(const int* __restrict__ infos, int batch, int nwords, int* out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  int v = 0;
+  for (int w = 0; w < nwords; ++w)
+    if (infos[(long long)w * batch + b] != 0) v = -1;
+  out[b] = v;
+}
+
+static long long patterned_bwd_items() { return 38; }
+
+size_t patterned_layer_bwd_ws_bytes(int n, int batch) {
+  return (size_t)patterned_bwd_items() * n * n * sizeof(cd) * batch + (size_t)8 * n * sizeof(cd) * batch +
+         (size_t)batch * sizeof(int) * 8 + 4096;
+}
+
+// Cbar (batch, N, N) = adjoint of the convolution matrix C given Sbar, the
+// adjoint of the layer S-matrix (batch, 2, 2, n, n).  W, V, lam are the
+// layer's modes (khx_patterned_layer outputs); the scattering-matrix
+// intermediates are recomputed here, not stored by the forward pass.
+// Forward steps and their adjoints (X = diag(x), x = exp(-lam k0d)):
+//   t1 = W^-1, t2 = V^-1 V0, A = t1 + t2, B = t1 - t2, [G1 | G2] = A^-1 [X | B],
+//   N1 = B G1, M = X N1, T = A - M B, H = A - B G2,
+//   [S11 | S12] = T^-1 [M A - B | X H], S22 = S11, S21 = S12;
+//   V = U diag(1/lam), U = Q W, lam = sqrt(mu), (mu, W) = eig(P Q),
+//   P, Q from C and IC = C^-1 (khepri/alternative.py:158-195).
+int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstride, const cd* kx, const cd* ky,
+                        int N, int batch, const double* k0d, const cd* W, const cd* V, const cd* lam, const cd* Sbar,
+                        double eta_rel, cd* Cbar, int* info, void* wsv) {
+  if (batch <= 0) return 0;
+  const int n = 2 * N;
+  const long long nn = (long long)n * n, n2 = 2 * nn, NN = (long long)N * N;
+  cd* w = static_cast<cd*>(wsv);
+  auto take = [&](long long items) {
+    cd* p = w;
+    w += items * nn * batch;
+    return p;
+  };
+  cd* P = take(1);
+  cd* Q = take(1);
+  cd* U = take(1);
+  cd* t1 = take(1);
+  cd* t2 = take(1);
+  cd* Am = take(1);
+  cd* Bm = take(1);
+  cd* AH = take(1);
+  cd* R1 = take(2);  // [G1 | G2], n x 2n
+  cd* N1 = take(1);
+  cd* Mm = take(1);
+  cd* Tm = take(1);
+  cd* TH = take(1);
+  cd* R2 = take(2);  // [S11 | S12]
+  cd* Hm = take(1);
+  cd* RB = take(2);  // [K1 | K2]
+  cd* Wc = take(1);
+  cd* Abar = take(1);
+  cd* Bbar = take(1);
+  cd* Tbar = take(1);
+  cd* Mbar = take(1);
+  cd* Hbar = take(1);
+  cd* N1bar = take(1);
+  cd* GJ = take(2);  // [G1bar | G2bar] -> [J1 | J2]
+  cd* t1bar = take(1);
+  cd* t2bar = take(1);
+  cd* Vbar = take(1);
+  cd* Wbar = take(1);
+  cd* Ubar = take(1);
+  cd* Qbar = take(1);
+  cd* Pbar = take(1);
+  cd* tmp = take(1);
+  cd* ICbar = take(1);  // N x N used
+  cd* x = w;
+  cd* xc = x + (long long)n * batch;
+  cd* ilc = xc + (long long)n * batch;
+  cd* mu = ilc + (long long)n * batch;
+  cd* xbar = mu + (long long)n * batch;
+  cd* ilbar = xbar + (long long)n * batch;
+  cd* mubar = ilbar + (long long)n * batch;
+  int* infos = reinterpret_cast<int*>(mubar + (long long)n * batch);
+
+  const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0), zero = mkc(0.0, 0.0);
+  const Mat mP{P, n, nn}, mQ{Q, n, nn}, mU{U, n, nn}, mW{const_cast<cd*>(W), n, nn}, mV{const_cast<cd*>(V), n, nn};
+  const Mat mt1{t1, n, nn}, mt2{t2, n, nn}, mA{Am, n, nn}, mB{Bm, n, nn}, mAH{AH, n, nn};
+  const Mat mG1{R1, 2 * n, n2}, mG2{R1 + n, 2 * n, n2}, mR1{R1, 2 * n, n2};
+  const Mat mN1{N1, n, nn}, mM{Mm, n, nn}, mT{Tm, n, nn}, mTH{TH, n, nn}, mH{Hm, n, nn}, mWc{Wc, n, nn};
+  const Mat mR2{R2, 2 * n, n2}, mR2L{R2, 2 * n, n2}, mR2R{R2 + n, 2 * n, n2};
+  const Mat mRB{RB, 2 * n, n2}, mK1{RB, 2 * n, n2}, mK2{RB + n, 2 * n, n2};
+  const Mat mAb{Abar, n, nn}, mBb{Bbar, n, nn}, mTb{Tbar, n, nn}, mMb{Mbar, n, nn}, mHb{Hbar, n, nn};
+  const Mat mN1b{N1bar, n, nn}, mGJ{GJ, 2 * n, n2}, mJ1{GJ, 2 * n, n2}, mJ2{GJ + n, 2 * n, n2};
+  const Mat mt1b{t1bar, n, nn}, mt2b{t2bar, n, nn}, mVb{Vbar, n, nn}, mWb{Wbar, n, nn}, mUb{Ubar, n, nn};
+  const Mat mQb{Qbar, n, nn}, mPb{Pbar, n, nn}, mtmp{tmp, n, nn};
+  auto sbar = [&](int q) { return Mat{const_cast<cd*>(Sbar) + q * nn, n, 4 * nn}; };
+  const long long nb = (long long)batch * n;
+
+  // ---- recompute the forward intermediates
+  KHX_TRY(build_pq(s, C, IC, cstride, kx, ky, N, batch, P, Q));
+  KHX_TRY(zgemm(s, n, n, n, one, mQ, mW, zero, mU, batch));
+  mode_vectors_kernel<<<grid_for(nb), 256, 0, s>>>(lam, k0d, n, batch, xc, ilc, mu);
+  KHX_LAUNCH_CHECK();
+  KHX_TRY(zcopy(s, mWc, mW, n, n, batch));
+  KHX_TRY(zset_identity(s, mt1, n, n, one, batch));
+  KHX_TRY(zgesv(s, n, n, mWc, mt1, batch, infos));
+  KHX_TRY(zcopy(s, mWc, mV, n, n, batch));
+  fill_v0_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(kx, ky, N, batch, t2, nn, n);
+  KHX_LAUNCH_CHECK();
+  KHX_TRY(zgesv(s, n, n, mWc, mt2, batch, infos + batch));
+  KHX_TRY(zaxpby(s, mA, one, mt1, one, mt2, n, n, batch));
+  KHX_TRY(zaxpby(s, mB, one, mt1, mone, mt2, n, n, batch));
+  KHX_TRY(conj_transpose(s, mA, n, n, mAH, batch));
+  x_rhs_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(lam, k0d, n, batch, Bm, nn, R1, n2, x);
+  KHX_LAUNCH_CHECK();
+  KHX_TRY(zcopy(s, mWc, mA, n, n, batch));
+  KHX_TRY(zgesv(s, n, 2 * n, mWc, mR1, batch, infos + 2 * batch));
+  KHX_TRY(zgemm(s, n, n, n, one, mB, mG1, zero, mN1, batch));
+  KHX_TRY(zrow_scale(s, mM, x, n, mN1, n, n, batch));
+  KHX_TRY(zcopy(s, mT, mA, n, n, batch));
+  KHX_TRY(zgemm(s, n, n, n, mone, mM, mB, one, mT, batch));
+  KHX_TRY(zaxpby(s, mR2L, mone, mB, zero, Mat{nullptr, 0, 0}, n, n, batch));
+  KHX_TRY(zgemm(s, n, n, n, one, mM, mA, one, mR2L, batch));
+  KHX_TRY(zcopy(s, mH, mA, n, n, batch));
+  KHX_TRY(zgemm(s, n, n, n, mone, mB, mG2, one, mH, batch));
+  KHX_TRY(zrow_scale(s, mR2R, x, n, mH, n, n, batch));
+  KHX_TRY(conj_transpose(s, mT, n, n, mTH, batch));
+  KHX_TRY(zcopy(s, mWc, mT, n, n, batch));
+  KHX_TRY(zgesv(s, n, 2 * n, mWc, mR2, batch, infos + 3 * batch));
+
+  // ---- reverse sweep
+  // (1) [K1 | K2] = T^-H [S11bar + S22bar | S12bar + S21bar]
+  KHX_TRY(zaxpby(s, mK1, one, sbar(0), one, sbar(3), n, n, batch));
+  KHX_TRY(zaxpby(s, mK2, one, sbar(1), one, sbar(2), n, n, batch));
+  KHX_TRY(zgesv(s, n, 2 * n, mTH, mRB, batch, infos + 4 * batch));
+  // (2) Tbar = -[K1 | K2] [S11 | S12]^H
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, 2 * n, mone, mRB, mR2, zero, mTb, batch));
+  // (3) right block X H: xbar = rowdot(K2, H), Hbar = conj(X) K2; H = A - B G2
+  KHX_TRY(reduce_dot(s, mK2, mH, n, n, 1, 0, xbar, n, batch));
+  KHX_TRY(zrow_scale(s, mHb, xc, n, mK2, n, n, batch));
+  KHX_TRY(zcopy(s, mAb, mHb, n, n, batch));
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, n, mone, mHb, mG2, zero, mBb, batch));
+  KHX_TRY(zgemm_op(s, 2, 0, n, n, n, mone, mB, mHb, zero, Mat{GJ + n, 2 * n, n2}, batch));
+  // (4) left block M A - B
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, n, one, mK1, mA, zero, mMb, batch));
+  KHX_TRY(zgemm_op(s, 2, 0, n, n, n, one, mM, mK1, one, mAb, batch));
+  KHX_TRY(zaxpby(s, mBb, one, mBb, mone, mK1, n, n, batch));
+  // (5) T = A - M B
+  KHX_TRY(zaxpby(s, mAb, one, mAb, one, mTb, n, n, batch));
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, n, mone, mTb, mB, one, mMb, batch));
+  KHX_TRY(zgemm_op(s, 2, 0, n, n, n, mone, mM, mTb, one, mBb, batch));
+  // (6) M = X N1, N1 = B G1
+  KHX_TRY(reduce_dot(s, mMb, mN1, n, n, 1, 1, xbar, n, batch));
+  KHX_TRY(zrow_scale(s, mN1b, xc, n, mMb, n, n, batch));
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, n, one, mN1b, mG1, one, mBb, batch));
+  KHX_TRY(zgemm_op(s, 2, 0, n, n, n, one, mB, mN1b, zero, mGJ, batch));
+  // (7) [G1 | G2] = A^-1 [X | B]
+  KHX_TRY(zgesv(s, n, 2 * n, mAH, mGJ, batch, infos + 5 * batch));
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, 2 * n, mone, mGJ, mR1, one, mAb, batch));
+  diag_accum_kernel<<<grid_for(nb), 256, 0, s>>>(GJ, 2 * n, n2, n, batch, xbar);
+  KHX_LAUNCH_CHECK();
+  KHX_TRY(zaxpby(s, mBb, one, mBb, one, mJ2, n, n, batch));
+  // (8) A = t1 + t2, B = t1 - t2
+  KHX_TRY(zaxpby(s, mt1b, one, mAb, one, mBb, n, n, batch));
+  KHX_TRY(zaxpby(s, mt2b, one, mAb, mone, mBb, n, n, batch));
+  // (9) t2 = V^-1 V0: Vbar = -(V^-H t2bar) t2^H
+  KHX_TRY(conj_transpose(s, mV, n, n, mWc, batch));
+  KHX_TRY(zgesv(s, n, n, mWc, mt2b, batch, infos + 6 * batch));
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, n, mone, mt2b, mt2, zero, mVb, batch));
+  // (10) t1 = W^-1: Wbar = -t1^H t1bar t1^H
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, n, one, mt1b, mt1, zero, mtmp, batch));
+  KHX_TRY(zgemm_op(s, 2, 0, n, n, n, mone, mt1, mtmp, zero, mWb, batch));
+  // (11) V = U diag(1/lam), U = Q W
+  KHX_TRY(zdiag_scale(s, mVb, nullptr, ilc, n, n, batch, mUb));
+  KHX_TRY(reduce_dot(s, mVb, mU, n, n, 0, 0, ilbar, n, batch));
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, n, one, mUb, mW, zero, mQb, batch));
+  KHX_TRY(zgemm_op(s, 2, 0, n, n, n, one, mQ, mUb, one, mWb, batch));
+  // (12) lam = sqrt(mu), x = exp(-lam k0d), 1/lam
+  KHX_TRY(lambda_adjoint(s, lam, k0d, xbar, ilbar, n, batch, mubar));
+  // (13) eigen-decomposition: (PQ)bar = W^-H [(W^H Wbar) o F + diag(mubar)] W^H, W^-H = t1^H
+  KHX_TRY(zgemm_op(s, 2, 0, n, n, n, one, mW, mWb, zero, mtmp, batch));
+  KHX_TRY(eig_adjoint_mix(s, tmp, mu, mubar, n, batch, eta_rel));
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, n, one, mtmp, mW, zero, mUb, batch));
+  KHX_TRY(zgemm_op(s, 2, 0, n, n, n, one, mt1, mUb, zero, mTb, batch));
+  // (14) PQ = P Q
+  KHX_TRY(zgemm_op(s, 0, 2, n, n, n, one, mTb, mQ, zero, mPb, batch));
+  KHX_TRY(zgemm_op(s, 2, 0, n, n, n, one, mP, mTb, one, mQb, batch));
+  // (15) P, Q from IC, C;  (16) IC = C^-1: Cbar -= IC^H ICbar IC^H
+  KHX_TRY(pq_adjoint(s, Pbar, Qbar, kx, ky, N, batch, ICbar, Cbar));
+  const Mat mIC{const_cast<cd*>(IC), N, cstride}, mICb{ICbar, N, NN}, mCb{Cbar, N, NN}, mtN{Hbar, N, NN};
+  KHX_TRY(zgemm_op(s, 0, 2, N, N, N, one, mICb, mIC, zero, mtN, batch));
+  KHX_TRY(zgemm_op(s, 2, 0, N, N, N, mone, mIC, mtN, one, mCb, batch));
+  if (info) {
+    info_any_kernel<<<dim3((batch + 255) / 256), 256, 0, s>>>(infos, batch, 7, info);
+    KHX_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
 }  // namespace khx
 
 namespace khx {

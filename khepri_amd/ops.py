@@ -103,12 +103,16 @@ def _mat_view(t, name):
     return t.stride(1), sb, t.shape[0]
 
 
-def matmul(A, B, alpha=1.0, beta=0.0, C=None):
-    """alpha A B + beta C for strided batches (views of S-matrix blocks welcome);
-    a 2-D operand is shared by the batch.  C, if given, is written in place."""
+_OPS = {"N": 0, "T": 1, "C": 2}
+
+
+def matmul(A, B, alpha=1.0, beta=0.0, C=None, opa="N", opb="N"):
+    """alpha op(A) op(B) + beta C for strided batches (views of S-matrix blocks
+    welcome); op is "N" (as is), "T" (transpose) or "C" (conjugate transpose).
+    A 2-D operand is shared by the batch.  C, if given, is written in place."""
     lib = _lib.load()
-    M, K = A.shape[-2:]
-    N = B.shape[-1]
+    M, K = A.shape[-2:] if opa == "N" else A.shape[-2:][::-1]
+    N = B.shape[-1] if opb == "N" else B.shape[-2]
     lda, sa, ba = _mat_view(A, "A")
     ldb, sb, bb = _mat_view(B, "B")
     batch = max(ba, bb)
@@ -121,9 +125,109 @@ def matmul(A, B, alpha=1.0, beta=0.0, C=None):
         raise ValueError("C must hold one matrix per batch item")
     al = np.array([complex(alpha).real, complex(alpha).imag])
     be = np.array([complex(beta).real, complex(beta).imag])
-    check(lib.khx_zgemm(M, N, K, al.ctypes.data, _p(A), lda, sa, _p(B), ldb, sb, be.ctypes.data, _p(C), ldc, sc,
-                        batch, _stream()), "khx_zgemm")
+    if opa == "N" and opb == "N":
+        check(lib.khx_zgemm(M, N, K, al.ctypes.data, _p(A), lda, sa, _p(B), ldb, sb, be.ctypes.data, _p(C), ldc,
+                            sc, batch, _stream()), "khx_zgemm")
+    else:
+        check(lib.khx_zgemm_op(_OPS[opa], _OPS[opb], M, N, K, al.ctypes.data, _p(A), lda, sa, _p(B), ldb, sb,
+                               be.ctypes.data, _p(C), ldc, sc, batch, _stream()), "khx_zgemm_op")
     return C
+
+
+def conj_transpose(A, out=None):
+    """A^H of a (b, rows, cols) strided batch -> (b, cols, rows)."""
+    lib = _lib.load()
+    rows, cols = A.shape[-2:]
+    lda, sa, ba = _mat_view(A, "A")
+    if out is None:
+        out = torch.empty((ba, cols, rows), dtype=CDT, device=A.device)
+    ldb, sb, _ = _mat_view(out, "out")
+    check(lib.khx_conj_transpose(_p(A), lda, sa, rows, cols, _p(out), ldb, sb, ba, _stream()), "khx_conj_transpose")
+    return out
+
+
+def reduce_dot(A, B, axis, out=None, accumulate=False):
+    """sum over the other axis of A * conj(B): (b, rows) for axis 1, (b, cols) for axis 0."""
+    lib = _lib.load()
+    rows, cols = A.shape[-2:]
+    lda, sa, ba = _mat_view(A, "A")
+    ldb, sb, bb = _mat_view(B, "B")
+    batch = max(ba, bb)
+    nout = rows if axis == 1 else cols
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs out")
+        out = torch.empty((batch, nout), dtype=CDT, device=A.device)
+    if out.stride(-1) != 1 or out.shape[-1] != nout:
+        raise ValueError("out: (batch, outputs) with contiguous rows")
+    check(lib.khx_reduce_dot(_p(A), lda, sa, _p(B), ldb, sb, rows, cols, axis, int(accumulate), _p(out),
+                             out.stride(0), batch, _stream()), "khx_reduce_dot")
+    return out
+
+
+def eig_adjoint_mix(G, mu, mubar, eta_rel):
+    lib = _lib.load()
+    _require(G, CDT, "G")
+    b, n, _ = G.shape
+    check(lib.khx_eig_adjoint_mix(_p(G), _p(mu.contiguous()), _p(mubar.contiguous()), n, b, float(eta_rel),
+                                  _stream()), "khx_eig_adjoint_mix")
+    return G
+
+
+def lambda_adjoint(lam, k0d, xbar=None, ilbar=None):
+    lib = _lib.load()
+    b, n = lam.shape
+    mubar = torch.empty_like(lam)
+    check(lib.khx_lambda_adjoint(_p(lam), _p(k0d), _p(xbar), _p(ilbar), n, b, _p(mubar), _stream()),
+          "khx_lambda_adjoint")
+    return mubar
+
+
+def build_pq(C, IC, kx, ky):
+    lib = _lib.load()
+    B, N = kx.shape
+    n = 2 * N
+    per_item = int(C.dim() == 3)
+    P = torch.empty((B, n, n), dtype=CDT, device=kx.device)
+    Q = torch.empty_like(P)
+    check(lib.khx_build_pq(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(P), _p(Q), _stream()), "khx_build_pq")
+    return P, Q
+
+
+def pq_adjoint(Pbar, Qbar, kx, ky):
+    lib = _lib.load()
+    B, N = kx.shape
+    ICbar = torch.empty((B, N, N), dtype=CDT, device=kx.device)
+    Cbar = torch.empty_like(ICbar)
+    check(lib.khx_pq_adjoint(_p(Pbar), _p(Qbar), _p(kx), _p(ky), N, B, _p(ICbar), _p(Cbar), _stream()),
+          "khx_pq_adjoint")
+    return ICbar, Cbar
+
+
+def convmat_adjoint(Cbar, pw, shape, is_complex=False):
+    """Pixel adjoint (b, nx, ny) (float64, or complex128 when is_complex) of Cbar (b, N, N)."""
+    lib = _lib.load()
+    _require(Cbar, CDT, "Cbar")
+    b = Cbar.shape[0]
+    nx, ny = shape
+    out = torch.empty((b, nx, ny), dtype=CDT if is_complex else RDT, device=Cbar.device)
+    ws = _ws(lib.khx_convmat_adjoint_ws_bytes(pw[0], pw[1], b), Cbar.device)
+    check(lib.khx_convmat_adjoint(_p(Cbar), pw[0], pw[1], nx, ny, b, int(is_complex), _p(out), _p(ws), _stream()),
+          "khx_convmat_adjoint")
+    return out
+
+
+def poynting_vec_adjoint(sR, sT, g, kp, wl, epsi, epse, wR=None, wT=None):
+    lib = _lib.load()
+    B, n = sT.shape
+    ei = np.array([complex(epsi).real, complex(epsi).imag])
+    ee = np.array([complex(epse).real, complex(epse).imag])
+    sRbar = torch.empty_like(sR)
+    sTbar = torch.empty_like(sT)
+    check(lib.khx_poynting_vec_adjoint(_p(sR), _p(sT), n // 2, _p(g), _p(kp), _p(wl), ei.ctypes.data, ee.ctypes.data,
+                                       _p(wR), _p(wT), B, _p(sRbar), _p(sTbar), _stream()),
+          "khx_poynting_vec_adjoint")
+    return sRbar, sTbar
 
 
 def eig(A):
@@ -203,6 +307,39 @@ def redheffer(SA, SB, out=None, ws=None):
     check(lib.khx_redheffer(_p(SA.contiguous()), int(ab), _p(SB.contiguous()), int(bb), n, B, _p(S), _p(info),
                             _p(ws), _stream()), "khx_redheffer")
     return S, info
+
+
+def patterned_layer_bwd(C, IC, kx, ky, k0d, W, V, lam, Sbar, eta_rel=1e-12):
+    """Cbar (B, N, N): adjoint of the convolution matrix of a patterned layer
+    given the adjoint Sbar (B, 2, 2, n, n) of its S-matrix and its modes."""
+    lib = _lib.load()
+    B, N = kx.shape
+    n = 2 * N
+    dev = kx.device
+    per_item = 1 if C.dim() == 3 else 0
+    C, IC = C.contiguous(), IC.contiguous()
+    Cbar = torch.empty((B, N, N), dtype=CDT, device=dev)
+    info = torch.empty(B, dtype=torch.int32, device=dev)
+    ws = _ws(lib.khx_patterned_layer_bwd_ws_bytes(n, B), dev)
+    check(lib.khx_patterned_layer_bwd(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(W.contiguous()),
+                                      _p(V.contiguous()), _p(lam.contiguous()), _p(Sbar.contiguous()),
+                                      float(eta_rel), _p(Cbar), _p(info), _p(ws), _stream()),
+          "khx_patterned_layer_bwd")
+    return Cbar, info
+
+
+def redheffer_bwd(SA, SB, Sbar):
+    """(SAbar, SBbar, info) for S = SA * SB (both (B, 2, 2, n, n))."""
+    lib = _lib.load()
+    B, n = SA.shape[0], SA.shape[-1]
+    dev = SA.device
+    SAbar = torch.empty((B, 2, 2, n, n), dtype=CDT, device=dev)
+    SBbar = torch.empty_like(SAbar)
+    info = torch.empty(B, dtype=torch.int32, device=dev)
+    ws = _ws(lib.khx_redheffer_bwd_ws_bytes(n, B), dev)
+    check(lib.khx_redheffer_bwd(_p(SA.contiguous()), _p(SB.contiguous()), n, B, _p(Sbar.contiguous()), _p(SAbar),
+                                _p(SBbar), _p(info), _p(ws), _stream()), "khx_redheffer_bwd")
+    return SAbar, SBbar, info
 
 
 def poynting(Stot, g, kp, wl, te, tm, epsi, epse, orders=False):
@@ -285,6 +422,26 @@ def diag_scale(A, dl=None, dr=None, out=None):
     dr = None if dr is None else dr.contiguous()
     check(lib.khx_diag_scale(_p(A), lda, sa, _p(dl), _p(dr), rows, cols, b, _p(out), ldo, so, _stream()),
           "khx_diag_scale")
+    return out
+
+
+def axpby(a, A, b=0.0, B=None, out=None):
+    """a A + b B for strided batches of matrices (out may alias A or B)."""
+    lib = _lib.load()
+    rows, cols = A.shape[-2:]
+    lda, sa, ba = _mat_view(A, "A")
+    if B is not None:
+        ldb, sb, bb = _mat_view(B, "B")
+    else:
+        ldb, sb, bb = cols, 0, ba
+    batch = max(ba, bb)
+    if out is None:
+        out = torch.empty((batch, rows, cols), dtype=CDT, device=A.device)
+    ldd, sd, _ = _mat_view(out, "out")
+    al = np.array([complex(a).real, complex(a).imag])
+    be = np.array([complex(b).real, complex(b).imag])
+    check(lib.khx_zaxpby(_p(out), ldd, sd, al.ctypes.data, _p(A), lda, sa, be.ctypes.data, _p(B), ldb, sb, rows, cols,
+                         batch, _stream()), "khx_zaxpby")
     return out
 
 
