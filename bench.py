@@ -80,6 +80,33 @@ def cpu_baseline(cfg, gpu_R, gpu_T, idx_all, sample):
                        f"reference op sequence incl. reverse star chain), 1 BLAS thread, {el:.1f} s"), dR, dT
 
 
+def plain_batch1(cfg, idx_all, sample):
+    """The plain FP64 path one solve at a time (batch 1, no batching across
+    sources): the same kernels and algorithm, timed on an evenly spread
+    sample of the workload."""
+    import torch
+    from khepri_amd import Crystal
+    pos = np.linspace(0, len(idx_all) - 1, sample).astype(int)
+    cl = Crystal(cfg["pw"])
+    cl.add_layer_uniform("S1", cfg["eps_gap"], cfg["depth_gap"])
+    cl.add_layer_pixmap("Scyl", cfg["pixmap"], cfg["depth_disc"])
+    cl.set_device(["Scyl", "S1", "Scyl"], False)
+    s0 = idx_all[pos[0]]
+    cl.set_source(1 / cfg["freqs"][s0], 1.0, 0.0, kp=tuple(cfg["kp"][s0]))
+    cl.solve()
+    cl.poynting_flux_end()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in pos:
+        s = idx_all[p]
+        cl.set_source(1 / cfg["freqs"][s], 1.0, 0.0, kp=tuple(cfg["kp"][s]))
+        cl.solve()
+        cl.poynting_flux_end()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return dict(value=round(sample / el, 3), unit="solves/s", sample=f"{sample} solves one at a time, {el:.2f} s")
+
+
 def extra_workload(name, args):
     """Secondary measurements (not the driver's line): returns a JSON dict."""
     import torch
@@ -180,6 +207,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--solves", type=int, default=65536, help="subset of the workload (default: all)")
     ap.add_argument("--cpu-sample", type=int, default=40, help="oracle solves timed for cpu_baseline (0: skip)")
+    ap.add_argument("--b1-sample", type=int, default=24, help="solves timed one at a time for the plain batch-1 line")
     ap.add_argument("--workload", default="disc_stack_bz",
                     help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted | grad_grid")
     ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
@@ -249,6 +277,10 @@ def main():
         "roofline": roofline,
         "kernel_time_share": share,
     }
+    if args.b1_sample > 0:
+        pb = plain_batch1(cfg, idx, args.b1_sample)
+        out["plain_fp64_batch1"] = pb
+        out["speedup_vs_plain_batch1"] = round(out["value"] / pb["value"], 2)
     if args.cpu_sample > 0:
         Rn, Tn = np.asarray(R), np.asarray(T)
         cb, dR, dT = cpu_baseline(cfg, Rn, Tn, idx, args.cpu_sample)

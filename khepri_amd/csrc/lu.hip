@@ -156,6 +156,77 @@ __global__ __launch_bounds__(LU_THREADS) void zgetrs_kernel(int n, int m, const 
   }
 }
 
+// Few right-hand sides (m <= ZGETRS_VEC_MAX): one wavefront per system.  The
+// right-hand sides live in LDS; each substitution step is a row of L or U
+// (contiguous, read once) dotted with the solved part, reduced across the
+// wave -- no workgroup barrier on the sequential path.
+constexpr int ZGETRS_VEC_MAX = 4;
+
+__global__ __launch_bounds__(64) void zgetrs_vec_kernel(int n, int m, const cd* A, int lda, long long sA,
+                                                        const int* ipiv, cd* B, int ldb, long long sB) {
+  extern __shared__ cd xs[];  // n x m, row-major
+  const long long item = blockIdx.x;
+  const cd* a = A + item * sA;
+  const int* pv = ipiv + item * n;
+  cd* bm = B + item * sB;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < n * m; e += 64) xs[e] = bm[(long long)(e / m) * ldb + e % m];
+  __syncthreads();
+  if (lane == 0)
+    for (int k = 0; k < n; ++k) {
+      const int p = pv[k];
+      if (p != k)
+        for (int j = 0; j < m; ++j) {
+          const cd t = xs[k * m + j];
+          xs[k * m + j] = xs[p * m + j];
+          xs[p * m + j] = t;
+        }
+    }
+  __syncthreads();
+  // L y = P b (unit diagonal): y_i = b_i - sum_{k<i} L_ik y_k
+  for (int i = 1; i < n; ++i) {
+    const cd* row = a + (long long)i * lda;
+    cd acc[ZGETRS_VEC_MAX];
+#pragma unroll
+    for (int j = 0; j < ZGETRS_VEC_MAX; ++j) acc[j] = mkc(0.0, 0.0);
+    for (int k = lane; k < i; k += 64) {
+      const cd l = row[k];
+#pragma unroll
+      for (int j = 0; j < ZGETRS_VEC_MAX; ++j)
+        if (j < m) acc[j] = fmac(acc[j], l, xs[k * m + j]);
+    }
+#pragma unroll
+    for (int j = 0; j < ZGETRS_VEC_MAX; ++j) {
+      if (j >= m) break;
+      const double re = wave_sum(acc[j].x), im = wave_sum(acc[j].y);
+      if (lane == 0) xs[i * m + j] = xs[i * m + j] - mkc(re, im);
+    }
+    __syncthreads();
+  }
+  // U x = y: x_i = (y_i - sum_{k>i} U_ik x_k) / U_ii
+  for (int i = n - 1; i >= 0; --i) {
+    const cd* row = a + (long long)i * lda;
+    cd acc[ZGETRS_VEC_MAX];
+#pragma unroll
+    for (int j = 0; j < ZGETRS_VEC_MAX; ++j) acc[j] = mkc(0.0, 0.0);
+    for (int k = i + 1 + lane; k < n; k += 64) {
+      const cd u = row[k];
+#pragma unroll
+      for (int j = 0; j < ZGETRS_VEC_MAX; ++j)
+        if (j < m) acc[j] = fmac(acc[j], u, xs[k * m + j]);
+    }
+    const cd uii = row[i];
+#pragma unroll
+    for (int j = 0; j < ZGETRS_VEC_MAX; ++j) {
+      if (j >= m) break;
+      const double re = wave_sum(acc[j].x), im = wave_sum(acc[j].y);
+      if (lane == 0) xs[i * m + j] = divc(xs[i * m + j] - mkc(re, im), uii);
+    }
+    __syncthreads();
+  }
+  for (int e = lane; e < n * m; e += 64) bm[(long long)(e / m) * ldb + e % m] = xs[e];
+}
+
 int zgetrf(hipStream_t s, int n, Mat A, int batch, int* ipiv, int* info) {
   if (batch <= 0 || n <= 0) return 0;
   if (n >= 24 && lu_blocked_fits(n) && getenv("KHX_LU_UNBLOCKED") == nullptr)
@@ -170,6 +241,13 @@ int zgetrf(hipStream_t s, int n, Mat A, int batch, int* ipiv, int* info) {
 
 int zgetrs(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch) {
   if (batch <= 0 || n <= 0 || m <= 0) return 0;
+  if (m <= ZGETRS_VEC_MAX && (size_t)n * m * sizeof(cd) <= 64 * 1024) {
+    ProfScope ps("zgetrs_kernel", s);
+    zgetrs_vec_kernel<<<dim3(batch), dim3(64), (size_t)n * m * sizeof(cd), s>>>(n, m, A.p, A.ld, A.stride, ipiv,
+                                                                              B.p, B.ld, B.stride);
+    KHX_LAUNCH_CHECK();
+    return 0;
+  }
   const size_t lds = (size_t)m * sizeof(cd);
   KHX_REQUIRE(lds <= 64 * 1024, "zgetrs: too many right-hand sides for the LDS row buffer");
   ProfScope ps("zgetrs_kernel", s);

@@ -182,7 +182,8 @@ def test_poynting_matches_oracle(gpu):
     np.testing.assert_allclose(Tog, To.real, atol=1e-14)
 
 
-@pytest.mark.parametrize("n,m,batch", [(1, 1, 2), (50, 1, 3), (162, 7, 2)])
+@pytest.mark.parametrize("n,m,batch", [(1, 1, 2), (50, 1, 3), (162, 7, 2), (162, 1, 3), (242, 3, 2), (97, 4, 2),
+                                       (1250, 1, 1)])
 def test_lu_factor_solve(gpu, n, m, batch):
     from khepri_amd import ops
     rng = np.random.default_rng(n + 100 * m)
