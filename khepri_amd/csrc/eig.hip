@@ -29,6 +29,15 @@ constexpr int EIG_THREADS_DEFAULT = 256;
 constexpr double ULP = 2.220446049250313e-16;
 constexpr double SAFMIN = 2.2250738585072014e-308;
 constexpr int KHX_EIG_LDS_N = 64;  // H resident in LDS up to this order
+constexpr int EIG_QB = 32;         // QR chase block held in LDS when H is in global memory
+
+// Elements of the LDS region shared by the Hessenberg partial sums and the
+// QR chase block.
+__host__ __device__ __forceinline__ int eig_part_elems(int n, int waves) {
+  const int a = waves * 2 * n;
+  if (n <= KHX_EIG_LDS_N) return a;
+  return a > EIG_QB * EIG_QB ? a : EIG_QB * EIG_QB;
+}
 
 template <int EIG_THREADS>
 __device__ __forceinline__ double block_sum(double v, double* red) {
@@ -158,11 +167,314 @@ __device__ __forceinline__ void slide(cd* v, int step, int l, int ihi, const dou
   v[ihi * step] = xv;
 }
 
+// Blocked Householder reduction to Hessenberg form for H in global memory
+// (the panel scheme of LAPACK zgehrd / zlahr2, with reflectors P = I - u u^H,
+// |u|^2 = 2).  For a panel of HB_NB columns starting at k0 the reflectors
+// form Q = P_0 ... P_{nb-1} = I - V T V^H, and Y = A V T is built column by
+// column from one matrix-vector product per reflector.  Each panel column is
+// brought up to date (right update by Y, left update by V T^H V^H) just
+// before its reflector is generated; after the panel the trailing columns get
+// A <- (I - V T^H V^H)(A - Y V^H) and the Schur-vector matrix Zt <- Q^T Zt,
+// each as one pass per column instead of one pass per reflector.
+// V and Y (n x HB_NB, row-major) live in global scratch; T, the partial sums
+// and the reflector in LDS.
+constexpr int HB_NB = 16;
+
+template <int EIG_THREADS>
+__device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, cd* u, cd* vh, cd* part,
+                                   double* red) {
+  constexpr int EIG_WAVES = EIG_THREADS / 64;
+  static_assert(64 % HB_NB == 0 && EIG_THREADS % HB_NB == 0, "panel width must divide the wavefront");
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  cd* Ts = part;                      // HB_NB x HB_NB, zero outside the panel's reflectors
+  cd* wsum = part + HB_NB * HB_NB;    // EIG_WAVES x HB_NB partial sums
+  cd* wv = wsum + EIG_WAVES * HB_NB;  // HB_NB reduced vector
+  cd* wt = wv + HB_NB;                // HB_NB transformed vector
+  // wv[q] = sum_{i >= r0} conj(V(i, q)) x_i for every q (columns of V past the
+  // current reflector are zero): thread -> (q = tid % HB_NB, every 16th row).
+  auto vh_dot = [&](const cd* x, long long xstride, int r0) {
+    const int q = tid % HB_NB;
+    cd acc = mkc(0.0, 0.0);
+    for (int i = r0 + tid / HB_NB; i < n; i += EIG_THREADS / HB_NB)
+      acc = fmac_cj(acc, V[(long long)i * HB_NB + q], x[(long long)i * xstride]);
+#pragma unroll
+    for (int o = HB_NB; o < 64; o <<= 1) {
+      acc.x += __shfl_xor(acc.x, o, 64);
+      acc.y += __shfl_xor(acc.y, o, 64);
+    }
+    if (lane < HB_NB) wsum[wid * HB_NB + lane] = acc;
+    __syncthreads();
+    if (tid < HB_NB) {
+      cd t = wsum[tid];
+#pragma unroll
+      for (int w = 1; w < EIG_WAVES; ++w) t = t + wsum[w * HB_NB + tid];
+      wv[tid] = t;
+    }
+    __syncthreads();
+  };
+
+  for (int k0 = 0; k0 + 2 < n; k0 += HB_NB) {
+    const int nbp = min(HB_NB, n - 2 - k0);  // reflectors of this panel
+    for (int e = tid; e < HB_NB * HB_NB; e += EIG_THREADS) Ts[e] = mkc(0.0, 0.0);
+    for (int e = tid; e < n * HB_NB; e += EIG_THREADS) {
+      V[e] = mkc(0.0, 0.0);
+      Y[e] = mkc(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int j = 0; j < nbp; ++j) {
+      const int c = k0 + j;
+      if (j > 0) {
+        // (1) right update of column c by the panel's previous reflectors: a -= Y V(c, :)^H
+        for (int i = tid; i < n; i += EIG_THREADS) {
+          const cd* yr = Y + (long long)i * HB_NB;
+          const cd* vc = V + (long long)c * HB_NB;
+          cd a = H[(long long)i * n + c];
+          for (int q = 0; q < j; ++q) a = fmsc(a, yr[q], conjc(vc[q]));
+          H[(long long)i * n + c] = a;
+        }
+        __syncthreads();
+        // (2) left update of rows k0+1.. of column c: x -= V T^H V^H x
+        vh_dot(H + c, n, k0 + 1);
+        if (tid < j) {
+          cd t = mkc(0.0, 0.0);
+          for (int p = 0; p <= tid; ++p) t = fmac_cj(t, Ts[p * HB_NB + tid], wv[p]);
+          wt[tid] = t;
+        }
+        __syncthreads();
+        for (int i = k0 + 1 + tid; i < n; i += EIG_THREADS) {
+          const cd* vr = V + (long long)i * HB_NB;
+          cd a = H[(long long)i * n + c];
+          for (int q = 0; q < j; ++q) a = fmsc(a, vr[q], wt[q]);
+          H[(long long)i * n + c] = a;
+        }
+        __syncthreads();
+      }
+      // (3) reflector from H[c+1.., c]
+      const int L = n - c - 1;
+      const cd alpha = H[(long long)(c + 1) * n + c];  // read before the barrier in block_sum
+      double part2 = 0.0;
+      for (int i = c + 2 + tid; i < n; i += EIG_THREADS) part2 += abs2c(H[(long long)i * n + c]);
+      const double rest2 = block_sum<EIG_THREADS>(part2, red);
+      const double aa = absc(alpha);
+      const double xn = sqrt(aa * aa + rest2);
+      const cd ph = aa != 0.0 ? (1.0 / aa) * alpha : mkc(1.0, 0.0);
+      const bool skip = (rest2 == 0.0);  // already reduced: P = I (u = 0)
+      const double sc = skip ? 0.0 : 1.0 / sqrt(xn * (xn + aa));
+      for (int i = c + 1 + tid; i < n; i += EIG_THREADS) {
+        cd x = H[(long long)i * n + c];
+        if (i == c + 1) x = x + xn * ph;
+        const cd v = sc * x;
+        u[i - c - 1] = v;
+        V[(long long)i * HB_NB + j] = v;
+        if (!skip) H[(long long)i * n + c] = (i == c + 1) ? -(xn * ph) : mkc(0.0, 0.0);
+      }
+      __syncthreads();
+      // (4) vh = H(:, c+1:) u  (panel-start columns), one wavefront per row
+      for (int i = wid; i < n; i += EIG_WAVES) {
+        const cd* hr = H + (long long)i * n + c + 1;
+        double sx = 0.0, sy = 0.0;
+        for (int q = lane; q < L; q += 64) {
+          const cd t = hr[q] * u[q];
+          sx += t.x;
+          sy += t.y;
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        if (lane == 0) vh[i] = mkc(sx, sy);
+      }
+      __syncthreads();
+      // (5) z = V(:, 0:j)^H u; T(0:j, j) = -T z, T(j, j) = 1; Y(:, j) = vh - Y(:, 0:j) z
+      if (j > 0) vh_dot(V + j, HB_NB, c + 1);
+      if (tid < j) {
+        cd t = mkc(0.0, 0.0);
+        for (int p = tid; p < j; ++p) t = fmsc(t, Ts[tid * HB_NB + p], wv[p]);
+        Ts[tid * HB_NB + j] = t;
+      } else if (tid == j) {
+        Ts[j * HB_NB + j] = mkc(skip ? 0.0 : 1.0, 0.0);
+      }
+      for (int i = tid; i < n; i += EIG_THREADS) {
+        const cd* yr = Y + (long long)i * HB_NB;
+        cd a = vh[i];
+        for (int q = 0; q < j; ++q) a = fmsc(a, yr[q], wv[q]);
+        Y[(long long)i * HB_NB + j] = a;
+      }
+      __syncthreads();
+    }
+    // Vt = V T (the left factor of both trailing updates), one thread per row
+    for (int i = tid; i < n; i += EIG_THREADS) {
+      const cd* vr = V + (long long)i * HB_NB;
+      cd* tr = Vt + (long long)i * HB_NB;
+#pragma unroll 1
+      for (int q = 0; q < HB_NB; ++q) {
+        cd t = mkc(0.0, 0.0);
+#pragma unroll 1
+        for (int p = 0; p <= q; ++p) t = fmac(t, vr[p], Ts[p * HB_NB + q]);
+        tr[q] = t;
+      }
+    }
+    __syncthreads();
+    // (6)-(7) trailing columns: A <- (I - V T^H V^H)(A - Y V^H) = (I - V Vt^H)(A - Y V^H), one thread per column
+    for (int col = k0 + nbp + tid; col < n; col += EIG_THREADS) {
+      cd w[HB_NB];
+#pragma unroll
+      for (int q = 0; q < HB_NB; ++q) w[q] = conjc(V[(long long)col * HB_NB + q]);
+#pragma unroll 1
+      for (int i = 0; i < n; ++i) {
+        const cd* yr = Y + (long long)i * HB_NB;
+        cd a = H[(long long)i * n + col];
+#pragma unroll
+        for (int q = 0; q < HB_NB; ++q) a = fmsc(a, yr[q], w[q]);
+        H[(long long)i * n + col] = a;
+      }
+#pragma unroll
+      for (int q = 0; q < HB_NB; ++q) w[q] = mkc(0.0, 0.0);
+#pragma unroll 1
+      for (int i = k0 + 1; i < n; ++i) {
+        const cd a = H[(long long)i * n + col];
+        const cd* tr = Vt + (long long)i * HB_NB;
+#pragma unroll
+        for (int q = 0; q < HB_NB; ++q) w[q] = fmac_cj(w[q], tr[q], a);
+      }
+#pragma unroll 1
+      for (int i = k0 + 1; i < n; ++i) {
+        const cd* vr = V + (long long)i * HB_NB;
+        cd a = H[(long long)i * n + col];
+#pragma unroll
+        for (int q = 0; q < HB_NB; ++q) a = fmsc(a, vr[q], w[q]);
+        H[(long long)i * n + col] = a;
+      }
+    }
+    // (8) Zt <- Q^T Zt = Zt - conj(V) Vt^T Zt  (rows k0+1..), one thread per column
+    for (int col = tid; col < n; col += EIG_THREADS) {
+      cd w[HB_NB];
+#pragma unroll
+      for (int q = 0; q < HB_NB; ++q) w[q] = mkc(0.0, 0.0);
+#pragma unroll 1
+      for (int i = k0 + 1; i < n; ++i) {
+        const cd z = Zt[(long long)i * n + col];
+        const cd* tr = Vt + (long long)i * HB_NB;
+#pragma unroll
+        for (int q = 0; q < HB_NB; ++q) w[q] = fmac(w[q], tr[q], z);
+      }
+#pragma unroll 1
+      for (int i = k0 + 1; i < n; ++i) {
+        const cd* vr = V + (long long)i * HB_NB;
+        cd z = Zt[(long long)i * n + col];
+#pragma unroll
+        for (int q = 0; q < HB_NB; ++q) z = fmsc(z, conjc(vr[q]), w[q]);
+        Zt[(long long)i * n + col] = z;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One implicit single-shift QR sweep over the window [l, ihi] with H in
+// global memory.  The bulge is chased through EIG_QB x EIG_QB diagonal blocks
+// staged in LDS: inside a block every rotation updates only the block (one
+// barrier per rotation, LDS latency); the block's rotations are logged and,
+// once the block is written back, applied to the rows above it, the columns
+// right of it and to Zt -- each thread sliding along one line.  Consecutive
+// blocks overlap by three indices so the next rotation finds its inputs
+// (the bulge) inside the new block.  Every element receives the same
+// rotations in the same order as an unblocked sweep.
+template <int EIG_THREADS>
+__device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift, cd* hb, double* crot, cd* srot,
+                                 cd* sxy) {
+  const int tid = threadIdx.x;
+  int kstart = l, b0 = l;
+  while (true) {
+    const int b1 = min(b0 + EIG_QB - 1, ihi);
+    const int nb = b1 - b0 + 1;
+    const int kend = (b1 == ihi) ? ihi - 1 : b1 - 2;  // last rotation whose bulge stays in the block
+    for (int e = tid; e < nb * nb; e += EIG_THREADS) hb[e] = H[(long long)(b0 + e / nb) * n + b0 + e % nb];
+    __syncthreads();
+    if (tid == 0) {
+      cd* xy = sxy + 2 * (kstart & 1);
+      if (kstart == l) {  // b0 == l
+        xy[0] = hb[0] - shift;
+        xy[1] = hb[nb];
+      } else {  // b0 == kstart - 1
+        xy[0] = hb[nb];
+        xy[1] = hb[2 * nb];
+      }
+    }
+    __syncthreads();
+    for (int k = kstart; k <= kend; ++k) {
+      const cd x = sxy[2 * (k & 1)], y = sxy[2 * (k & 1) + 1];
+      cd* nxt = sxy + 2 * ((k + 1) & 1);
+      double c;
+      cd s, r;
+      givens(x, y, c, s, r);
+      if (tid == 0) {
+        crot[k] = c;
+        srot[k] = s;
+      }
+      const int kk = k - b0;                                   // block coordinates
+      const int nrow = b1 - k - 1;                             // row part: block columns kk+2 .. nb-1
+      const int ncol = kk + ((k + 2 <= b1) ? 1 : 0);           // col part: block rows 0 .. kk-1 (+ kk+2)
+      const int total = 1 + nrow + ncol;
+      for (int e = tid; e < total; e += EIG_THREADS) {
+        if (e == 0) {
+          if (k > l) {
+            hb[kk * nb + kk - 1] = r;
+            hb[(kk + 1) * nb + kk - 1] = mkc(0.0, 0.0);
+          }
+          cd* p0 = hb + kk * nb + kk;
+          cd* p1 = hb + (kk + 1) * nb + kk;
+          cd b00 = p0[0], b01 = p0[1], b10 = p1[0], b11 = p1[1];
+          rot_rows(b00, b10, c, s);
+          rot_rows(b01, b11, c, s);
+          rot_cols(b00, b01, c, s);
+          rot_cols(b10, b11, c, s);
+          p0[0] = b00;
+          p0[1] = b01;
+          p1[0] = b10;
+          p1[1] = b11;
+          nxt[0] = b10;
+        } else if (e <= nrow) {
+          const int j = kk + 1 + e;
+          cd xk = hb[kk * nb + j], yk = hb[(kk + 1) * nb + j];
+          rot_rows(xk, yk, c, s);
+          hb[kk * nb + j] = xk;
+          hb[(kk + 1) * nb + j] = yk;
+        } else {
+          const int q = e - nrow - 1;
+          const int i = (q < kk) ? q : kk + 2;
+          cd* pr = hb + i * nb + kk;
+          cd xk = pr[0], yk = pr[1];
+          rot_cols(xk, yk, c, s);
+          pr[0] = xk;
+          pr[1] = yk;
+          if (i == kk + 2) nxt[1] = xk;
+        }
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < nb * nb; e += EIG_THREADS) H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[e];
+    // deferred: rows above the block, columns right of it, and Z
+    const int nabove = b0, nright = n - 1 - b1;
+    const int tot = nabove + nright + n;
+    for (int e = tid; e < tot; e += EIG_THREADS) {
+      if (e < nabove)
+        slide<false>(H + (long long)e * n, 1, kstart, kend + 1, crot, srot);
+      else if (e < nabove + nright)
+        slide<true>(H + b1 + 1 + (e - nabove), n, kstart, kend + 1, crot, srot);
+      else
+        slide<false>(Zt + (e - nabove - nright), n, kstart, kend + 1, crot, srot);
+    }
+    __syncthreads();
+    if (kend == ihi - 1) break;
+    kstart = kend + 1;
+    b0 = kstart - 1;
+  }
+}
+
 // Stages 1-3.  A (n x n) becomes T (upper triangle); Zt gets the transposed
 // Schur vectors; X the eigenvectors of T (upper triangular, unit diagonal);
 // lam the eigenvalues.
 template <int EIG_THREADS>
-__global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, long long sA, cd* Ztg, cd* Xg,
+__global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREADS : 1) void zeig_schur_kernel(int n, cd* A, long long sA, cd* Ztg, cd* Xg,
                                                                  long long sW, cd* lam, long long sL, int* info,
                                                                  unsigned long long* tphase) {
   constexpr int EIG_WAVES = EIG_THREADS / 64;
@@ -171,7 +483,7 @@ __global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, l
   cd* wv = u + n;                     // 2n    u^H H (columns of H) and u^T Zt
   cd* vh = wv + 2 * n;                // n     H u
   cd* part = vh + n;                  // waves*2n  partial sums
-  cd* srot = part + EIG_WAVES * 2 * n;  // n     rotation sines of the current sweep
+  cd* srot = part + eig_part_elems(n, EIG_WAVES);  // n rotation sines (part doubles as the QR chase block)
   double* crot = reinterpret_cast<double*>(srot + n);  // n cosines
   cd* hl = srot + n + (n + 1) / 2;                      // n*n LDS copy of H (small n)
   __shared__ double red[EIG_WAVES];
@@ -198,7 +510,9 @@ __global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, l
   __syncthreads();
 
   // ------------------------------------------------------------ 1. Hessenberg
-  for (int k = 0; k + 2 < n; ++k) {
+  if (!in_lds)
+    hessenberg_blocked<EIG_THREADS>(H, Zt, n, X, X + (long long)n * HB_NB, X + 2ll * n * HB_NB, u, vh, part, red);
+  for (int k = 0; in_lds && k + 2 < n; ++k) {
     const int L = n - k - 1;  // reflector acts on rows/cols k+1..n-1
     double part2 = 0.0;
     for (int i = k + 2 + tid; i < n; i += EIG_THREADS) part2 += abs2c(H[i * n + k]);
@@ -312,6 +626,10 @@ __global__ __launch_bounds__(EIG_THREADS) void zeig_schur_kernel(int n, cd* A, l
         const cd den = abs2c(p1) >= abs2c(p2) ? p1 : p2;
         shift = (den.x == 0.0 && den.y == 0.0) ? d : d - divc(bc, den);
       }
+    }
+    if (!in_lds) {
+      qr_sweep_blocked<EIG_THREADS>(H, Zt, n, l, ihi, shift, part, crot, srot, &s_xy[0][0]);
+      continue;
     }
     cd x = H[l * n + l] - shift;
     cd y = H[(l + 1) * n + l];
@@ -455,7 +773,8 @@ size_t zeig_workspace_bytes(int n, int batch) {
 }
 
 static size_t eig_lds_bytes(int n, int threads) {
-  size_t b = (size_t)(1 + 2 + 1 + 2 * (threads / 64) + 1) * n * sizeof(cd) + (size_t)((n + 1) / 2) * sizeof(cd);
+  size_t b = (size_t)(1 + 2 + 1 + 1) * n * sizeof(cd) + (size_t)eig_part_elems(n, threads / 64) * sizeof(cd) +
+             (size_t)((n + 1) / 2) * sizeof(cd);
   if (n <= KHX_EIG_LDS_N) b += (size_t)n * n * sizeof(cd);
   return b;
 }
@@ -471,13 +790,25 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   const long long sZ = (long long)n * n;
   const char* env = getenv("KHX_EIG_THREADS");
   const int threads = env ? atoi(env) : EIG_THREADS_DEFAULT;
-  KHX_REQUIRE(threads == 64 || threads == 128 || threads == 256, "zeig: KHX_EIG_THREADS must be 64, 128 or 256");
-  const size_t lds = eig_lds_bytes(n, threads);
+  KHX_REQUIRE(threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024,
+              "zeig: KHX_EIG_THREADS must be 64, 128, 256, 512 or 1024");
+  // optional LDS padding: fewer resident workgroups per CU keep the working
+  // set of the resident matrices inside the caches
+  const char* pad_env = getenv("KHX_EIG_LDS_PAD");
+  const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
   KHX_REQUIRE(lds <= 150 * 1024, "zeig: n too large for the LDS work vectors");
-  static size_t lds_limit[3] = {0, 0, 0};
+  static size_t lds_limit[5] = {0, 0, 0, 0, 0};
   {
     ProfScope ps("zeig_schur_kernel", s);
-    if (threads == 64) {
+    if (threads == 1024) {
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<1024>, lds, lds_limit[4]));
+      zeig_schur_kernel<1024><<<dim3(batch), dim3(1024), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
+                                                                   tphase);
+    } else if (threads == 512) {
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<512>, lds, lds_limit[3]));
+      zeig_schur_kernel<512><<<dim3(batch), dim3(512), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
+                                                                 tphase);
+    } else if (threads == 64) {
       KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<64>, lds, lds_limit[0]));
       zeig_schur_kernel<64><<<dim3(batch), dim3(64), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase);
     } else if (threads == 128) {
