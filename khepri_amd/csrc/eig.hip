@@ -369,6 +369,102 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
   }
 }
 
+// Deferred rotations of up to EIG_KLOG QR sweeps, applied in one pass per line.
+// Sweep q rotated the pairs (p, p+1), p in [lo_q, hi_q - 1]; its rotation at p
+// must follow the previous sweep's at p - 1 .. p + 1, so at step t the pass
+// applies sweep q at p = t - 2q: a line is read and written once for all the
+// logged sweeps, its live part (2 EIG_KLOG elements) staying in registers.
+constexpr int EIG_KLOG = 4;
+
+template <bool ROWS>
+__device__ void slide_pipe(cd* v, int step, int nsw, int pmin, int pend, const int* lo, const int* hi,
+                           unsigned mask, const double* lc, const cd* ls, int n) {
+  constexpr int W = 2 * EIG_KLOG;
+  // r[i] holds element t - 2 (EIG_KLOG - 1) + i at step t; the window shifts
+  // by one element per step so every index below is a compile-time constant
+  cd r[W];
+  int l_[EIG_KLOG], h_[EIG_KLOG];
+  // the sweep bounds and positions are the same for every lane: keep them
+  // scalar so the rotation parameters come in through scalar loads
+  pmin = __builtin_amdgcn_readfirstlane(pmin);
+  pend = __builtin_amdgcn_readfirstlane(pend);
+  nsw = __builtin_amdgcn_readfirstlane(nsw);
+#pragma unroll
+  for (int q = 0; q < EIG_KLOG; ++q) {
+    const bool on = q < nsw && ((mask >> q) & 1u);
+    l_[q] = on ? __builtin_amdgcn_readfirstlane(lo[q]) : 1;
+    h_[q] = on ? __builtin_amdgcn_readfirstlane(hi[q]) : 0;  // empty range when the sweep does not apply
+  }
+  // this line's own range: the elements its sweeps rotate.  Lines of H may
+  // share elements they do not rotate, so nothing outside it is loaded or
+  // stored (the step loop itself runs over the common range).
+  int lp = pend + 1, le = pmin - 1;
+#pragma unroll
+  for (int q = 0; q < EIG_KLOG; ++q)
+    if (l_[q] < h_[q]) {
+      lp = min(lp, l_[q]);
+      le = max(le, h_[q]);
+    }
+#pragma unroll
+  for (int i = 0; i < W; ++i) r[i] = mkc(0.0, 0.0);
+  if (pmin >= lp && pmin <= le) r[W - 2] = v[(long long)pmin * step];  // element pmin at step t = pmin - 1
+  const int tend = pend + 2 * (EIG_KLOG - 1);
+  for (int t = pmin; t <= tend; ++t) {
+    if (t + 1 >= lp && t + 1 <= le) r[W - 1] = v[(long long)(t + 1) * step];
+#pragma unroll
+    for (int q = 0; q < EIG_KLOG; ++q) {
+      const int p = t - 2 * q;
+      if (p >= l_[q] && p < h_[q]) {
+        const double c = lc[q * n + p];
+        const cd sn = ls[q * n + p];
+        const int a = 2 * (EIG_KLOG - 1 - q);  // constant after unrolling
+        if (ROWS)
+          rot_rows(r[a], r[a + 1], c, sn);
+        else
+          rot_cols(r[a], r[a + 1], c, sn);
+      }
+    }
+    const int e = t - 2 * (EIG_KLOG - 1);
+    if (e >= lp && e <= le) v[(long long)e * step] = r[0];
+#pragma unroll
+    for (int i = 0; i + 1 < W; ++i) r[i] = r[i + 1];
+  }
+}
+
+// Apply the logged sweeps to Z (every sweep), to the rows of H above each
+// sweep's window and to the columns right of it.
+template <int EIG_THREADS>
+__device__ void flush_log(cd* H, cd* Zt, int n, int nsw, const int* sl, const int* sh, const double* lc,
+                          const cd* ls) {
+  int pmin = n, pend = 0, lmax = 0, hmin = n;
+  for (int q = 0; q < nsw; ++q) {
+    pmin = min(pmin, sl[q]);
+    pend = max(pend, sh[q]);
+    lmax = max(lmax, sl[q]);
+    hmin = min(hmin, sh[q]);
+  }
+  const int nrows = lmax, ncols = n - 1 - hmin;
+  const unsigned all = (1u << nsw) - 1u;
+  for (int e = threadIdx.x; e < n + nrows + ncols; e += EIG_THREADS) {
+    if (e < n) {
+      slide_pipe<false>(Zt + e, n, nsw, pmin, pend, sl, sh, all, lc, ls, n);
+    } else if (e < n + nrows) {  // row i of H: sweeps whose window starts below i
+      const int i = e - n;
+      unsigned m = 0;
+      for (int q = 0; q < nsw; ++q)
+        if (i < sl[q]) m |= 1u << q;
+      slide_pipe<false>(H + (long long)i * n, 1, nsw, pmin, pend, sl, sh, m, lc, ls, n);
+    } else {  // column j of H: sweeps whose window ends above j
+      const int j = hmin + 1 + (e - n - nrows);
+      unsigned m = 0;
+      for (int q = 0; q < nsw; ++q)
+        if (j > sh[q]) m |= 1u << q;
+      slide_pipe<true>(H + j, n, nsw, pmin, pend, sl, sh, m, lc, ls, n);
+    }
+  }
+  __syncthreads();
+}
+
 // One implicit single-shift QR sweep over the window [l, ihi] with H in
 // global memory.  The bulge is chased through EIG_QB x EIG_QB diagonal blocks
 // staged in LDS: inside a block every rotation updates only the block (one
@@ -452,16 +548,15 @@ __device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift,
       __syncthreads();
     }
     for (int e = tid; e < nb * nb; e += EIG_THREADS) H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[e];
-    // deferred: rows above the block, columns right of it, and Z
-    const int nabove = b0, nright = n - 1 - b1;
-    const int tot = nabove + nright + n;
+    // deferred inside the window: rows l..b0-1 above the block, columns
+    // b1+1..ihi right of it (the rest of H and Z wait for flush_log)
+    const int nabove = b0 - l, nright = ihi - b1;
+    const int tot = nabove + nright;
     for (int e = tid; e < tot; e += EIG_THREADS) {
       if (e < nabove)
-        slide<false>(H + (long long)e * n, 1, kstart, kend + 1, crot, srot);
-      else if (e < nabove + nright)
-        slide<true>(H + b1 + 1 + (e - nabove), n, kstart, kend + 1, crot, srot);
+        slide<false>(H + (long long)(l + e) * n, 1, kstart, kend + 1, crot, srot);
       else
-        slide<false>(Zt + (e - nabove - nright), n, kstart, kend + 1, crot, srot);
+        slide<true>(H + b1 + 1 + (e - nabove), n, kstart, kend + 1, crot, srot);
     }
     __syncthreads();
     if (kend == ihi - 1) break;
@@ -588,6 +683,12 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   if (tp && tid == 0) tp[1] = wall_clock64();
   // ------------------------------------------------------------ 2. Schur form
   int ihi = n - 1, its = 0, itot = 0, fail = 0;
+  // rotation log of the sweeps not yet applied outside their windows (X is
+  // free until the eigenvector stage)
+  __shared__ int s_lo[EIG_KLOG], s_hi[EIG_KLOG];
+  int nlog = 0;
+  cd* log_s = X;
+  double* log_c = reinterpret_cast<double*>(X + (long long)EIG_KLOG * n);
   const int itmax = 30 * max(10, n);
   while (ihi >= 1) {
     int lk = 0;
@@ -628,7 +729,25 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       }
     }
     if (!in_lds) {
+      // rows l.. of H must be current before the window reaches them
+      if (nlog > 0 && l < s_lo[nlog - 1]) {
+        flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s);
+        nlog = 0;
+      }
       qr_sweep_blocked<EIG_THREADS>(H, Zt, n, l, ihi, shift, part, crot, srot, &s_xy[0][0]);
+      for (int k = l + tid; k < ihi; k += EIG_THREADS) {
+        log_c[nlog * n + k] = crot[k];
+        log_s[nlog * n + k] = srot[k];
+      }
+      if (tid == 0) {
+        s_lo[nlog] = l;
+        s_hi[nlog] = ihi;
+      }
+      __syncthreads();
+      if (++nlog == EIG_KLOG) {
+        flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s);
+        nlog = 0;
+      }
       continue;
     }
     cd x = H[l * n + l] - shift;
@@ -700,6 +819,7 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
     }
     __syncthreads();
   }
+  if (nlog > 0) flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s);
   if (tid == 0 && info) info[item] = fail;
   if (tp && tid == 0) {
     tp[2] = wall_clock64();
