@@ -141,6 +141,8 @@ int khx_poynting_vec(const double* sR, const double* sT, int N, const double* g,
  * device ints; B (batch, n, nrhs) overwritten by X.                        */
 int khx_lu_factor(int n, double* A, int batch, int* ipiv, int* info, void* stream);
 int khx_lu_solve(int n, int nrhs, const double* LU, const int* ipiv, double* B, int batch, void* stream);
+/* Both at once: A -> its factors (as khx_lu_factor), B -> A^-1 B.          */
+int khx_lu_factor_solve(int n, int nrhs, double* A, double* B, int batch, int* ipiv, int* info, void* stream);
 
 /* Elementwise helpers of the S-matrix algebra: out = diag(dl) A diag(dr)
  * (strided batches like khx_zgemm; dl (batch, rows), dr (batch, cols),

@@ -38,6 +38,7 @@ SIGNATURES = {
     "khx_poynting_vec": (_i, [_dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _i, _dp, _dp, _dp, _dp, _vp]),
     "khx_lu_factor": (_i, [_i, _dp, _i, _ip, _ip, _vp]),
     "khx_lu_solve": (_i, [_i, _i, _dp, _ip, _dp, _i, _vp]),
+    "khx_lu_factor_solve": (_i, [_i, _i, _dp, _dp, _i, _ip, _ip, _vp]),
     "khx_diag_scale": (_i, [_dp, _i, C.c_longlong, _dp, _dp, _i, _i, _i, _dp, _i, C.c_longlong, _vp]),
     "khx_identity_minus": (_i, [_dp, _i, _i, _dp, _vp]),
     "khx_zaxpby": (_i, [_dp, _i, C.c_longlong, _dp, _dp, _i, C.c_longlong, _dp, _dp, _i, C.c_longlong, _i, _i, _i,

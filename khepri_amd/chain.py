@@ -88,12 +88,13 @@ def rt_amplitudes(items, c):
     for j in range(m - 2, -1, -1):
         Sj = seq[j]
         D = ops.identity_minus(ops.matmul(R11[j + 1], Sj[:, 1, 1]))
-        lu, piv, info = ops.lu_factor(D)
+        if j >= 1:  # factor D_j and solve for R11_{j+1} S21_j in the same pass
+            lu, piv, info, Y = ops.lu_factor_solve(D, ops.matmul(R11[j + 1], Sj[:, 1, 0]))
+            R11[j] = ops.matmul(Sj[:, 0, 1], Y, beta=1.0, C=Sj[:, 0, 0].clone())
+        else:
+            lu, piv, info = ops.lu_factor(D)
         infos.append(info)
         LU[j] = (lu, piv)
-        if j >= 1:
-            Y = ops.lu_solve(lu, piv, ops.matmul(R11[j + 1], Sj[:, 1, 0]))
-            R11[j] = ops.matmul(Sj[:, 0, 1], Y, beta=1.0, C=Sj[:, 0, 0].clone())
     # left-to-right vector pass
     S1 = seq[0]
     v = _mv(S1[:, 1, 0], c)

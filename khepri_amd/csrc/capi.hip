@@ -42,8 +42,6 @@ int poynting_vec(hipStream_t s, const cd* sR, const cd* sT, int N, const double*
                  cd epsi, cd epse, int batch, double* R, double* T, double* Rord, double* Tord);
 int incident(hipStream_t s, int N, const cd* kp, const double* wl, const cd* te, const cd* tm, cd epsi, int batch,
              cd* c);
-int zgetrf(hipStream_t s, int n, Mat A, int batch, int* ipiv, int* info);
-int zgetrs(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch);
 int zdiag_scale(hipStream_t s, Mat A, const cd* dl, const cd* dr, int rows, int cols, int batch, Mat D);
 int zidentity_minus(hipStream_t s, const cd* A, int n, int batch, cd* D);
 int propagation(hipStream_t s, const cd* kx, const cd* ky, int N, int batch, const double* k0d, cd* X);
@@ -308,6 +306,14 @@ int khx_lu_factor(int n, double* A, int batch, int* ipiv, int* info, void* strea
   KHX_ENTRY_BEGIN
   KHX_REQUIRE(n > 0 && batch >= 0, "khx_lu_factor: bad sizes");
   return khx::zgetrf(as_stream(stream), n, khx::Mat{as_cd(A), n, (long long)n * n}, batch, ipiv, info);
+}
+
+int khx_lu_factor_solve(int n, int nrhs, double* A, double* B, int batch, int* ipiv, int* info, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && nrhs > 0 && batch >= 0, "khx_lu_factor_solve: bad sizes");
+  const long long nn = (long long)n * n;
+  return khx::zgetrf_solve(as_stream(stream), n, nrhs, khx::Mat{as_cd(A), n, nn},
+                           khx::Mat{as_cd(B), nrhs, (long long)n * nrhs}, batch, ipiv, info);
 }
 
 int khx_lu_solve(int n, int nrhs, const double* LU, const int* ipiv, double* B, int batch, void* stream) {

@@ -24,6 +24,11 @@ int zgemm_op(hipStream_t s, int opa, int opb, int M, int N, int K, cd alpha, Mat
 // Solve A X = B (A n x n destroyed, B n x m overwritten); info[b] = first zero pivot (1-based) or 0
 int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info);
 
+// LU factors with pivots (zgetrf layout) and the solve of their system
+int zgetrf(hipStream_t s, int n, Mat A, int batch, int* ipiv, int* info);
+int zgetrs(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch);
+int zgetrf_solve(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info);
+
 // blocked LU (one workgroup per system, panel in LDS): keepL -> zgetrf layout
 bool lu_blocked_fits(int n);
 int zlu_blocked(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL);

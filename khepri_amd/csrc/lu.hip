@@ -239,6 +239,15 @@ int zgetrf(hipStream_t s, int n, Mat A, int batch, int* ipiv, int* info) {
   return 0;
 }
 
+// Factor and solve in one pass: A -> P L U (zgetrf layout, ipiv), B -> A^-1 B.
+int zgetrf_solve(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info) {
+  if (batch <= 0 || n <= 0) return 0;
+  if (n >= 24 && lu_blocked_fits(n) && getenv("KHX_LU_UNBLOCKED") == nullptr)
+    return zlu_blocked(s, n, m, A, B, batch, ipiv, info, true);
+  KHX_TRY(zgetrf(s, n, A, batch, ipiv, info));
+  return zgetrs(s, n, m, A, ipiv, B, batch);
+}
+
 int zgetrs(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch) {
   if (batch <= 0 || n <= 0 || m <= 0) return 0;
   if (m <= ZGETRS_VEC_MAX && (size_t)n * m * sizeof(cd) <= 64 * 1024) {

@@ -151,7 +151,9 @@ class Layer:
         self.info = None
         self.skind, self.X = "dense", None
         gap_medium = np.isscalar(self.epsilon) and complex(self.epsilon) == 1
-        if rt_only and not modes and gap_medium:
+        # rt_only never reconstructs fields, so the retained eigenspaces of the
+        # half spaces (fields flag) are not needed either
+        if rt_only and gap_medium:
             if self.formulation == Formulation.UNIFORM:
                 self.skind, self.S = "prop", None
                 self.X = ops.propagation(kx, ky, (k0 * self.depth).contiguous())

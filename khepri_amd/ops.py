@@ -401,6 +401,18 @@ def lu_factor(A):
     return LU, piv, info
 
 
+def lu_factor_solve(A, B):
+    """(LU, ipiv, info, X = A^-1 B): the factors kept for later solves (inputs untouched)."""
+    lib = _lib.load()
+    LU = A.clone().contiguous()
+    X = B.clone().contiguous()
+    b, n, m = X.shape
+    piv = torch.empty((b, n), dtype=torch.int32, device=A.device)
+    info = torch.empty(b, dtype=torch.int32, device=A.device)
+    check(lib.khx_lu_factor_solve(n, m, _p(LU), _p(X), b, _p(piv), _p(info), _stream()), "khx_lu_factor_solve")
+    return LU, piv, info, X
+
+
 def lu_solve(LU, piv, B):
     lib = _lib.load()
     X = B.clone().contiguous()

@@ -193,6 +193,12 @@ def test_lu_factor_solve(gpu, n, m, batch):
     X = ops.lu_solve(lu, piv, dev(B, gpu)).cpu().numpy()
     ref = np.linalg.solve(A, B)
     np.testing.assert_allclose(X, ref, atol=1e-10 * np.abs(ref).max())
+    # factor and solve in one pass: same solution, factors reusable
+    lu2, piv2, info2, X2 = ops.lu_factor_solve(dev(A, gpu), dev(B, gpu))
+    assert (info2.cpu().numpy() == 0).all()
+    np.testing.assert_allclose(X2.cpu().numpy(), ref, atol=1e-10 * np.abs(ref).max())
+    X3 = ops.lu_solve(lu2, piv2, dev(B, gpu)).cpu().numpy()
+    np.testing.assert_allclose(X3, ref, atol=1e-10 * np.abs(ref).max())
     # the factors reproduce P A = L U
     lu, piv = lu.cpu().numpy(), piv.cpu().numpy()
     for b in range(batch):
