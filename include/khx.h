@@ -73,10 +73,12 @@ int khx_zgemm(int M, int N, int K, const double* alpha, const double* A, int lda
  * info[b] > 0: QR iteration did not converge.                             */
 size_t khx_eig_ws_bytes(int n, int batch);
 int khx_eig(int n, double* A, int batch, double* lam, double* W, int* info, void* ws, void* stream);
-/* khx_eig that also writes, per matrix, 5 words to tphase (device,
- * batch x 5): wall-clock stamps (100 MHz) at start / after the Hessenberg
- * reduction / after the Schur iteration / after the eigenvectors, and the
- * number of QR sweeps -- measurement support only.                          */
+/* khx_eig that also writes, per matrix, 10 words to tphase (device,
+ * batch x 10): wall-clock stamps (100 MHz) at start / after the Hessenberg
+ * reduction / after the Schur iteration / after the eigenvectors, the
+ * number of QR sweeps, and (H in global memory) the ticks spent in the
+ * deflation scan and shift, the in-block chase, the rest of the blocked
+ * sweeps, the rotation log and its flushes -- measurement support only.     */
 int khx_eig_timed(int n, double* A, int batch, double* lam, double* W, int* info, void* ws,
                   unsigned long long* tphase, void* stream);
 

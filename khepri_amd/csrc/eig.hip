@@ -31,12 +31,16 @@ constexpr double SAFMIN = 2.2250738585072014e-308;
 constexpr int KHX_EIG_LDS_N = 64;  // H resident in LDS up to this order
 constexpr int EIG_QB = 32;         // QR chase block held in LDS when H is in global memory
 
-// Elements of the LDS region shared by the Hessenberg partial sums and the
-// QR chase block.
+constexpr int HB_NB = 16;          // panel width of the blocked Hessenberg reduction
+
+// LDS layout: u (n), wv (2n, unblocked Hessenberg only), vh (n), part (the
+// Hessenberg partial sums / T factor, then the QR chase block), srot (n),
+// crot (n doubles), and H itself when n <= KHX_EIG_LDS_N.
+__host__ __device__ __forceinline__ int eig_wv_elems(int n) { return n <= KHX_EIG_LDS_N ? 2 * n : 0; }
 __host__ __device__ __forceinline__ int eig_part_elems(int n, int waves) {
-  const int a = waves * 2 * n;
-  if (n <= KHX_EIG_LDS_N) return a;
-  return a > EIG_QB * EIG_QB ? a : EIG_QB * EIG_QB;
+  if (n <= KHX_EIG_LDS_N) return waves * 2 * n;
+  const int hess = HB_NB * HB_NB + waves * HB_NB + 2 * HB_NB;
+  return hess > EIG_QB * EIG_QB ? hess : EIG_QB * EIG_QB;
 }
 
 template <int EIG_THREADS>
@@ -178,8 +182,6 @@ __device__ __forceinline__ void slide(cd* v, int step, int l, int ihi, const dou
 // each as one pass per column instead of one pass per reflector.
 // V and Y (n x HB_NB, row-major) live in global scratch; T, the partial sums
 // and the reflector in LDS.
-constexpr int HB_NB = 16;
-
 template <int EIG_THREADS>
 __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, cd* u, cd* vh, cd* part,
                                    double* red) {
@@ -375,6 +377,7 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
 // applies sweep q at p = t - 2q: a line is read and written once for all the
 // logged sweeps, its live part (2 EIG_KLOG elements) staying in registers.
 constexpr int EIG_KLOG = 4;
+constexpr int EIG_PF = 4;  // loads in flight per line in the pipelined pass
 
 template <bool ROWS>
 __device__ void slide_pipe(cd* v, int step, int nsw, int pmin, int pend, const int* lo, const int* hi,
@@ -408,9 +411,23 @@ __device__ void slide_pipe(cd* v, int step, int nsw, int pmin, int pend, const i
 #pragma unroll
   for (int i = 0; i < W; ++i) r[i] = mkc(0.0, 0.0);
   if (pmin >= lp && pmin <= le) r[W - 2] = v[(long long)pmin * step];  // element pmin at step t = pmin - 1
+  // the next EIG_PF incoming elements are loaded ahead (independent loads in
+  // flight instead of one memory round trip per step)
+  cd pf[EIG_PF];
+#pragma unroll
+  for (int i = 0; i < EIG_PF; ++i) {
+    const int e = pmin + 1 + i;
+    pf[i] = (e >= lp && e <= le) ? v[(long long)e * step] : mkc(0.0, 0.0);
+  }
   const int tend = pend + 2 * (EIG_KLOG - 1);
   for (int t = pmin; t <= tend; ++t) {
-    if (t + 1 >= lp && t + 1 <= le) r[W - 1] = v[(long long)(t + 1) * step];
+    r[W - 1] = pf[0];  // element t + 1 (zero outside the line's range)
+#pragma unroll
+    for (int i = 0; i + 1 < EIG_PF; ++i) pf[i] = pf[i + 1];
+    {
+      const int e = t + 1 + EIG_PF;
+      pf[EIG_PF - 1] = (e >= lp && e <= le) ? v[(long long)e * step] : mkc(0.0, 0.0);
+    }
 #pragma unroll
     for (int q = 0; q < EIG_KLOG; ++q) {
       const int p = t - 2 * q;
@@ -476,7 +493,7 @@ __device__ void flush_log(cd* H, cd* Zt, int n, int nsw, const int* sl, const in
 // rotations in the same order as an unblocked sweep.
 template <int EIG_THREADS>
 __device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift, cd* hb, double* crot, cd* srot,
-                                 cd* sxy) {
+                                 cd* sxy, bool one_wave, unsigned long long* acc) {
   const int tid = threadIdx.x;
   int kstart = l, b0 = l;
   while (true) {
@@ -496,7 +513,13 @@ __device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift,
       }
     }
     __syncthreads();
-    for (int k = kstart; k <= kend; ++k) {
+    // The chase inside the block runs on one wavefront when one_wave is set:
+    // its tasks (at most 2 EIG_QB - 3 per rotation) fit in the 64 lanes, and
+    // lanes hand values to each other through LDS in program order, so no
+    // workgroup barrier is needed per rotation; the other wavefronts wait.
+    const int cw = one_wave ? 64 : EIG_THREADS;
+    unsigned long long t_c0 = acc ? wall_clock64() : 0;
+    for (int k = kstart; k <= kend && (!one_wave || tid < 64); ++k) {
       const cd x = sxy[2 * (k & 1)], y = sxy[2 * (k & 1) + 1];
       cd* nxt = sxy + 2 * ((k + 1) & 1);
       double c;
@@ -510,7 +533,7 @@ __device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift,
       const int nrow = b1 - k - 1;                             // row part: block columns kk+2 .. nb-1
       const int ncol = kk + ((k + 2 <= b1) ? 1 : 0);           // col part: block rows 0 .. kk-1 (+ kk+2)
       const int total = 1 + nrow + ncol;
-      for (int e = tid; e < total; e += EIG_THREADS) {
+      for (int e = tid; e < total; e += cw) {
         if (e == 0) {
           if (k > l) {
             hb[kk * nb + kk - 1] = r;
@@ -545,8 +568,15 @@ __device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift,
           if (i == kk + 2) nxt[1] = xk;
         }
       }
-      __syncthreads();
+      if (one_wave) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      } else {
+        __syncthreads();
+      }
     }
+    if (one_wave) __syncthreads();
+    if (acc) acc[1] += wall_clock64() - t_c0;
     for (int e = tid; e < nb * nb; e += EIG_THREADS) H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[e];
     // deferred inside the window: rows l..b0-1 above the block, columns
     // b1+1..ihi right of it (the rest of H and Z wait for flush_log)
@@ -571,12 +601,12 @@ __device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift,
 template <int EIG_THREADS>
 __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREADS : 1) void zeig_schur_kernel(int n, cd* A, long long sA, cd* Ztg, cd* Xg,
                                                                  long long sW, cd* lam, long long sL, int* info,
-                                                                 unsigned long long* tphase) {
+                                                                 unsigned long long* tphase, int flags) {
   constexpr int EIG_WAVES = EIG_THREADS / 64;
   extern __shared__ cd sm[];
   cd* u = sm;                         // n     Householder vector / solved row
-  cd* wv = u + n;                     // 2n    u^H H (columns of H) and u^T Zt
-  cd* vh = wv + 2 * n;                // n     H u
+  cd* wv = u + n;                     // 2n    u^H H (columns of H) and u^T Zt (n <= KHX_EIG_LDS_N)
+  cd* vh = wv + eig_wv_elems(n);      // n     H u
   cd* part = vh + n;                  // waves*2n  partial sums
   cd* srot = part + eig_part_elems(n, EIG_WAVES);  // n rotation sines (part doubles as the QR chase block)
   double* crot = reinterpret_cast<double*>(srot + n);  // n cosines
@@ -594,7 +624,8 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   const bool in_lds = n <= KHX_EIG_LDS_N;
   cd* H = in_lds ? hl : Hg;
   // optional phase timestamps (wall clock): start, Hessenberg, Schur, eigenvectors, sweeps
-  unsigned long long* tp = tphase ? tphase + item * 5 : nullptr;
+  unsigned long long* tp = tphase ? tphase + item * 10 : nullptr;
+  unsigned long long acc_t[5] = {0, 0, 0, 0, 0};  // scan, chase, block rest, log, flush
   if (tp && tid == 0) tp[0] = wall_clock64();
 
   for (int i = wid; i < n; i += EIG_WAVES)
@@ -691,6 +722,7 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   double* log_c = reinterpret_cast<double*>(X + (long long)EIG_KLOG * n);
   const int itmax = 30 * max(10, n);
   while (ihi >= 1) {
+    const unsigned long long t_s0 = tp ? wall_clock64() : 0;
     int lk = 0;
     for (int k = 1 + tid; k <= ihi; k += EIG_THREADS) {
       if (negligible(H, n, k, smlnum)) {
@@ -729,12 +761,19 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       }
     }
     if (!in_lds) {
+      unsigned long long t1 = tp ? wall_clock64() : 0;
+      if (tp) acc_t[0] += t1 - t_s0;
       // rows l.. of H must be current before the window reaches them
       if (nlog > 0 && l < s_lo[nlog - 1]) {
         flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s);
         nlog = 0;
       }
-      qr_sweep_blocked<EIG_THREADS>(H, Zt, n, l, ihi, shift, part, crot, srot, &s_xy[0][0]);
+      unsigned long long t2 = tp ? wall_clock64() : 0;
+      if (tp) acc_t[4] += t2 - t1;
+      qr_sweep_blocked<EIG_THREADS>(H, Zt, n, l, ihi, shift, part, crot, srot, &s_xy[0][0], (flags & 1) != 0,
+                                    tp ? acc_t : nullptr);
+      unsigned long long t3 = tp ? wall_clock64() : 0;
+      if (tp) acc_t[2] += t3 - t2;
       for (int k = l + tid; k < ihi; k += EIG_THREADS) {
         log_c[nlog * n + k] = crot[k];
         log_s[nlog * n + k] = srot[k];
@@ -744,10 +783,13 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
         s_hi[nlog] = ihi;
       }
       __syncthreads();
+      unsigned long long t4 = tp ? wall_clock64() : 0;
+      if (tp) acc_t[3] += t4 - t3;
       if (++nlog == EIG_KLOG) {
         flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s);
         nlog = 0;
       }
+      if (tp) acc_t[4] += wall_clock64() - t4;
       continue;
     }
     cd x = H[l * n + l] - shift;
@@ -824,6 +866,11 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   if (tp && tid == 0) {
     tp[2] = wall_clock64();
     tp[4] = (unsigned long long)itot;
+    tp[5] = acc_t[0];
+    tp[6] = acc_t[1];
+    tp[7] = acc_t[2] - acc_t[1];  // block load / store / in-window slides
+    tp[8] = acc_t[3];
+    tp[9] = acc_t[4];
   }
 
   cd* lm = lam + item * sL;
@@ -893,7 +940,7 @@ size_t zeig_workspace_bytes(int n, int batch) {
 }
 
 static size_t eig_lds_bytes(int n, int threads) {
-  size_t b = (size_t)(1 + 2 + 1 + 1) * n * sizeof(cd) + (size_t)eig_part_elems(n, threads / 64) * sizeof(cd) +
+  size_t b = (size_t)(3 * n + eig_wv_elems(n)) * sizeof(cd) + (size_t)eig_part_elems(n, threads / 64) * sizeof(cd) +
              (size_t)((n + 1) / 2) * sizeof(cd);
   if (n <= KHX_EIG_LDS_N) b += (size_t)n * n * sizeof(cd);
   return b;
@@ -915,6 +962,9 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   // optional LDS padding: fewer resident workgroups per CU keep the working
   // set of the resident matrices inside the caches
   const char* pad_env = getenv("KHX_EIG_LDS_PAD");
+  // bit 0: chase each LDS block on one wavefront (default on; KHX_EIG_CHASE_WAVE=0 turns it off)
+  const char* cw_env = getenv("KHX_EIG_CHASE_WAVE");
+  const int flags = (cw_env && atoi(cw_env) == 0) ? 0 : 1;
   const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
   KHX_REQUIRE(lds <= 150 * 1024, "zeig: n too large for the LDS work vectors");
   static size_t lds_limit[5] = {0, 0, 0, 0, 0};
@@ -923,22 +973,22 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
     if (threads == 1024) {
       KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<1024>, lds, lds_limit[4]));
       zeig_schur_kernel<1024><<<dim3(batch), dim3(1024), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
-                                                                   tphase);
+                                                                   tphase, flags);
     } else if (threads == 512) {
       KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<512>, lds, lds_limit[3]));
       zeig_schur_kernel<512><<<dim3(batch), dim3(512), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
-                                                                 tphase);
+                                                                 tphase, flags);
     } else if (threads == 64) {
       KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<64>, lds, lds_limit[0]));
-      zeig_schur_kernel<64><<<dim3(batch), dim3(64), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase);
+      zeig_schur_kernel<64><<<dim3(batch), dim3(64), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase, flags);
     } else if (threads == 128) {
       KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<128>, lds, lds_limit[1]));
       zeig_schur_kernel<128><<<dim3(batch), dim3(128), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
-                                                                 tphase);
+                                                                 tphase, flags);
     } else {
       KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<256>, lds, lds_limit[2]));
       zeig_schur_kernel<256><<<dim3(batch), dim3(256), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
-                                                                 tphase);
+                                                                 tphase, flags);
     }
     KHX_LAUNCH_CHECK();
   }
