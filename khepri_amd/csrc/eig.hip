@@ -379,80 +379,17 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
 constexpr int EIG_KLOG = 4;
 constexpr int EIG_PF = 4;  // loads in flight per line in the pipelined pass
 
-template <bool ROWS>
-__device__ void slide_pipe(cd* v, int step, int nsw, int pmin, int pend, const int* lo, const int* hi,
-                           unsigned mask, const double* lc, const cd* ls, int n) {
-  constexpr int W = 2 * EIG_KLOG;
-  // r[i] holds element t - 2 (EIG_KLOG - 1) + i at step t; the window shifts
-  // by one element per step so every index below is a compile-time constant
-  cd r[W];
-  int l_[EIG_KLOG], h_[EIG_KLOG];
-  // the sweep bounds and positions are the same for every lane: keep them
-  // scalar so the rotation parameters come in through scalar loads
-  pmin = __builtin_amdgcn_readfirstlane(pmin);
-  pend = __builtin_amdgcn_readfirstlane(pend);
-  nsw = __builtin_amdgcn_readfirstlane(nsw);
-#pragma unroll
-  for (int q = 0; q < EIG_KLOG; ++q) {
-    const bool on = q < nsw && ((mask >> q) & 1u);
-    l_[q] = on ? __builtin_amdgcn_readfirstlane(lo[q]) : 1;
-    h_[q] = on ? __builtin_amdgcn_readfirstlane(hi[q]) : 0;  // empty range when the sweep does not apply
-  }
-  // this line's own range: the elements its sweeps rotate.  Lines of H may
-  // share elements they do not rotate, so nothing outside it is loaded or
-  // stored (the step loop itself runs over the common range).
-  int lp = pend + 1, le = pmin - 1;
-#pragma unroll
-  for (int q = 0; q < EIG_KLOG; ++q)
-    if (l_[q] < h_[q]) {
-      lp = min(lp, l_[q]);
-      le = max(le, h_[q]);
-    }
-#pragma unroll
-  for (int i = 0; i < W; ++i) r[i] = mkc(0.0, 0.0);
-  if (pmin >= lp && pmin <= le) r[W - 2] = v[(long long)pmin * step];  // element pmin at step t = pmin - 1
-  // the next EIG_PF incoming elements are loaded ahead (independent loads in
-  // flight instead of one memory round trip per step)
-  cd pf[EIG_PF];
-#pragma unroll
-  for (int i = 0; i < EIG_PF; ++i) {
-    const int e = pmin + 1 + i;
-    pf[i] = (e >= lp && e <= le) ? v[(long long)e * step] : mkc(0.0, 0.0);
-  }
-  const int tend = pend + 2 * (EIG_KLOG - 1);
-  for (int t = pmin; t <= tend; ++t) {
-    r[W - 1] = pf[0];  // element t + 1 (zero outside the line's range)
-#pragma unroll
-    for (int i = 0; i + 1 < EIG_PF; ++i) pf[i] = pf[i + 1];
-    {
-      const int e = t + 1 + EIG_PF;
-      pf[EIG_PF - 1] = (e >= lp && e <= le) ? v[(long long)e * step] : mkc(0.0, 0.0);
-    }
-#pragma unroll
-    for (int q = 0; q < EIG_KLOG; ++q) {
-      const int p = t - 2 * q;
-      if (p >= l_[q] && p < h_[q]) {
-        const double c = lc[q * n + p];
-        const cd sn = ls[q * n + p];
-        const int a = 2 * (EIG_KLOG - 1 - q);  // constant after unrolling
-        if (ROWS)
-          rot_rows(r[a], r[a + 1], c, sn);
-        else
-          rot_cols(r[a], r[a + 1], c, sn);
-      }
-    }
-    const int e = t - 2 * (EIG_KLOG - 1);
-    if (e >= lp && e <= le) v[(long long)e * step] = r[0];
-#pragma unroll
-    for (int i = 0; i + 1 < W; ++i) r[i] = r[i + 1];
-  }
-}
+// Lines are processed EIG_THREADS at a time; the rotation parameters of the
+// next EIG_CH steps are staged in LDS (stage, at least 1.5 EIG_KLOG (EIG_CH + 6)
+// complex) so no step waits on a global load for them.
+constexpr int EIG_CH = 32;
 
-// Apply the logged sweeps to Z (every sweep), to the rows of H above each
-// sweep's window and to the columns right of it.
 template <int EIG_THREADS>
 __device__ void flush_log(cd* H, cd* Zt, int n, int nsw, const int* sl, const int* sh, const double* lc,
-                          const cd* ls) {
+                          const cd* ls, cd* stage) {
+  constexpr int K = EIG_KLOG, W = 2 * EIG_KLOG, LAG = 2 * (EIG_KLOG - 1), CHP = EIG_CH + LAG;
+  cd* st_s = stage;
+  double* st_c = reinterpret_cast<double*>(stage + K * CHP);
   int pmin = n, pend = 0, lmax = 0, hmin = n;
   for (int q = 0; q < nsw; ++q) {
     pmin = min(pmin, sl[q]);
@@ -461,22 +398,92 @@ __device__ void flush_log(cd* H, cd* Zt, int n, int nsw, const int* sl, const in
     hmin = min(hmin, sh[q]);
   }
   const int nrows = lmax, ncols = n - 1 - hmin;
-  const unsigned all = (1u << nsw) - 1u;
-  for (int e = threadIdx.x; e < n + nrows + ncols; e += EIG_THREADS) {
+  const int nlines = n + nrows + ncols;
+  const int tend = pend + LAG;
+  for (int base = 0; base < nlines; base += EIG_THREADS) {
+    // this thread's line: a column of Zt (every sweep), a row of H above a
+    // sweep's window (column rotations) or a column of H right of it (row rotations)
+    const int e = base + threadIdx.x;
+    cd* v = Zt;
+    int step = n;
+    bool rows = false;
+    unsigned mask = 0;
     if (e < n) {
-      slide_pipe<false>(Zt + e, n, nsw, pmin, pend, sl, sh, all, lc, ls, n);
-    } else if (e < n + nrows) {  // row i of H: sweeps whose window starts below i
+      v = Zt + e;
+      mask = (1u << nsw) - 1u;
+    } else if (e < n + nrows) {
       const int i = e - n;
-      unsigned m = 0;
+      v = H + (long long)i * n;
+      step = 1;
       for (int q = 0; q < nsw; ++q)
-        if (i < sl[q]) m |= 1u << q;
-      slide_pipe<false>(H + (long long)i * n, 1, nsw, pmin, pend, sl, sh, m, lc, ls, n);
-    } else {  // column j of H: sweeps whose window ends above j
+        if (i < sl[q]) mask |= 1u << q;
+    } else if (e < nlines) {
       const int j = hmin + 1 + (e - n - nrows);
-      unsigned m = 0;
+      v = H + j;
+      rows = true;
       for (int q = 0; q < nsw; ++q)
-        if (j > sh[q]) m |= 1u << q;
-      slide_pipe<true>(H + j, n, nsw, pmin, pend, sl, sh, m, lc, ls, n);
+        if (j > sh[q]) mask |= 1u << q;
+    }
+    int l_[K], h_[K];
+    int lp = pend + 1, le = pmin - 1;  // the line's own range: elements its sweeps rotate
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const bool on = q < nsw && ((mask >> q) & 1u);
+      l_[q] = on ? sl[q] : 1;
+      h_[q] = on ? sh[q] : 0;
+      if (on) {
+        lp = min(lp, l_[q]);
+        le = max(le, h_[q]);
+      }
+    }
+    // r[i] holds element t - LAG + i at step t; shifted by one per step
+    cd r[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) r[i] = mkc(0.0, 0.0);
+    if (pmin >= lp && pmin <= le) r[W - 2] = v[(long long)pmin * step];
+    cd pf[EIG_PF];
+#pragma unroll
+    for (int i = 0; i < EIG_PF; ++i) {
+      const int el = pmin + 1 + i;
+      pf[i] = (el >= lp && el <= le) ? v[(long long)el * step] : mkc(0.0, 0.0);
+    }
+    for (int t0 = pmin; t0 <= tend; t0 += EIG_CH) {
+      __syncthreads();  // the previous chunk's parameters are no longer read
+      for (int idx = threadIdx.x; idx < K * CHP; idx += EIG_THREADS) {
+        const int q = idx / CHP, p = t0 - LAG + idx % CHP;
+        const bool ok = q < nsw && p >= 0 && p < n;
+        st_s[idx] = ok ? ls[q * n + p] : mkc(0.0, 0.0);
+        st_c[idx] = ok ? lc[q * n + p] : 0.0;
+      }
+      __syncthreads();
+      const int t1 = min(t0 + EIG_CH - 1, tend);
+      for (int t = t0; t <= t1; ++t) {
+        r[W - 1] = pf[0];
+#pragma unroll
+        for (int i = 0; i + 1 < EIG_PF; ++i) pf[i] = pf[i + 1];
+        {
+          const int el = t + 1 + EIG_PF;
+          pf[EIG_PF - 1] = (el >= lp && el <= le) ? v[(long long)el * step] : mkc(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          const int p = t - 2 * q;
+          if (p >= l_[q] && p < h_[q]) {
+            const int idx = q * CHP + (p - t0 + LAG);
+            const double c = st_c[idx];
+            const cd sn = st_s[idx];
+            const int a = 2 * (K - 1 - q);  // constant after unrolling
+            if (rows)
+              rot_rows(r[a], r[a + 1], c, sn);
+            else
+              rot_cols(r[a], r[a + 1], c, sn);
+          }
+        }
+        const int eo = t - LAG;
+        if (eo >= lp && eo <= le) v[(long long)eo * step] = r[0];
+#pragma unroll
+        for (int i = 0; i + 1 < W; ++i) r[i] = r[i + 1];
+      }
     }
   }
   __syncthreads();
@@ -765,7 +772,7 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       if (tp) acc_t[0] += t1 - t_s0;
       // rows l.. of H must be current before the window reaches them
       if (nlog > 0 && l < s_lo[nlog - 1]) {
-        flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s);
+        flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s, part);
         nlog = 0;
       }
       unsigned long long t2 = tp ? wall_clock64() : 0;
@@ -786,7 +793,7 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       unsigned long long t4 = tp ? wall_clock64() : 0;
       if (tp) acc_t[3] += t4 - t3;
       if (++nlog == EIG_KLOG) {
-        flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s);
+        flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s, part);
         nlog = 0;
       }
       if (tp) acc_t[4] += wall_clock64() - t4;
@@ -861,7 +868,7 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
     }
     __syncthreads();
   }
-  if (nlog > 0) flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s);
+  if (nlog > 0) flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s, part);
   if (tid == 0 && info) info[item] = fail;
   if (tp && tid == 0) {
     tp[2] = wall_clock64();
