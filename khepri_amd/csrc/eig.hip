@@ -31,7 +31,7 @@ constexpr double SAFMIN = 2.2250738585072014e-308;
 constexpr int KHX_EIG_LDS_N = 64;  // H resident in LDS up to this order
 constexpr int EIG_QB = 32;         // QR chase block held in LDS when H is in global memory
 
-constexpr int HB_NB = 16;          // panel width of the blocked Hessenberg reduction
+constexpr int HB_NB = 8;          // panel width of the blocked Hessenberg reduction
 
 // LDS layout: u (n), wv (2n, unblocked Hessenberg only), vh (n), part (the
 // Hessenberg partial sums / T factor, then the QR chase block), srot (n),
@@ -39,7 +39,7 @@ constexpr int HB_NB = 16;          // panel width of the blocked Hessenberg redu
 __host__ __device__ __forceinline__ int eig_wv_elems(int n) { return n <= KHX_EIG_LDS_N ? 2 * n : 0; }
 __host__ __device__ __forceinline__ int eig_part_elems(int n, int waves) {
   if (n <= KHX_EIG_LDS_N) return waves * 2 * n;
-  const int hess = HB_NB * HB_NB + waves * HB_NB + 2 * HB_NB;
+  const int hess = HB_NB * HB_NB + waves * HB_NB + 2 * HB_NB + 2 * 8 * HB_NB;  // T, sums, staged rows
   return hess > EIG_QB * EIG_QB ? hess : EIG_QB * EIG_QB;
 }
 
@@ -182,6 +182,97 @@ __device__ __forceinline__ void slide(cd* v, int step, int l, int ihi, const dou
 // each as one pass per column instead of one pass per reflector.
 // V and Y (n x HB_NB, row-major) live in global scratch; T, the partial sums
 // and the reflector in LDS.
+constexpr int HB_RC = 4;  // rows per staged chunk in the trailing passes
+
+// One pass over the columns [c0, n) of M (rows r0..n-1), one thread per column:
+//   MODE 0:  M(i, col) -= sum_q F(i, q) conj(G(col, q))            (right update, F = Y, G = V)
+//   MODE 1:  w_q = sum_i conj(F(i, q)) M(i, col);  M(i, col) -= sum_q G(i, q) w_q    (F = Vt, G = V)
+//   MODE 2:  w_q = sum_i F(i, q) M(i, col);        M(i, col) -= sum_q conj(G(i, q)) w_q
+// F and G rows (n x HB_NB, row-major) are staged HB_RC rows at a time in LDS.
+template <int EIG_THREADS>
+__device__ __forceinline__ void stage_rows(cd* sf, const cd* src, int i0, int n) {
+  __syncthreads();  // previous chunk consumed
+  for (int e = threadIdx.x; e < HB_RC * HB_NB; e += EIG_THREADS) {
+    const int i = i0 + e / HB_NB;
+    sf[e] = (i < n) ? src[(long long)i * HB_NB + e % HB_NB] : mkc(0.0, 0.0);
+  }
+  __syncthreads();
+}
+
+// M(i, col) -= sum_q S(i, q) w_q for rows [r0, n) (CONJ: conj(S)); S staged HB_RC rows at a time
+template <int EIG_THREADS, bool CONJ>
+__device__ __forceinline__ void update_rows(cd* M, int n, int cc, bool has, int r0, const cd* S, const cd* w,
+                                            cd* sf) {
+#pragma unroll 1
+  for (int i0 = r0; i0 < n; i0 += HB_RC) {
+    stage_rows<EIG_THREADS>(sf, S, i0, n);
+    if (has) {
+      cd a[HB_RC];
+#pragma unroll
+      for (int r = 0; r < HB_RC; ++r) a[r] = (i0 + r < n) ? M[(long long)(i0 + r) * n + cc] : mkc(0.0, 0.0);
+#pragma unroll
+      for (int r = 0; r < HB_RC; ++r) {
+#pragma unroll
+        for (int q = 0; q < HB_NB; ++q) {
+          const cd g = sf[r * HB_NB + q];
+          a[r] = fmsc(a[r], CONJ ? conjc(g) : g, w[q]);
+        }
+        if (i0 + r < n) M[(long long)(i0 + r) * n + cc] = a[r];
+        __builtin_amdgcn_sched_barrier(0);  // keep the next row's LDS reads from being hoisted
+      }
+    }
+  }
+}
+
+// w_q += sum_i S(i, q) M(i, col) over rows [r0, n) (CONJ: conj(S))
+template <int EIG_THREADS, bool CONJ>
+__device__ __forceinline__ void accumulate_rows(const cd* M, int n, int cc, bool has, int r0, const cd* S, cd* w,
+                                                cd* sf) {
+#pragma unroll 1
+  for (int i0 = r0; i0 < n; i0 += HB_RC) {
+    stage_rows<EIG_THREADS>(sf, S, i0, n);
+    if (has) {
+      cd a[HB_RC];
+#pragma unroll
+      for (int r = 0; r < HB_RC; ++r) a[r] = (i0 + r < n) ? M[(long long)(i0 + r) * n + cc] : mkc(0.0, 0.0);
+#pragma unroll
+      for (int r = 0; r < HB_RC; ++r) {
+#pragma unroll
+        for (int q = 0; q < HB_NB; ++q) {
+          if (CONJ)
+            w[q] = fmac_cj(w[q], sf[r * HB_NB + q], a[r]);
+          else
+            w[q] = fmac(w[q], sf[r * HB_NB + q], a[r]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+}
+
+// One pass over the columns [c0, n) of M, one thread per column:
+//   MODE 0:  M(i, col) -= sum_q F(i, q) conj(G(col, q)), all rows         (F = Y, G = V)
+//   MODE 1:  w = F^H M(r0:, col);  M(r0:, col) -= G w                      (F = Vt, G = V)
+//   MODE 2:  w = F^T M(r0:, col);  M(r0:, col) -= conj(G) w
+template <int EIG_THREADS, int MODE>
+__device__ void trailing_pass(cd* M, int n, int c0, int r0, const cd* F, const cd* G, cd* stage) {
+#pragma unroll 1
+  for (int cb = c0; cb < n; cb += EIG_THREADS) {
+    const int cc = cb + threadIdx.x;
+    const bool has = cc < n;
+    cd w[HB_NB];
+#pragma unroll
+    for (int q = 0; q < HB_NB; ++q) w[q] = (MODE == 0 && has) ? conjc(G[(long long)cc * HB_NB + q]) : mkc(0.0, 0.0);
+    if (MODE == 0) {
+      update_rows<EIG_THREADS, false>(M, n, cc, has, r0, F, w, stage);
+    } else {
+      accumulate_rows<EIG_THREADS, MODE == 1>(M, n, cc, has, r0, F, w, stage);
+      update_rows<EIG_THREADS, MODE == 2>(M, n, cc, has, r0, G, w, stage);
+    }
+  }
+  __syncthreads();
+}
+
 template <int EIG_THREADS>
 __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, cd* u, cd* vh, cd* part,
                                    double* red) {
@@ -192,6 +283,7 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
   cd* wsum = part + HB_NB * HB_NB;    // EIG_WAVES x HB_NB partial sums
   cd* wv = wsum + EIG_WAVES * HB_NB;  // HB_NB reduced vector
   cd* wt = wv + HB_NB;                // HB_NB transformed vector
+  cd* stage = wt + HB_NB;             // 2 x HB_RC x HB_NB staged factor rows
   // wv[q] = sum_{i >= r0} conj(V(i, q)) x_i for every q (columns of V past the
   // current reflector are zero): thread -> (q = tid % HB_NB, every 16th row).
   auto vh_dot = [&](const cd* x, long long xstride, int r0) {
@@ -271,18 +363,25 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
         if (!skip) H[(long long)i * n + c] = (i == c + 1) ? -(xn * ph) : mkc(0.0, 0.0);
       }
       __syncthreads();
-      // (4) vh = H(:, c+1:) u  (panel-start columns), one wavefront per row
-      for (int i = wid; i < n; i += EIG_WAVES) {
-        const cd* hr = H + (long long)i * n + c + 1;
-        double sx = 0.0, sy = 0.0;
+      // (4) vh = H(:, c+1:) u  (panel-start columns): each wavefront takes four
+      // rows at a time (four independent loads in flight per lane)
+      for (int i0 = wid * 4; i0 < n; i0 += EIG_WAVES * 4) {
+        double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
         for (int q = lane; q < L; q += 64) {
-          const cd t = hr[q] * u[q];
-          sx += t.x;
-          sy += t.y;
+          const cd uq = u[q];
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (i0 + r < n) {
+              const cd t = H[(long long)(i0 + r) * n + c + 1 + q] * uq;
+              sx[r] += t.x;
+              sy[r] += t.y;
+            }
         }
-        sx = wave_sum(sx);
-        sy = wave_sum(sy);
-        if (lane == 0) vh[i] = mkc(sx, sy);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double a = wave_sum(sx[r]), b = wave_sum(sy[r]);
+          if (lane == 0 && i0 + r < n) vh[i0 + r] = mkc(a, b);
+        }
       }
       __syncthreads();
       // (5) z = V(:, 0:j)^H u; T(0:j, j) = -T z, T(j, j) = 1; Y(:, j) = vh - Y(:, 0:j) z
@@ -315,58 +414,12 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
       }
     }
     __syncthreads();
-    // (6)-(7) trailing columns: A <- (I - V T^H V^H)(A - Y V^H) = (I - V Vt^H)(A - Y V^H), one thread per column
-    for (int col = k0 + nbp + tid; col < n; col += EIG_THREADS) {
-      cd w[HB_NB];
-#pragma unroll
-      for (int q = 0; q < HB_NB; ++q) w[q] = conjc(V[(long long)col * HB_NB + q]);
-#pragma unroll 1
-      for (int i = 0; i < n; ++i) {
-        const cd* yr = Y + (long long)i * HB_NB;
-        cd a = H[(long long)i * n + col];
-#pragma unroll
-        for (int q = 0; q < HB_NB; ++q) a = fmsc(a, yr[q], w[q]);
-        H[(long long)i * n + col] = a;
-      }
-#pragma unroll
-      for (int q = 0; q < HB_NB; ++q) w[q] = mkc(0.0, 0.0);
-#pragma unroll 1
-      for (int i = k0 + 1; i < n; ++i) {
-        const cd a = H[(long long)i * n + col];
-        const cd* tr = Vt + (long long)i * HB_NB;
-#pragma unroll
-        for (int q = 0; q < HB_NB; ++q) w[q] = fmac_cj(w[q], tr[q], a);
-      }
-#pragma unroll 1
-      for (int i = k0 + 1; i < n; ++i) {
-        const cd* vr = V + (long long)i * HB_NB;
-        cd a = H[(long long)i * n + col];
-#pragma unroll
-        for (int q = 0; q < HB_NB; ++q) a = fmsc(a, vr[q], w[q]);
-        H[(long long)i * n + col] = a;
-      }
-    }
-    // (8) Zt <- Q^T Zt = Zt - conj(V) Vt^T Zt  (rows k0+1..), one thread per column
-    for (int col = tid; col < n; col += EIG_THREADS) {
-      cd w[HB_NB];
-#pragma unroll
-      for (int q = 0; q < HB_NB; ++q) w[q] = mkc(0.0, 0.0);
-#pragma unroll 1
-      for (int i = k0 + 1; i < n; ++i) {
-        const cd z = Zt[(long long)i * n + col];
-        const cd* tr = Vt + (long long)i * HB_NB;
-#pragma unroll
-        for (int q = 0; q < HB_NB; ++q) w[q] = fmac(w[q], tr[q], z);
-      }
-#pragma unroll 1
-      for (int i = k0 + 1; i < n; ++i) {
-        const cd* vr = V + (long long)i * HB_NB;
-        cd z = Zt[(long long)i * n + col];
-#pragma unroll
-        for (int q = 0; q < HB_NB; ++q) z = fmsc(z, conjc(vr[q]), w[q]);
-        Zt[(long long)i * n + col] = z;
-      }
-    }
+    // (6)-(8) trailing columns A <- (I - V Vt^H)(A - Y V^H) and Zt <- Zt - conj(V) Vt^T Zt,
+    // one thread per column; the factor rows (Y, V, Vt) of HB_RC rows at a time
+    // are staged in LDS and the column's HB_RC elements are loaded together.
+    trailing_pass<EIG_THREADS, 0>(H, n, k0 + nbp, 0, Y, V, stage);       // A(:, trail) -= Y V(trail, :)^H
+    trailing_pass<EIG_THREADS, 1>(H, n, k0 + nbp, k0 + 1, Vt, V, stage);  // A -= V (Vt^H A)  (rows k0+1..)
+    trailing_pass<EIG_THREADS, 2>(Zt, n, 0, k0 + 1, Vt, V, stage);        // Zt -= conj(V) (Vt^T Zt)
     __syncthreads();
   }
 }
@@ -602,6 +655,140 @@ __device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift,
   }
 }
 
+// Two shifts per sweep: two bulges, EIG_SEP positions apart (the leading one
+// with shift s0), chased through the same LDS blocks by one wavefront.  Each
+// step computes both rotations, then applies all row rotations (phase R) and
+// all column rotations (phase C): rotations of one kind act on disjoint
+// pairs, and a row and a column rotation commute, so each element is written
+// once per phase.  The block's rotations are applied to the rest of the
+// window bulge by bulge (leading bulge first, which the separation makes
+// equivalent to the step order) and logged per bulge for the deferred pass.
+constexpr int EIG_SEP = 3;
+constexpr int EIG_TWO_MIN_W = 12;  // smaller windows use one shift per sweep
+
+template <int EIG_THREADS>
+__device__ void qr_sweep2_blocked(cd* H, int n, int l, int ihi, cd s0, cd s1, cd* hb, double* crot0, cd* srot0,
+                                  double* crot1, cd* srot1, unsigned long long* acc) {
+  const int tid = threadIdx.x;
+  const int nsteps = (ihi - l) + EIG_SEP;  // bulge j is at k = l + step - EIG_SEP j, active for l <= k < ihi
+  int st = 0;
+  while (st < nsteps) {
+    const int kt = l + st - EIG_SEP;  // trailing bulge at the first step of the block
+    const int b0 = (kt <= l) ? l : kt - 1;
+    const int b1 = min(b0 + EIG_QB - 1, ihi);
+    const int nb = b1 - b0 + 1;
+    // steps [st, se) fit the block: every active bulge keeps its inputs
+    // (column k - 1, or the window start) and its bulge (row k + 2, or the
+    // window end) inside
+    int se = st;
+    while (se < nsteps) {
+      bool ok = true;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int k = l + se - EIG_SEP * j;
+        if (k < l || k >= ihi) continue;
+        if (!(k == l ? b0 == l : k - 1 >= b0)) ok = false;
+        if (!(k + 2 <= b1 || (k + 1 == ihi && b1 == ihi))) ok = false;
+      }
+      if (!ok) break;
+      ++se;
+    }
+    if (se == st) break;  // cannot happen (the first step always fits); never loop without progress
+    for (int e = tid; e < nb * nb; e += EIG_THREADS) hb[e] = H[(long long)(b0 + e / nb) * n + b0 + e % nb];
+    __syncthreads();
+    unsigned long long t_c0 = acc ? wall_clock64() : 0;
+    if (tid < 64) {
+      for (int step = st; step < se; ++step) {
+        int kj[2], cs[2], cntR[2], cntC[2];
+        double cj[2];
+        cd sj[2], rj[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int k = l + step - EIG_SEP * j;
+          const bool act = (k >= l && k < ihi);
+          kj[j] = k;
+          cd x = mkc(1.0, 0.0), y = mkc(0.0, 0.0);
+          if (act) {
+            const int kk = k - b0;
+            if (k == l) {
+              x = hb[kk * nb + kk] - (j == 0 ? s0 : s1);
+              y = hb[(kk + 1) * nb + kk];
+            } else {
+              x = hb[kk * nb + kk - 1];
+              y = hb[(kk + 1) * nb + kk - 1];
+            }
+          }
+          givens(x, y, cj[j], sj[j], rj[j]);
+          cs[j] = (k > l) ? k - 1 : k;
+          cntR[j] = act ? b1 - cs[j] + 1 : 0;
+          cntC[j] = act ? min(k + 2, b1) - b0 + 1 : 0;
+          if (act && tid == 0) {
+            if (j == 0) {
+              crot0[k] = cj[0];
+              srot0[k] = sj[0];
+            } else {
+              crot1[k] = cj[1];
+              srot1[k] = sj[1];
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // phase R: rows (k, k+1) of every active bulge, columns cs .. b1
+        for (int e = tid; e < cntR[0] + cntR[1]; e += 64) {
+          const int j = (e < cntR[0]) ? 0 : 1;
+          const int col = cs[j] + (j == 0 ? e : e - cntR[0]);
+          const int kk = kj[j] - b0, cc = col - b0;
+          if (col == kj[j] - 1) {  // the annihilated bulge entry
+            hb[kk * nb + cc] = rj[j];
+            hb[(kk + 1) * nb + cc] = mkc(0.0, 0.0);
+          } else {
+            cd xk = hb[kk * nb + cc], yk = hb[(kk + 1) * nb + cc];
+            rot_rows(xk, yk, cj[j], sj[j]);
+            hb[kk * nb + cc] = xk;
+            hb[(kk + 1) * nb + cc] = yk;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // phase C: columns (k, k+1) of every active bulge, rows b0 .. min(k+2, b1)
+        for (int e = tid; e < cntC[0] + cntC[1]; e += 64) {
+          const int j = (e < cntC[0]) ? 0 : 1;
+          const int row = b0 + (j == 0 ? e : e - cntC[0]);
+          const int kk = kj[j] - b0;
+          cd* pr = hb + (row - b0) * nb + kk;
+          cd xk = pr[0], yk = pr[1];
+          rot_cols(xk, yk, cj[j], sj[j]);
+          pr[0] = xk;
+          pr[1] = yk;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    __syncthreads();
+    if (acc) acc[1] += wall_clock64() - t_c0;
+    for (int e = tid; e < nb * nb; e += EIG_THREADS) H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[e];
+    // the block's rotations on the rest of the window, leading bulge first
+    const int nabove = b0 - l, nright = ihi - b1;
+    const int ka0 = max(l, l + st), kb0 = min(ihi - 1, l + se - 1);
+    const int ka1 = max(l, l + st - EIG_SEP), kb1 = min(ihi - 1, l + se - 1 - EIG_SEP);
+    for (int e = tid; e < nabove + nright; e += EIG_THREADS) {
+      if (e < nabove) {
+        cd* v = H + (long long)(l + e) * n;
+        if (ka0 <= kb0) slide<false>(v, 1, ka0, kb0 + 1, crot0, srot0);
+        if (ka1 <= kb1) slide<false>(v, 1, ka1, kb1 + 1, crot1, srot1);
+      } else {
+        cd* v = H + b1 + 1 + (e - nabove);
+        if (ka0 <= kb0) slide<true>(v, n, ka0, kb0 + 1, crot0, srot0);
+        if (ka1 <= kb1) slide<true>(v, n, ka1, kb1 + 1, crot1, srot1);
+      }
+    }
+    __syncthreads();
+    st = se;
+  }
+}
+
 // Stages 1-3.  A (n x n) becomes T (upper triangle); Zt gets the transposed
 // Schur vectors; X the eigenvectors of T (upper triangular, unit diagonal);
 // lam the eigenvalues.
@@ -618,6 +805,8 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   cd* srot = part + eig_part_elems(n, EIG_WAVES);  // n rotation sines (part doubles as the QR chase block)
   double* crot = reinterpret_cast<double*>(srot + n);  // n cosines
   cd* hl = srot + n + (n + 1) / 2;                      // n*n LDS copy of H (small n)
+  cd* srot2 = hl;                                       // second bulge's rotations (n > KHX_EIG_LDS_N)
+  double* crot2 = reinterpret_cast<double*>(srot2 + n);
   __shared__ double red[EIG_WAVES];
   __shared__ int redi[EIG_WAVES];
   __shared__ cd s_xy[2][2];  // double-buffered inputs of the next rotation
@@ -728,6 +917,7 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   cd* log_s = X;
   double* log_c = reinterpret_cast<double*>(X + (long long)EIG_KLOG * n);
   const int itmax = 30 * max(10, n);
+  const bool two_bulge = !in_lds && (flags & 2) != 0;
   while (ihi >= 1) {
     const unsigned long long t_s0 = tp ? wall_clock64() : 0;
     int lk = 0;
@@ -748,7 +938,8 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       break;
     }
     ++its;
-    cd shift;
+    cd shift, shift2 = mkc(0.0, 0.0);
+    bool two = false;  // two-bulge sweep with both eigenvalues of the trailing 2x2
     if (its % 20 == 10) {  // exceptional shifts break cycles
       shift = H[l * n + l] + mkc(0.75 * fabs(H[(l + 1) * n + l].x), 0.0);
     } else if (its % 20 == 0) {
@@ -766,6 +957,8 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
         const cd den = abs2c(p1) >= abs2c(p2) ? p1 : p2;
         shift = (den.x == 0.0 && den.y == 0.0) ? d : d - divc(bc, den);
       }
+      shift2 = (a + d) - shift;  // the other eigenvalue of the trailing 2x2
+      two = two_bulge && (ihi - l + 1 >= EIG_TWO_MIN_W);
     }
     if (!in_lds) {
       unsigned long long t1 = tp ? wall_clock64() : 0;
@@ -777,21 +970,38 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       }
       unsigned long long t2 = tp ? wall_clock64() : 0;
       if (tp) acc_t[4] += t2 - t1;
-      qr_sweep_blocked<EIG_THREADS>(H, Zt, n, l, ihi, shift, part, crot, srot, &s_xy[0][0], (flags & 1) != 0,
-                                    tp ? acc_t : nullptr);
+      if (two && nlog > EIG_KLOG - 2) {  // room for both bulges' rotations in the log
+        flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s, part);
+        nlog = 0;
+      }
+      if (two)
+        qr_sweep2_blocked<EIG_THREADS>(H, n, l, ihi, shift, shift2, part, crot, srot, crot2, srot2,
+                                       tp ? acc_t : nullptr);
+      else
+        qr_sweep_blocked<EIG_THREADS>(H, Zt, n, l, ihi, shift, part, crot, srot, &s_xy[0][0], (flags & 1) != 0,
+                                      tp ? acc_t : nullptr);
       unsigned long long t3 = tp ? wall_clock64() : 0;
       if (tp) acc_t[2] += t3 - t2;
       for (int k = l + tid; k < ihi; k += EIG_THREADS) {
         log_c[nlog * n + k] = crot[k];
         log_s[nlog * n + k] = srot[k];
+        if (two) {
+          log_c[(nlog + 1) * n + k] = crot2[k];
+          log_s[(nlog + 1) * n + k] = srot2[k];
+        }
       }
       if (tid == 0) {
         s_lo[nlog] = l;
         s_hi[nlog] = ihi;
+        if (two) {
+          s_lo[nlog + 1] = l;
+          s_hi[nlog + 1] = ihi;
+        }
       }
       __syncthreads();
       unsigned long long t4 = tp ? wall_clock64() : 0;
       if (tp) acc_t[3] += t4 - t3;
+      nlog += two ? 1 : 0;
       if (++nlog == EIG_KLOG) {
         flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s, part);
         nlog = 0;
@@ -949,6 +1159,7 @@ size_t zeig_workspace_bytes(int n, int batch) {
 static size_t eig_lds_bytes(int n, int threads) {
   size_t b = (size_t)(3 * n + eig_wv_elems(n)) * sizeof(cd) + (size_t)eig_part_elems(n, threads / 64) * sizeof(cd) +
              (size_t)((n + 1) / 2) * sizeof(cd);
+  if (n > KHX_EIG_LDS_N) b += (size_t)(n + (n + 1) / 2) * sizeof(cd);  // rotations of the second bulge
   if (n <= KHX_EIG_LDS_N) b += (size_t)n * n * sizeof(cd);
   return b;
 }
@@ -971,7 +1182,11 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   const char* pad_env = getenv("KHX_EIG_LDS_PAD");
   // bit 0: chase each LDS block on one wavefront (default on; KHX_EIG_CHASE_WAVE=0 turns it off)
   const char* cw_env = getenv("KHX_EIG_CHASE_WAVE");
-  const int flags = (cw_env && atoi(cw_env) == 0) ? 0 : 1;
+  // bit 1: two shifts (two bulges) per sweep (KHX_EIG_TWO_SHIFT=1; off by default: on the layer
+  // eigenproblems it needs about 1.5x more shifts than single-shift sweeps, which outweighs the
+  // shared block overheads)
+  const char* ts_env = getenv("KHX_EIG_TWO_SHIFT");
+  const int flags = ((cw_env && atoi(cw_env) == 0) ? 0 : 1) | ((ts_env && atoi(ts_env) != 0) ? 2 : 0);
   const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
   KHX_REQUIRE(lds <= 150 * 1024, "zeig: n too large for the LDS work vectors");
   static size_t lds_limit[5] = {0, 0, 0, 0, 0};
