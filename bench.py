@@ -208,6 +208,8 @@ def main():
     ap.add_argument("--solves", type=int, default=65536, help="subset of the workload (default: all)")
     ap.add_argument("--cpu-sample", type=int, default=40, help="oracle solves timed for cpu_baseline (0: skip)")
     ap.add_argument("--b1-sample", type=int, default=24, help="solves timed one at a time for the plain batch-1 line")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="run one plain device copy of this many bytes before the workload (byte-counter calibration)")
     ap.add_argument("--workload", default="disc_stack_bz",
                     help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted | grad_grid")
     ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
@@ -224,6 +226,13 @@ def main():
     from khepri_amd import _lib
 
     lib = _lib.load()
+    if args.copy_bytes > 0:
+        nb = (args.copy_bytes // 16) * 16
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        _lib.check(lib.khx_copy(src.data_ptr(), dst.data_ptr(), nb, torch.cuda.current_stream().cuda_stream), "copy")
+        torch.cuda.synchronize()
+        del src, dst
     if args.workload != "disc_stack_bz":
         print(json.dumps(extra_workload(args.workload, args)), flush=True)
         return
