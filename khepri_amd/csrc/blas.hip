@@ -280,8 +280,9 @@ __global__ __launch_bounds__(GESV_THREADS) void zgesv_kernel(int n, int m, cd* A
 
 int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info) {
   if (batch <= 0 || n <= 0) return 0;
-  if (n >= 24 && lu_blocked_fits(n) && getenv("KHX_LU_UNBLOCKED") == nullptr)
-    return zlu_blocked(s, n, m, A, B, batch, nullptr, info, false);
+  const int path = lu_path(n);
+  if (path == 2 && (batch == 1 || (A.stride != 0 && B.stride != 0))) return zlu_multi(s, n, m, A, B, batch, nullptr, info, false);
+  if (path >= 1) return zlu_blocked(s, n, m, A, B, batch, nullptr, info, false);
   KHX_REQUIRE(A.stride != 0 && B.stride != 0 || batch == 1, "zgesv: operands cannot be broadcast");
   const size_t lds = (size_t)(2 * n + m) * sizeof(cd);
   KHX_REQUIRE(lds <= 150 * 1024, "zgesv: system too large for the LDS staging buffers");

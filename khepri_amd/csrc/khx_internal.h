@@ -31,6 +31,11 @@ int zgetrf_solve(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv
 
 // blocked LU (one workgroup per system, panel in LDS): keepL -> zgetrf layout
 bool lu_blocked_fits(int n);
+// multi-launch blocked LU (panel kernel + batched zgemm trailing updates)
+bool lu_multi_fits(int n);
+int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL);
+// LU path selection: 0 = unblocked, 1 = one-workgroup blocked, 2 = multi-launch blocked
+int lu_path(int n);
 int zlu_blocked(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL);
 
 // elementwise helpers (all batched, n_rows x n_cols views)

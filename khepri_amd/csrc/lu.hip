@@ -229,8 +229,9 @@ __global__ __launch_bounds__(64) void zgetrs_vec_kernel(int n, int m, const cd* 
 
 int zgetrf(hipStream_t s, int n, Mat A, int batch, int* ipiv, int* info) {
   if (batch <= 0 || n <= 0) return 0;
-  if (n >= 24 && lu_blocked_fits(n) && getenv("KHX_LU_UNBLOCKED") == nullptr)
-    return zlu_blocked(s, n, 0, A, Mat{nullptr, 0, 0}, batch, ipiv, info, true);
+  const int path = lu_path(n);
+  if (path == 2) return zlu_multi(s, n, 0, A, Mat{nullptr, 0, 0}, batch, ipiv, info, true);
+  if (path == 1) return zlu_blocked(s, n, 0, A, Mat{nullptr, 0, 0}, batch, ipiv, info, true);
   const size_t lds = (size_t)2 * n * sizeof(cd);
   KHX_REQUIRE(lds <= 64 * 1024, "zgetrf: n too large for the LDS staging buffers");
   ProfScope ps("zgetrf_kernel", s);
@@ -242,8 +243,9 @@ int zgetrf(hipStream_t s, int n, Mat A, int batch, int* ipiv, int* info) {
 // Factor and solve in one pass: A -> P L U (zgetrf layout, ipiv), B -> A^-1 B.
 int zgetrf_solve(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info) {
   if (batch <= 0 || n <= 0) return 0;
-  if (n >= 24 && lu_blocked_fits(n) && getenv("KHX_LU_UNBLOCKED") == nullptr)
-    return zlu_blocked(s, n, m, A, B, batch, ipiv, info, true);
+  const int path = lu_path(n);
+  if (path == 2) return zlu_multi(s, n, m, A, B, batch, ipiv, info, true);
+  if (path == 1) return zlu_blocked(s, n, m, A, B, batch, ipiv, info, true);
   KHX_TRY(zgetrf(s, n, A, batch, ipiv, info));
   return zgetrs(s, n, m, A, ipiv, B, batch);
 }

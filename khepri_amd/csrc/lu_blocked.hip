@@ -299,6 +299,224 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
   }
 }
 
+// ------------------------------------------------------------------ multi-launch variant
+// The same right-looking LU, one panel per launch: zlu_panel_kernel factors
+// the panel A[j0:n, j0:j0+16] in LDS, applies its interchanges to the rest of
+// the rows (and to B), and solves U12 = L11^-1 A12, Y1 = L11^-1 B1; the
+// trailing update [A22 | B2] -= L21 [U12 | Y1] is a batched zgemm launch, so
+// the bulk of the flops runs in 64 x 64 GEMM tiles spread over the whole GPU
+// instead of inside the one workgroup that owns the matrix.
+__global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
+                                                               cd* B, int ldb, long long sB, int* ipiv, int* info,
+                                                               int keepL) {
+  extern __shared__ cd smem[];
+  cd* pan = smem;  // (n - j0) x LB_NB
+  __shared__ double rv[LB_THREADS / 64];
+  __shared__ int ri[LB_THREADS / 64];
+  __shared__ cd rc[LB_THREADS / 64];
+  __shared__ int perm[LB_NB];
+  const long long item = blockIdx.x;
+  cd* a = A + item * sA;
+  cd* bm = m > 0 ? B + item * sB : nullptr;
+  const int tid = threadIdx.x;
+  int first_zero = 0;
+  const int nb = min(LB_NB, n - j0);
+  const int pr = n - j0;
+  for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
+    const int r = e / LB_NB, c = e % LB_NB;
+    pan[e] = (c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int jj = 0; jj < nb; ++jj) {
+    double best = -1.0;
+    int bi = pr;
+    cd bv = mkc(0.0, 0.0);
+    for (int r = jj + tid; r < pr; r += LB_THREADS) {
+      const cd v = pan[r * LB_NB + jj];
+      const double av = cabs1(v);
+      if (av > best) {
+        best = av;
+        bi = r;
+        bv = v;
+      }
+    }
+    argmax_abs1(best, bi, bv, rv, ri, rc);
+    int p = bi;
+    cd piv = bv;
+    if (p >= pr) {  // NaN column: keep the diagonal
+      p = jj;
+      piv = pan[jj * LB_NB + jj];
+    }
+    const bool zero = (piv.x == 0.0 && piv.y == 0.0);
+    if (zero && first_zero == 0) first_zero = j0 + jj + 1;
+    const cd rp = zero ? mkc(0.0, 0.0) : recipc(piv);
+    if (tid == 0) {
+      perm[jj] = p;
+      if (ipiv) ipiv[item * n + j0 + jj] = j0 + p;
+    }
+    __syncthreads();  // every thread has read the pivot column before the swap
+    if (p != jj && tid < LB_NB) {
+      const cd t = pan[jj * LB_NB + tid];
+      pan[jj * LB_NB + tid] = pan[p * LB_NB + tid];
+      pan[p * LB_NB + tid] = t;
+    }
+    __syncthreads();
+    for (int r = jj + 1 + tid; r < pr; r += LB_THREADS) {
+      cd* row = pan + r * LB_NB;
+      const cd l = row[jj] * rp;
+      row[jj] = l;
+      for (int c = jj + 1; c < nb; ++c) row[c] = fmsc(row[c], l, pan[jj * LB_NB + c]);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
+    const int r = e / LB_NB, c = e % LB_NB;
+    if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[e];
+  }
+  if (tid == 0 && info) {
+    if (j0 == 0)
+      info[item] = first_zero;
+    else if (info[item] == 0)
+      info[item] = first_zero;
+  }
+  // interchanges on the other columns and the triangular solve with L11,
+  // one thread per column
+  const int nleft = keepL ? j0 : 0;
+  const int nright = n - j0 - nb;
+  for (int e = tid; e < nleft + nright + m; e += LB_THREADS) {
+    cd* col;
+    long long ld;
+    if (e < nleft) {
+      col = a + e;
+      ld = lda;
+    } else if (e < nleft + nright) {
+      col = a + j0 + nb + (e - nleft);
+      ld = lda;
+    } else {
+      col = bm + (e - nleft - nright);
+      ld = ldb;
+    }
+    for (int jj = 0; jj < nb; ++jj) {
+      const int p = perm[jj];
+      if (p != jj) {
+        cd* x = col + (long long)(j0 + jj) * ld;
+        cd* y = col + (long long)(j0 + p) * ld;
+        const cd t = *x;
+        *x = *y;
+        *y = t;
+      }
+    }
+    if (e >= nleft) {
+      cd x[LB_NB];
+#pragma unroll
+      for (int i = 0; i < LB_NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ld] : mkc(0.0, 0.0);
+#pragma unroll
+      for (int i = 1; i < LB_NB; ++i)
+#pragma unroll
+        for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], pan[i * LB_NB + k], x[k]);
+#pragma unroll
+      for (int i = 0; i < LB_NB; ++i)
+        if (i < nb) col[(long long)(j0 + i) * ld] = x[i];
+    }
+  }
+}
+
+// X block = U11^-1 Y block for the rows j0 .. j0+nb of B (one thread per column)
+__global__ __launch_bounds__(LB_THREADS) void zlu_upper_block_kernel(int n, int m, int j0, const cd* A, int lda,
+                                                                     long long sA, cd* B, int ldb, long long sB) {
+  __shared__ cd u[LB_NB * LB_NB];
+  const long long item = blockIdx.y;
+  const cd* a = A + item * sA;
+  cd* bm = B + item * sB;
+  const int nb = min(LB_NB, n - j0);
+  for (int e = threadIdx.x; e < LB_NB * LB_NB; e += blockDim.x) {
+    const int r = e / LB_NB, c = e % LB_NB;
+    u[e] = (r < nb && c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(r == c ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  cd* col = bm + c;
+  cd x[LB_NB];
+#pragma unroll
+  for (int i = 0; i < LB_NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ldb] : mkc(0.0, 0.0);
+#pragma unroll
+  for (int i = LB_NB - 1; i >= 0; --i) {
+#pragma unroll
+    for (int k = i + 1; k < LB_NB; ++k) x[i] = fmsc(x[i], u[i * LB_NB + k], x[k]);
+    x[i] = divc(x[i], u[i * LB_NB + i]);
+  }
+#pragma unroll
+  for (int i = 0; i < LB_NB; ++i)
+    if (i < nb) col[(long long)(j0 + i) * ldb] = x[i];
+}
+
+static size_t lu_panel_lds(int n) { return (size_t)n * LB_NB * sizeof(cd); }
+
+bool lu_multi_fits(int n) { return lu_panel_lds(n) <= 150 * 1024; }
+
+int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL) {
+  if (batch <= 0 || n <= 0) return 0;
+  KHX_REQUIRE(A.stride != 0 && (m == 0 || B.stride != 0) || batch == 1, "zlu_multi: operands cannot be broadcast");
+  KHX_REQUIRE(batch <= 65535, "zlu_multi: at most 65535 systems per call");
+  const size_t lds = lu_panel_lds(n);
+  KHX_REQUIRE(lds <= 150 * 1024, "zlu_multi: n too large for the LDS panel");
+  static size_t lds_limit = 0;
+  KHX_TRY(ensure_dynamic_lds((const void*)zlu_panel_kernel, lds, lds_limit));
+  const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
+  ProfScope ps(m > 0 ? "zgesv_kernel" : "zgetrf_kernel", s);
+  for (int j0 = 0; j0 < n; j0 += LB_NB) {
+    const int nb = min(LB_NB, n - j0);
+    zlu_panel_kernel<<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld, B.stride,
+                                                               ipiv, info, keepL ? 1 : 0);
+    KHX_LAUNCH_CHECK();
+    const int r1 = j0 + nb, rest = n - r1;
+    if (rest > 0) {
+      const Mat L21{A.p + (long long)r1 * A.ld + j0, A.ld, A.stride};
+      const Mat U12{A.p + (long long)j0 * A.ld + r1, A.ld, A.stride};
+      const Mat A22{A.p + (long long)r1 * A.ld + r1, A.ld, A.stride};
+      KHX_TRY(zgemm(s, rest, rest, nb, mone, L21, U12, one, A22, batch));
+      if (m > 0) {
+        const Mat Y1{B.p + (long long)j0 * B.ld, B.ld, B.stride};
+        const Mat B2{B.p + (long long)r1 * B.ld, B.ld, B.stride};
+        KHX_TRY(zgemm(s, rest, m, nb, mone, L21, Y1, one, B2, batch));
+      }
+    }
+  }
+  if (m <= 0) return 0;
+  // back substitution U X = Y, LB_NB rows at a time from the bottom
+  const int nblk = (n + LB_NB - 1) / LB_NB;
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int j0 = blk * LB_NB;
+    const int nb = min(LB_NB, n - j0);
+    zlu_upper_block_kernel<<<dim3((m + LB_THREADS - 1) / LB_THREADS, batch), dim3(LB_THREADS), 0, s>>>(
+        n, m, j0, A.p, A.ld, A.stride, B.p, B.ld, B.stride);
+    KHX_LAUNCH_CHECK();
+    if (j0 > 0) {
+      const Mat U01{A.p + j0, A.ld, A.stride};
+      const Mat X1{B.p + (long long)j0 * B.ld, B.ld, B.stride};
+      KHX_TRY(zgemm(s, j0, m, nb, mone, U01, X1, one, B, batch));
+    }
+  }
+  return 0;
+}
+
+// KHX_LU_PATH=0/1/2 forces a path; by default the multi-launch variant takes
+// the larger systems (its trailing updates fill the GPU even for few systems)
+int lu_path(int n) {
+  const char* env = getenv("KHX_LU_PATH");
+  if (env) {
+    const int p = atoi(env);
+    if (p == 2 && lu_multi_fits(n)) return 2;
+    if (p >= 1 && n >= 24 && lu_blocked_fits(n)) return 1;
+    return 0;
+  }
+  if (getenv("KHX_LU_UNBLOCKED")) return 0;
+  if (n >= 96 && lu_multi_fits(n)) return 2;
+  if (n >= 24 && lu_blocked_fits(n)) return 1;
+  return 0;
+}
+
 static size_t lu_blocked_lds(int n) { return ((size_t)n * LB_NB + (size_t)LB_NB * LB_TW) * sizeof(cd); }
 
 bool lu_blocked_fits(int n) { return lu_blocked_lds(n) <= 150 * 1024; }
