@@ -306,11 +306,12 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
 // trailing update [A22 | B2] -= L21 [U12 | Y1] is a batched zgemm launch, so
 // the bulk of the flops runs in 64 x 64 GEMM tiles spread over the whole GPU
 // instead of inside the one workgroup that owns the matrix.
+// GP: the panel stays in global memory (systems too large for an LDS panel)
+template <bool GP>
 __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
                                                                cd* B, int ldb, long long sB, int* ipiv, int* info,
                                                                int keepL) {
   extern __shared__ cd smem[];
-  cd* pan = smem;  // (n - j0) x LB_NB
   __shared__ double rv[LB_THREADS / 64];
   __shared__ int ri[LB_THREADS / 64];
   __shared__ cd rc[LB_THREADS / 64];
@@ -322,9 +323,14 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
   int first_zero = 0;
   const int nb = min(LB_NB, n - j0);
   const int pr = n - j0;
-  for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
-    const int r = e / LB_NB, c = e % LB_NB;
-    pan[e] = (c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
+  // panel element (r, c): LDS copy (row stride LB_NB) or the matrix itself
+  cd* pan = GP ? a + (long long)j0 * lda + j0 : smem;
+  const long long pld = GP ? lda : LB_NB;
+  if (!GP) {
+    for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
+      const int r = e / LB_NB, c = e % LB_NB;
+      pan[e] = (c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
+    }
   }
   __syncthreads();
   for (int jj = 0; jj < nb; ++jj) {
@@ -332,7 +338,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
     int bi = pr;
     cd bv = mkc(0.0, 0.0);
     for (int r = jj + tid; r < pr; r += LB_THREADS) {
-      const cd v = pan[r * LB_NB + jj];
+      const cd v = pan[r * pld + jj];
       const double av = cabs1(v);
       if (av > best) {
         best = av;
@@ -345,7 +351,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
     cd piv = bv;
     if (p >= pr) {  // NaN column: keep the diagonal
       p = jj;
-      piv = pan[jj * LB_NB + jj];
+      piv = pan[jj * pld + jj];
     }
     const bool zero = (piv.x == 0.0 && piv.y == 0.0);
     if (zero && first_zero == 0) first_zero = j0 + jj + 1;
@@ -355,23 +361,25 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
       if (ipiv) ipiv[item * n + j0 + jj] = j0 + p;
     }
     __syncthreads();  // every thread has read the pivot column before the swap
-    if (p != jj && tid < LB_NB) {
-      const cd t = pan[jj * LB_NB + tid];
-      pan[jj * LB_NB + tid] = pan[p * LB_NB + tid];
-      pan[p * LB_NB + tid] = t;
+    if (p != jj && tid < nb) {
+      const cd t = pan[jj * pld + tid];
+      pan[jj * pld + tid] = pan[p * pld + tid];
+      pan[p * pld + tid] = t;
     }
     __syncthreads();
     for (int r = jj + 1 + tid; r < pr; r += LB_THREADS) {
-      cd* row = pan + r * LB_NB;
+      cd* row = pan + r * pld;
       const cd l = row[jj] * rp;
       row[jj] = l;
-      for (int c = jj + 1; c < nb; ++c) row[c] = fmsc(row[c], l, pan[jj * LB_NB + c]);
+      for (int c = jj + 1; c < nb; ++c) row[c] = fmsc(row[c], l, pan[jj * pld + c]);
     }
     __syncthreads();
   }
-  for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
-    const int r = e / LB_NB, c = e % LB_NB;
-    if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[e];
+  if (!GP) {
+    for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
+      const int r = e / LB_NB, c = e % LB_NB;
+      if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[e];
+    }
   }
   if (tid == 0 && info) {
     if (j0 == 0)
@@ -412,8 +420,10 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
       for (int i = 0; i < LB_NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ld] : mkc(0.0, 0.0);
 #pragma unroll
       for (int i = 1; i < LB_NB; ++i)
+        if (i < nb) {  // the last panel may be narrower: rows past it are outside the matrix
 #pragma unroll
-        for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], pan[i * LB_NB + k], x[k]);
+          for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], pan[i * pld + k], x[k]);
+        }
 #pragma unroll
       for (int i = 0; i < LB_NB; ++i)
         if (i < nb) col[(long long)(j0 + i) * ld] = x[i];
@@ -453,22 +463,26 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_upper_block_kernel(int n, int 
 
 static size_t lu_panel_lds(int n) { return (size_t)n * LB_NB * sizeof(cd); }
 
-bool lu_multi_fits(int n) { return lu_panel_lds(n) <= 150 * 1024; }
+bool lu_multi_fits(int n) { return n >= 1; }  // larger systems keep the panel in global memory
 
 int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL) {
   if (batch <= 0 || n <= 0) return 0;
   KHX_REQUIRE(A.stride != 0 && (m == 0 || B.stride != 0) || batch == 1, "zlu_multi: operands cannot be broadcast");
   KHX_REQUIRE(batch <= 65535, "zlu_multi: at most 65535 systems per call");
-  const size_t lds = lu_panel_lds(n);
-  KHX_REQUIRE(lds <= 150 * 1024, "zlu_multi: n too large for the LDS panel");
+  const bool gp = lu_panel_lds(n) > 150 * 1024;
+  const size_t lds = gp ? 0 : lu_panel_lds(n);
   static size_t lds_limit = 0;
-  KHX_TRY(ensure_dynamic_lds((const void*)zlu_panel_kernel, lds, lds_limit));
+  if (!gp) KHX_TRY(ensure_dynamic_lds((const void*)zlu_panel_kernel<false>, lds, lds_limit));
   const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
   ProfScope ps(m > 0 ? "zgesv_kernel" : "zgetrf_kernel", s);
   for (int j0 = 0; j0 < n; j0 += LB_NB) {
     const int nb = min(LB_NB, n - j0);
-    zlu_panel_kernel<<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld, B.stride,
-                                                               ipiv, info, keepL ? 1 : 0);
+    if (gp)
+      zlu_panel_kernel<true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
+                                                                      B.stride, ipiv, info, keepL ? 1 : 0);
+    else
+      zlu_panel_kernel<false><<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
+                                                                         B.stride, ipiv, info, keepL ? 1 : 0);
     KHX_LAUNCH_CHECK();
     const int r1 = j0 + nb, rest = n - r1;
     if (rest > 0) {
