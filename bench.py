@@ -142,22 +142,30 @@ def extra_workload(name, args):
             return batch.solve()
         units, unit = len(cls), "devices/s"
     elif name == "twisted":
+        # every twist angle is its own extended expansion (625 harmonics): one Crystal per angle,
+        # each solving all frequencies in one batch
         cfg = workloads.twisted_bilayer()
-        e1, e2 = Expansion(cfg["pw"]), Expansion(cfg["pw"])
-        e2.rotate(np.deg2rad(cfg["angles_deg"][0]))
-        etw = e1 + e2
-        cl = Crystal.from_expansion(etw)
-        cl.add_layer("L1", Layer.pixmap(e1, cfg["pixmap"], cfg["depth"]), extended=True)
-        cl.add_layer("gap", Layer.uniform(etw, 1.0, cfg["gap"]))
-        cl.add_layer("L2", Layer.pixmap(e2, cfg["pixmap"], cfg["depth"]), extended=True)
-        cl.set_device(["L1", "gap", "L2"], False)
         fr = cfg["freqs"][:args.freqs]
-        cl.set_source(1 / fr, 1.0, 0.0)
+        cls = []
+        for ang in cfg["angles_deg"][:args.angles]:
+            e1, e2 = Expansion(cfg["pw"]), Expansion(cfg["pw"])
+            e2.rotate(np.deg2rad(ang))
+            etw = e1 + e2
+            cl = Crystal.from_expansion(etw)
+            cl.add_layer("L1", Layer.pixmap(e1, cfg["pixmap"], cfg["depth"]), extended=True)
+            cl.add_layer("gap", Layer.uniform(etw, 1.0, cfg["gap"]))
+            cl.add_layer("L2", Layer.pixmap(e2, cfg["pixmap"], cfg["depth"]), extended=True)
+            cl.set_device(["L1", "gap", "L2"], False)
+            cl.set_source(1 / fr, 1.0, 0.0)
+            cls.append(cl)
 
         def step():
-            cl.solve(rt_only=True)
-            return cl.poynting_flux_end()
-        units, unit = len(fr), "solves/s"
+            out = []
+            for cl in cls:
+                cl.solve(rt_only=True)
+                out.append(cl.poynting_flux_end())
+            return out
+        units, unit = len(fr) * len(cls), "solves/s"
     elif name == "grad_grid":
         from khepri_amd.adjoint import flux_gradient
         cfg = workloads.grad_grid()
@@ -214,6 +222,7 @@ def main():
                     help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted | grad_grid")
     ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
     ap.add_argument("--freqs", type=int, default=32, help="twisted: number of frequencies")
+    ap.add_argument("--angles", type=int, default=8, help="twisted: number of twist angles")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
