@@ -183,6 +183,7 @@ def extra_workload(name, args):
         units, unit = len(cfg["freqs"]), "grad evals/s"
     else:
         raise SystemExit(f"unknown workload {name}")
+    tuned = run_autotune(step, args)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -194,16 +195,33 @@ def extra_workload(name, args):
     return {"metric": f"{name} throughput", "value": round(units * args.steps / el, 3), "unit": unit,
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
             "higher_is_better": True, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": name, "units_per_step": units}}
+            "config": {"workload": name, "units_per_step": units}, "gemm_autotune": tuned}
 
 
-def load_traffic(kernel, workload):
+def run_autotune(step, args):
+    """Record the GEMMs of one (untimed) step, time the tile shapes on each and
+    install the fastest; returns a summary for the JSON line (None when off)."""
+    if not args.autotune:
+        return None
+    from khepri_amd import autotune
+    t0 = time.perf_counter()
+    table = autotune.autotune(step)
+    changed = sum(1 for e in table.values() if e["tile"] != (1 if min(e["shape"][:2]) <= 40 else 0))
+    print(f"[bench] gemm autotune: {len(table)} classes, {changed} changed", file=sys.stderr, flush=True)
+    return {"classes": len(table), "changed": changed, "seconds": round(time.perf_counter() - t0, 2)}
+
+
+def load_traffic(kernel, workload, matrices_per_launch):
+    """HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE profile of
+    the same workload and launch size (None when the profile does not match)."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
     if d.get("workload") != workload:
+        return None
+    if abs(d.get("matrices_per_launch", 0) - matrices_per_launch) > 0.01 * matrices_per_launch:
         return None
     return d.get("bytes_per_launch")
 
@@ -223,6 +241,9 @@ def main():
     ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
     ap.add_argument("--freqs", type=int, default=32, help="twisted: number of frequencies")
     ap.add_argument("--angles", type=int, default=8, help="twisted: number of twist angles")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1 (default): pick GEMM tile shapes with khepri_amd.autotune on one untimed step first; 0: "
+                         "built-in tile rule")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -249,6 +270,11 @@ def main():
     nsol = len(idx)
     n = 2 * 81
     print(f"[bench] {nsol} solves per step, warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
+
+    def solve_step():
+        cl.solve()
+        return cl.poynting_flux_end()
+    tuned = run_autotune(solve_step, args)
     for w in range(args.warmup):
         cl.solve()
         R, T = cl.poynting_flux_end()
@@ -280,7 +306,7 @@ def main():
     workload = "disc_stack_bz pw(9,9) 65536 solves"
     roofline = dict(bound="mfma", kernel="zeig_schur_kernel", achieved=round(achieved, 4), peak=PEAK_FP64_TFLOPS,
                     unit="TFLOP/s", frac=round(achieved / PEAK_FP64_TFLOPS, 6),
-                    traffic=load_traffic("zeig_schur_kernel", workload),
+                    traffic=load_traffic("zeig_schur_kernel", workload, matrices_per_launch),
                     avg_launch_ms=round(avg_ms, 4), matrices_per_launch=matrices_per_launch,
                     flops_per_matrix=eig_flops(n))
     share = {k: round(v[0] / (el * 1e3 * 1.0), 4) for k, v in kern.items()}
@@ -294,6 +320,7 @@ def main():
                    "precision": "fp64 (plain)", "chunk": cl._chunk(nsol)},
         "roofline": roofline,
         "kernel_time_share": share,
+        "gemm_autotune": tuned,
     }
     if args.b1_sample > 0:
         pb = plain_batch1(cfg, idx, args.b1_sample)

@@ -257,6 +257,25 @@ int khx_poynting_vec_adjoint(const double* sR, const double* sT, int N, const do
                              const double* wl, const double* epsi_host, const double* epse_host, const double* wR,
                              const double* wT, int batch, double* sRbar, double* sTbar, void* stream);
 
+/* Number of matrices khx_eig keeps in flight at once on the current device
+ * (resident workgroups per CU x CUs; no counterpart in the reference).  A
+ * batch that is a multiple of it leaves no partly filled last round; callers
+ * that stream a large batch in chunks size the chunks by it.  Returns the
+ * count (>= 1) or a negative status.                                        */
+int khx_eig_slots(int n);
+
+/* GEMM tile autotuning (no counterpart in the reference).  Tile shapes:
+ * 0 = 64x64, 1 = 32x32, 2 = 64x32, 3 = 32x64 (M x N per workgroup).
+ * khx_gemm_record(1, NULL, 0) starts recording the distinct GEMM shapes the
+ * library runs; khx_gemm_record(0, shapes, max) stops and copies up to max
+ * records of 6 ints (M, N, K, opA, opB, batch) into shapes, returning the
+ * count.  khx_gemm_force(cfg) uses tile cfg for every GEMM (-1: off);
+ * khx_gemm_tune_set installs the choice for one shape class (batch is
+ * bucketed: < 16, < 256, >= 256; cfg -1 removes it).                       */
+int khx_gemm_record(int on, int* shapes, int max_shapes);
+int khx_gemm_force(int cfg);
+int khx_gemm_tune_set(int M, int N, int K, int opa, int opb, int batch, int cfg);
+
 /* Measurement support (no counterpart in the reference).
  * khx_prof_enable(1) clears the records and brackets every launch of the
  * instrumented kernels (zeig_schur_kernel, zgesv_kernel, zgemm_kernel) with

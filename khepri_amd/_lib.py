@@ -67,6 +67,10 @@ SIGNATURES = {
                                      _vp, _vp]),
     "khx_redheffer_bwd_ws_bytes": (_sz, [_i, _i]),
     "khx_redheffer_bwd": (_i, [_dp, _dp, _i, _i, _dp, _dp, _dp, _ip, _vp, _vp]),
+    "khx_eig_slots": (_i, [_i]),
+    "khx_gemm_record": (_i, [_i, _vp, _i]),
+    "khx_gemm_force": (_i, [_i]),
+    "khx_gemm_tune_set": (_i, [_i, _i, _i, _i, _i, _i, _i]),
     "khx_prof_enable": (_i, [_i]),
     "khx_prof_collect": (_i, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "khx_copy": (_i, [_vp, _vp, _sz, _vp]),

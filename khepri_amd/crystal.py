@@ -19,7 +19,7 @@ from . import chain, ops
 from .device import default_device
 from .expansion import Expansion, compute_kplanar
 from .extension import ExtendedLayer
-from .layer import Layer, stack_layers
+from .layer import Formulation, Layer, stack_layers
 
 
 def _bytes_per_item(n, unique, length):
@@ -166,7 +166,18 @@ class Crystal:
         budget = float(os.environ.get("KHX_MEM_BUDGET_GB", "48")) * 2 ** 30
         n = 2 * self.expansion.N
         per = _bytes_per_item(n, len(set(self.global_stacking)), len(self.global_stacking))
-        return int(max(1, min(B, budget // per)))
+        chunk = int(max(1, min(B, budget // per)))
+        if chunk < B and self._has_patterned():
+            # every chunk's eigensolver launch is a whole number of rounds of
+            # resident workgroups: a partly filled last round costs a full one
+            slots = ops.eig_slots(n)
+            if chunk >= slots:
+                chunk = chunk // slots * slots
+        return chunk
+
+    def _has_patterned(self):
+        return any(getattr(self.layers[name], "formulation", None) == Formulation.FFT
+                   for name in set(self.global_stacking) if name in self.layers)
 
     def solve(self, rt_only=None):
         """khepri/crystal.py:180-206.

@@ -6,7 +6,11 @@
 #include "khx_internal.h"
 #include "khx_prof.h"
 
+#include <array>
 #include <cstdlib>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
 
 namespace khx {
 
@@ -113,11 +117,81 @@ static int launch_zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat
   return 0;
 }
 
+// ---- tile selection.  Four tile shapes (all 256 threads):
+//   0: 64 x 64 (4 x 4 per thread), 1: 32 x 32 (2 x 2), 2: 64 x 32 (4 x 2), 3: 32 x 64 (2 x 4).
+// The default picks 32 x 32 when a side is small; a per-shape choice installed by the
+// autotuner (khepri_amd/autotune.py, through khx_gemm_tune_set) takes precedence, and
+// khx_gemm_force overrides both while candidates are timed.  With recording on, every
+// distinct (M, N, K, opA, opB, batch class) passing through is kept for the autotuner.
+namespace {
+std::mutex g_tune_mu;
+std::unordered_map<unsigned long long, int> g_tune;
+std::vector<std::array<int, 6>> g_shapes;
+bool g_record = false;
+int g_force = -1;
+unsigned long long tune_key(int M, int N, int K, int oa, int ob, int batch) {
+  const unsigned long long bc = batch >= 256 ? 2 : (batch >= 16 ? 1 : 0);
+  return ((unsigned long long)M << 42) | ((unsigned long long)N << 26) | ((unsigned long long)K << 10) |
+         ((unsigned long long)(oa * 3 + ob) << 2) | bc;
+}
+}  // namespace
+
+int gemm_tile_choice(int M, int N, int K, int oa, int ob, int batch) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (g_force >= 0) return g_force;
+  const unsigned long long key = tune_key(M, N, K, oa, ob, batch);
+  if (g_record) {
+    bool seen = false;
+    for (const auto& sh : g_shapes)
+      if (tune_key(sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]) == key) seen = true;
+    if (!seen && g_shapes.size() < 4096) g_shapes.push_back({M, N, K, oa, ob, batch});
+  }
+  const auto it = g_tune.find(key);
+  if (it != g_tune.end()) return it->second;
+  return (M <= 40 || N <= 40) ? 1 : 0;
+}
+
 template <int OA, int OB>
 static int zgemm_sized(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
   if (batch <= 0 || M <= 0 || N <= 0) return 0;
-  if (M <= 40 || N <= 40) return launch_zgemm<32, 32, 16, 2, 2, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
-  return launch_zgemm<64, 64, 16, 4, 4, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+  switch (gemm_tile_choice(M, N, K, OA, OB, batch)) {
+    case 1: return launch_zgemm<32, 32, 16, 2, 2, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 2: return launch_zgemm<64, 32, 16, 4, 2, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 3: return launch_zgemm<32, 64, 16, 2, 4, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+    default: return launch_zgemm<64, 64, 16, 4, 4, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+  }
+}
+
+int gemm_tune_set(int M, int N, int K, int oa, int ob, int batch, int cfg) {
+  KHX_REQUIRE(cfg >= -1 && cfg <= 3, "gemm tile configuration must be 0..3 (or -1 to clear)");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const unsigned long long key = tune_key(M, N, K, oa, ob, batch);
+  if (cfg < 0)
+    g_tune.erase(key);
+  else
+    g_tune[key] = cfg;
+  return 0;
+}
+
+int gemm_force(int cfg) {
+  KHX_REQUIRE(cfg >= -1 && cfg <= 3, "gemm tile configuration must be 0..3 (or -1)");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_force = cfg;
+  return 0;
+}
+
+int gemm_record(int on, int* shapes, int max_shapes) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  int count = 0;
+  if (shapes)
+    for (const auto& sh : g_shapes) {
+      if (count >= max_shapes) break;
+      for (int q = 0; q < 6; ++q) shapes[count * 6 + q] = sh[q];
+      ++count;
+    }
+  if (on == 1) g_shapes.clear();
+  g_record = (on == 1);
+  return shapes ? count : (int)g_shapes.size();
 }
 
 int zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {

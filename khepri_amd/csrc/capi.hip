@@ -302,6 +302,29 @@ int khx_poynting_vec_adjoint(const double* sR, const double* sT, int N, const do
                                    as_cd(sRbar), as_cd(sTbar));
 }
 
+int khx_eig_slots(int n) {
+  KHX_ENTRY_BEGIN
+  return khx::zeig_slots(n);
+}
+
+int khx_gemm_record(int on, int* shapes, int max_shapes) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(max_shapes >= 0, "khx_gemm_record: max_shapes < 0");
+  return khx::gemm_record(on, shapes, max_shapes);
+}
+
+int khx_gemm_force(int cfg) {
+  KHX_ENTRY_BEGIN
+  return khx::gemm_force(cfg);
+}
+
+int khx_gemm_tune_set(int M, int N, int K, int opa, int opb, int batch, int cfg) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(M > 0 && N > 0 && K > 0 && opa >= 0 && opa <= 2 && opb >= 0 && opb <= 2 && batch > 0,
+              "khx_gemm_tune_set: bad shape");
+  return khx::gemm_tune_set(M, N, K, opa, opb, batch, cfg);
+}
+
 int khx_lu_factor(int n, double* A, int batch, int* ipiv, int* info, void* stream) {
   KHX_ENTRY_BEGIN
   KHX_REQUIRE(n > 0 && batch >= 0, "khx_lu_factor: bad sizes");

@@ -1164,6 +1164,30 @@ static size_t eig_lds_bytes(int n, int threads) {
   return b;
 }
 
+// Matrices the eigensolver keeps in flight at once on the current device
+// (resident workgroups per CU x CUs) with the default launch shape: a batch
+// that is a multiple of this fills every round of workgroups.
+int zeig_slots(int n) {
+  KHX_REQUIRE(n >= 1 && n <= 8192, "zeig_slots: bad order");
+  const char* env = getenv("KHX_EIG_THREADS");
+  const int threads = env ? atoi(env) : EIG_THREADS_DEFAULT;
+  const char* pad_env = getenv("KHX_EIG_LDS_PAD");
+  const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
+  KHX_REQUIRE(lds <= 150 * 1024, "zeig_slots: n too large for the LDS work vectors");
+  const void* fn = threads == 1024 ? (const void*)zeig_schur_kernel<1024>
+                   : threads == 512 ? (const void*)zeig_schur_kernel<512>
+                   : threads == 128 ? (const void*)zeig_schur_kernel<128>
+                   : threads == 64  ? (const void*)zeig_schur_kernel<64>
+                                    : (const void*)zeig_schur_kernel<256>;
+  static size_t lim = 0;
+  KHX_TRY(ensure_dynamic_lds(fn, lds, lim));
+  int per_cu = 0, dev = 0, cus = 0;
+  KHX_CHECK_HIP(hipGetDevice(&dev));
+  KHX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  KHX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds));
+  return per_cu * cus > 0 ? per_cu * cus : 1;
+}
+
 int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws,
          unsigned long long* tphase) {
   if (batch <= 0) return 0;

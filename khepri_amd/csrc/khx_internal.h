@@ -21,6 +21,10 @@ int zgemm_ta(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta
 // C = alpha op(A) op(B) + beta C with op 0 = X, 1 = X^T, 2 = X^H
 int zgemm_op(hipStream_t s, int opa, int opb, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C,
              int batch);
+// GEMM tile selection (tuning table, forced choice, shape recording)
+int gemm_tune_set(int M, int N, int K, int oa, int ob, int batch, int cfg);
+int gemm_force(int cfg);
+int gemm_record(int on, int* shapes, int max_shapes);
 // Solve A X = B (A n x n destroyed, B n x m overwritten); info[b] = first zero pivot (1-based) or 0
 int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info);
 
@@ -65,6 +69,8 @@ int pq_adjoint(hipStream_t s, const cd* Pbar, const cd* Qbar, const cd* kx, cons
 size_t zeig_workspace_bytes(int n, int batch);
 int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws,
          unsigned long long* tphase = nullptr);
+// matrices in flight at once on the current device (a full round of workgroups)
+int zeig_slots(int n);
 
 // Raise a kernel's dynamic-LDS limit to `bytes` when it exceeds the 64 KiB
 // default (`current` caches the value already set for that kernel).

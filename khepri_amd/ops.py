@@ -35,6 +35,21 @@ def _require(t, dtype, name):
         raise TypeError(f"{name}: expected a contiguous {dtype} tensor on a HIP device")
 
 
+_SLOTS = {}
+
+
+def eig_slots(n):
+    """Matrices the eigensolver runs at once on the current device (cached per
+    order and device)."""
+    key = (n, torch.cuda.current_device())
+    if key not in _SLOTS:
+        rc = _lib.load().khx_eig_slots(n)
+        if rc < 0:
+            check(rc, "khx_eig_slots")
+        _SLOTS[key] = rc
+    return _SLOTS[key]
+
+
 def raise_on_info(info, what):
     """Map per-item status words to numpy-style exceptions (one host sync)."""
     if info is None:

@@ -250,3 +250,11 @@ def test_blocked_lu_paths(gpu, n, m, batch):
     lu, piv, info = ops.lu_factor(dev(A, gpu))
     X2 = ops.lu_solve(lu, piv, dev(B, gpu)).cpu().numpy()
     np.testing.assert_allclose(X2, ref, atol=1e-9 * np.abs(ref).max())
+
+
+def test_eig_slots_fill_whole_cus(gpu):
+    from khepri_amd import ops
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    for n in (50, 162, 242):
+        slots = ops.eig_slots(n)
+        assert slots >= cus and slots % cus == 0, (n, slots, cus)
