@@ -206,9 +206,11 @@ def run_autotune(step, args):
     from khepri_amd import autotune
     t0 = time.perf_counter()
     table = autotune.autotune(step)
-    changed = sum(1 for e in table.values() if e["tile"] != (1 if min(e["shape"][:2]) <= 40 else 0))
-    print(f"[bench] gemm autotune: {len(table)} classes, {changed} changed", file=sys.stderr, flush=True)
-    return {"classes": len(table), "changed": changed, "seconds": round(time.perf_counter() - t0, 2)}
+    changed = sum(1 for e in table.values() if e["tile"] != autotune.default_tile(*e["shape"][:2]))
+    mfma = sum(1 for e in table.values() if e["tile"] >= 4)
+    print(f"[bench] gemm autotune: {len(table)} classes, {changed} changed, {mfma} on MFMA", file=sys.stderr,
+          flush=True)
+    return {"classes": len(table), "changed": changed, "mfma": mfma, "seconds": round(time.perf_counter() - t0, 2)}
 
 
 def load_traffic(kernel, workload, matrices_per_launch):

@@ -265,7 +265,8 @@ int khx_poynting_vec_adjoint(const double* sR, const double* sT, int N, const do
 int khx_eig_slots(int n);
 
 /* GEMM tile autotuning (no counterpart in the reference).  Tile shapes:
- * 0 = 64x64, 1 = 32x32, 2 = 64x32, 3 = 32x64 (M x N per workgroup).
+ * 0 = 64x64, 1 = 32x32, 2 = 64x32, 3 = 32x64 (M x N per workgroup, FP64 FMA),
+ * 4..7 = the same tiles on the FP64 matrix cores (MFMA).
  * khx_gemm_record(1, NULL, 0) starts recording the distinct GEMM shapes the
  * library runs; khx_gemm_record(0, shapes, max) stops and copies up to max
  * records of 6 ints (M, N, K, opA, opB, batch) into shapes, returning the

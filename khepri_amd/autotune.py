@@ -1,9 +1,11 @@
 """Autotuner for the tile shapes of the batched complex GEMM.
 
-libkhx's zgemm kernels are compiled for four workgroup tile shapes (M x N per
-256-thread workgroup): 0 = 64x64, 1 = 32x32, 2 = 64x32, 3 = 32x64.  Without a
-tuning table the library takes 32x32 when either side is at most 40 and 64x64
-otherwise.  The tuner
+libkhx's zgemm comes in eight kernels of 256 threads: FP64 FMA register
+tiles (M x N per workgroup) 0 = 64x64, 1 = 32x32, 2 = 64x32, 3 = 32x64, and
+the same tiles on the FP64 matrix cores (v_mfma_f64_16x16x4f64) 4 = 64x64,
+5 = 32x32, 6 = 64x32, 7 = 32x64.  Without a tuning table the library takes
+MFMA 32x32 when either side is at most 96 and MFMA 32x64 otherwise
+(``default_tile``).  The tuner
 
 1. runs a workload once with shape recording on (``record``), which collects
    every distinct (M, N, K, opA, opB, batch) the library launches;
@@ -32,8 +34,14 @@ import torch
 from . import _lib
 from ._lib import KhxError, check
 
-TILES = {0: "64x64", 1: "32x32", 2: "64x32", 3: "32x64"}
+TILES = {0: "64x64", 1: "32x32", 2: "64x32", 3: "32x64",
+         4: "mfma64x64", 5: "mfma32x32", 6: "mfma64x32", 7: "mfma32x64"}
 _OPNAME = {0: "N", 1: "T", 2: "C"}
+
+
+def default_tile(M, N):
+    """The library's choice for an (M x N) product when no table entry applies."""
+    return 5 if min(M, N) <= 96 else 7
 
 
 def _bucket(batch):

@@ -117,9 +117,157 @@ static int launch_zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat
   return 0;
 }
 
-// ---- tile selection.  Four tile shapes (all 256 threads):
-//   0: 64 x 64 (4 x 4 per thread), 1: 32 x 32 (2 x 2), 2: 64 x 32 (4 x 2), 3: 32 x 64 (2 x 4).
-// The default picks 32 x 32 when a side is small; a per-shape choice installed by the
+// ------------------------------------------------------------- MFMA GEMM
+// The same product on the FP64 matrix cores: v_mfma_f64_16x16x4f64 computes a
+// 16 x 16 block D += A (16 x 4) B (4 x 16); lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] and holds D[l / 16 + 4 r][l % 16] in register r.
+// A complex product is four real ones: Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br.
+// A workgroup (4 wavefronts in a 2 x 2 grid) owns a (32 WM) x (32 WN) tile of
+// C; each wavefront owns WM x WN blocks of 16 x 16.  K advances 16 at a time
+// through LDS (k-major, one complex of padding per row against bank
+// conflicts); the next K slice is fetched into registers while the current
+// one is multiplied.  Consecutive tiles of one batch item are kept on one XCD
+// (workgroups are dealt round-robin to the 8 XCDs) so they share its L2.
+typedef double khx_d4 __attribute__((ext_vector_type(4)));
+
+template <int WM, int WN, int OA, int OB>
+__global__ __launch_bounds__(256) void zgemm_mfma_kernel(int M, int N, int K, cd alpha,
+                                                         const cd* __restrict__ A, int lda, long long sA,
+                                                         const cd* __restrict__ B, int ldb, long long sB,
+                                                         cd beta, cd* __restrict__ C, int ldc, long long sC,
+                                                         int tilesN, int tilesPerItem, int xcd_remap) {
+  constexpr int BM = 32 * WM, BN = 32 * WN, BK = 16;
+  constexpr int LA = (BM * BK) / 256, LBn = (BK * BN) / 256;  // complex loads per thread
+  __shared__ cd As[BK][BM + 1];
+  __shared__ cd Bs[BK][BN + 1];
+
+  unsigned bid = blockIdx.x;
+  if (xcd_remap) {  // gridDim.x % 8 == 0: logical tile order = XCD-major
+    const unsigned per = gridDim.x / 8;
+    bid = (bid % 8) * per + bid / 8;
+  }
+  const long long item = bid / tilesPerItem;
+  const int tile = bid % tilesPerItem;
+  const int row0 = (tile / tilesN) * BM;
+  const int col0 = (tile % tilesN) * BN;
+  const cd* a = A + item * sA;
+  const cd* b = B + item * sB;
+  cd* c = C + item * sC;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * (16 * WM), wc = (wave & 1) * (16 * WN);
+  const int li = lane & 15, lk = lane >> 4;
+
+  khx_d4 accR[WM][WN], accI[WM][WN];
+#pragma unroll
+  for (int t = 0; t < WM; ++t)
+#pragma unroll
+    for (int u = 0; u < WN; ++u) {
+      accR[t][u] = khx_d4{0.0, 0.0, 0.0, 0.0};
+      accI[t][u] = khx_d4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  cd ra[LA], rb[LBn];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      const int e = tid + 256 * q;
+      const int r = OA ? e % BM : e / BK, kk = OA ? e / BM : e % BK;
+      const int gr = row0 + r, gk = k0 + kk;
+      const long long off = OA ? (long long)gk * lda + gr : (long long)gr * lda + gk;
+      cd v = (gr < M && gk < K) ? a[off] : mkc(0.0, 0.0);
+      if (OA == 2) v.y = -v.y;
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < LBn; ++q) {
+      const int e = tid + 256 * q;
+      const int kk = OB ? e % BK : e / BN, cc = OB ? e / BK : e % BN;
+      const int gk = k0 + kk, gc = col0 + cc;
+      const long long off = OB ? (long long)gc * ldb + gk : (long long)gk * ldb + gc;
+      cd v = (gk < K && gc < N) ? b[off] : mkc(0.0, 0.0);
+      if (OB == 2) v.y = -v.y;
+      rb[q] = v;
+    }
+  };
+
+  fetch(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      const int e = tid + 256 * q;
+      const int r = OA ? e % BM : e / BK, kk = OA ? e / BM : e % BK;
+      As[kk][r] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < LBn; ++q) {
+      const int e = tid + 256 * q;
+      const int kk = OB ? e % BK : e / BN, cc = OB ? e / BK : e % BN;
+      Bs[kk][cc] = rb[q];
+    }
+    __syncthreads();
+    if (k0 + BK < K) fetch(k0 + BK);
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; ++kq) {
+      cd av[WM], bv[WN];
+#pragma unroll
+      for (int t = 0; t < WM; ++t) av[t] = As[kq * 4 + lk][wr + t * 16 + li];
+#pragma unroll
+      for (int u = 0; u < WN; ++u) bv[u] = Bs[kq * 4 + lk][wc + u * 16 + li];
+#pragma unroll
+      for (int t = 0; t < WM; ++t) {
+        const double nai = -av[t].y;
+#pragma unroll
+        for (int u = 0; u < WN; ++u) {
+          accR[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].x, bv[u].x, accR[t][u], 0, 0, 0);
+          accR[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bv[u].y, accR[t][u], 0, 0, 0);
+          accI[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].x, bv[u].y, accI[t][u], 0, 0, 0);
+          accI[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].y, bv[u].x, accI[t][u], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+#pragma unroll
+  for (int t = 0; t < WM; ++t)
+#pragma unroll
+    for (int u = 0; u < WN; ++u) {
+      const int gc = col0 + wc + u * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = row0 + wr + t * 16 + lk + 4 * r;
+        if (gr >= M || gc >= N) continue;
+        cd v = alpha * mkc(accR[t][u][r], accI[t][u][r]);
+        cd* dst = c + (long long)gr * ldc + gc;
+        if (use_beta) v = fmac(v, beta, *dst);
+        *dst = v;
+      }
+    }
+}
+
+template <int WM, int WN, int OA, int OB>
+static int launch_zgemm_mfma(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C,
+                             int batch) {
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
+  const long long blocks = (long long)tilesM * tilesN * batch;
+  KHX_REQUIRE(blocks < (1ll << 31), "zgemm: grid too large");
+  ProfScope ps("zgemm_kernel", s);
+  zgemm_mfma_kernel<WM, WN, OA, OB><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+      M, N, K, alpha, A.p, A.ld, A.stride, B.p, B.ld, B.stride, beta, C.p, C.ld, C.stride, tilesN,
+      tilesM * tilesN, (blocks % 8 == 0) ? 1 : 0);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+// ---- tile selection.  Eight kernels (all 256 threads): FMA tiles
+//   0: 64 x 64 (4 x 4 per thread), 1: 32 x 32 (2 x 2), 2: 64 x 32 (4 x 2), 3: 32 x 64 (2 x 4);
+// MFMA tiles 4: 64 x 64, 5: 32 x 32, 6: 64 x 32, 7: 32 x 64.
+// The default is MFMA 32 x 32 when a side is at most 96 and MFMA 32 x 64 otherwise
+// (the fastest of the eight on the batched layer shapes); a per-shape choice installed by the
 // autotuner (khepri_amd/autotune.py, through khx_gemm_tune_set) takes precedence, and
 // khx_gemm_force overrides both while candidates are timed.  With recording on, every
 // distinct (M, N, K, opA, opB, batch class) passing through is kept for the autotuner.
@@ -148,7 +296,7 @@ int gemm_tile_choice(int M, int N, int K, int oa, int ob, int batch) {
   }
   const auto it = g_tune.find(key);
   if (it != g_tune.end()) return it->second;
-  return (M <= 40 || N <= 40) ? 1 : 0;
+  return (M <= 96 || N <= 96) ? 5 : 7;
 }
 
 template <int OA, int OB>
@@ -158,12 +306,16 @@ static int zgemm_sized(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat 
     case 1: return launch_zgemm<32, 32, 16, 2, 2, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
     case 2: return launch_zgemm<64, 32, 16, 4, 2, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
     case 3: return launch_zgemm<32, 64, 16, 2, 4, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 4: return launch_zgemm_mfma<2, 2, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 5: return launch_zgemm_mfma<1, 1, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 6: return launch_zgemm_mfma<2, 1, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
+    case 7: return launch_zgemm_mfma<1, 2, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
     default: return launch_zgemm<64, 64, 16, 4, 4, OA, OB>(s, M, N, K, alpha, A, B, beta, C, batch);
   }
 }
 
 int gemm_tune_set(int M, int N, int K, int oa, int ob, int batch, int cfg) {
-  KHX_REQUIRE(cfg >= -1 && cfg <= 3, "gemm tile configuration must be 0..3 (or -1 to clear)");
+  KHX_REQUIRE(cfg >= -1 && cfg <= 7, "gemm tile configuration must be 0..7 (or -1 to clear)");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   const unsigned long long key = tune_key(M, N, K, oa, ob, batch);
   if (cfg < 0)
@@ -174,7 +326,7 @@ int gemm_tune_set(int M, int N, int K, int oa, int ob, int batch, int cfg) {
 }
 
 int gemm_force(int cfg) {
-  KHX_REQUIRE(cfg >= -1 && cfg <= 3, "gemm tile configuration must be 0..3 (or -1)");
+  KHX_REQUIRE(cfg >= -1 && cfg <= 7, "gemm tile configuration must be 0..7 (or -1)");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_force = cfg;
   return 0;

@@ -40,7 +40,7 @@ __host__ __device__ __forceinline__ int eig_wv_elems(int n) { return n <= KHX_EI
 __host__ __device__ __forceinline__ int eig_part_elems(int n, int waves) {
   if (n <= KHX_EIG_LDS_N) return waves * 2 * n;
   const int hess = HB_NB * HB_NB + waves * HB_NB + 2 * HB_NB + 2 * 8 * HB_NB;  // T, sums, staged rows
-  return hess > EIG_QB * EIG_QB ? hess : EIG_QB * EIG_QB;
+  return hess > EIG_QB * (EIG_QB + 1) ? hess : EIG_QB * (EIG_QB + 1);  // chase block with padded rows
 }
 
 template <int EIG_THREADS>
@@ -655,6 +655,166 @@ __device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift,
   }
 }
 
+// The same sweep with the bulge neighbourhood in registers.  In a block only
+// a few entries feed the next rotation: the 2 x 2 core at rows/columns
+// (k, k+1), the subdiagonal entry below it and entry (k, k+1), which depends
+// on every row rotation applied to column k+1 so far.  One wavefront runs the
+// chase; every lane computes the rotations and the core identically (uniform
+// values, no divergence), and lane j owns block column j: it carries the
+// entry of its column on the current row as a register and applies each row
+// rotation to it as soon as it exists, writing the finished entry of the row
+// above.  Entry (k+1, k+2) reaches the core by a cross-lane read from lane
+// k+2.  Row and column rotations commute (G H G^H = (G H) G^H), so the
+// column rotations of the rows above the core are applied afterwards, one
+// lane per row sliding through the block's rotations.  The rest (rows above
+// the block, columns right of it, Zt and beyond the window) is as in
+// qr_sweep_blocked.  Block rows are padded by one element so the per-row
+// slides hit distinct LDS banks.
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int EIG_THREADS>
+__device__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, double* crot, cd* srot,
+                             unsigned long long* acc) {
+  const int tid = threadIdx.x;
+  int kstart = l, b0 = l;
+  while (true) {
+    const int b1 = min(b0 + EIG_QB - 1, ihi);
+    const int nb = b1 - b0 + 1;
+    const int ld = nb + 1;
+    const int kend = (b1 == ihi) ? ihi - 1 : b1 - 2;  // last rotation whose bulge stays in the block
+    for (int e = tid; e < nb * nb; e += EIG_THREADS)
+      hb[(e / nb) * ld + e % nb] = H[(long long)(b0 + e / nb) * n + b0 + e % nb];
+    __syncthreads();
+    unsigned long long t_c0 = acc ? wall_clock64() : 0;
+    if (tid < 64) {
+      const int j = tid;  // block column (row rotations), then block row (column rotations)
+      int q = kstart - b0;  // block index of the current rotation: 0 in the first block, 1 after
+      cd x, y;
+      if (kstart == l) {
+        x = hb[0] - shift;
+        y = hb[ld];
+      } else {
+        x = hb[q * ld + q - 1];
+        y = hb[(q + 1) * ld + q - 1];
+      }
+      cd a00 = hb[q * ld + q], a01 = hb[q * ld + q + 1];
+      cd a10 = hb[(q + 1) * ld + q], a11 = hb[(q + 1) * ld + q + 1];
+      const bool mine0 = (j >= q + 2 && j < nb);
+      cd v = mine0 ? hb[q * ld + j] : mkc(0.0, 0.0);
+      cd yj = mine0 ? hb[(q + 1) * ld + j] : mkc(0.0, 0.0);
+      cd wq = (q + 2 < nb) ? hb[(q + 2) * ld + q + 1] : mkc(0.0, 0.0);
+      cd dq = (q + 2 < nb) ? hb[(q + 2) * ld + q + 2] : mkc(0.0, 0.0);
+      for (int k = kstart; k <= kend; ++k, ++q) {
+        double c;
+        cd s, r;
+        givens(x, y, c, s, r);
+        // prefetch the next step's untouched inputs
+        const bool mine = (j >= q + 2 && j < nb);
+        const bool more = (k < kend);
+        cd yj_n = mkc(0.0, 0.0), wq_n = mkc(0.0, 0.0), dq_n = mkc(0.0, 0.0);
+        if (more) {
+          if (j >= q + 3 && j < nb) yj_n = hb[(q + 2) * ld + j];
+          if (q + 3 < nb) {
+            wq_n = hb[(q + 3) * ld + q + 2];
+            dq_n = hb[(q + 3) * ld + q + 3];
+          }
+        }
+        if (mine) {  // row rotation of the columns right of the core
+          hb[q * ld + j] = fmac(c * v, s, yj);
+          v = fmsc(c * yj, conjc(s), v);
+        }
+        rot_rows(a00, a10, c, s);
+        rot_rows(a01, a11, c, s);
+        rot_cols(a00, a01, c, s);
+        rot_cols(a10, a11, c, s);
+        // row k+2: (k+2, k) was zero, (k+2, k+1) is untouched so far
+        const cd bulge = conjc(s) * wq;
+        const cd w2 = c * wq;
+        if (j == 0) {
+          crot[k] = c;
+          srot[k] = s;
+          if (k > l) {
+            hb[q * ld + q - 1] = r;
+            hb[(q + 1) * ld + q - 1] = mkc(0.0, 0.0);
+          }
+          hb[q * ld + q] = a00;
+          hb[q * ld + q + 1] = a01;
+        }
+        if (!more) {
+          if (j == 0) {
+            hb[(q + 1) * ld + q] = a10;
+            hb[(q + 1) * ld + q + 1] = a11;
+            if (q + 2 < nb) {
+              hb[(q + 2) * ld + q] = bulge;
+              hb[(q + 2) * ld + q + 1] = w2;
+            }
+          }
+          if (mine) hb[(q + 1) * ld + j] = v;
+          break;
+        }
+        x = a10;
+        y = bulge;
+        a00 = a11;
+        a01 = mkc(readlane_d(v.x, q + 2), readlane_d(v.y, q + 2));
+        a10 = w2;
+        a11 = dq;
+        yj = yj_n;
+        wq = wq_n;
+        dq = dq_n;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // column rotations of the block rows above the core: row i takes the
+      // rotations k >= b0 + i + 1 (columns k - b0, k - b0 + 1)
+      const int k0 = max(kstart, b0 + j + 1);
+      if (j < nb && k0 <= kend) {
+        cd* row = hb + j * ld - b0;  // row[col] = block entry (j, col - b0)
+        cd xv = row[k0];
+        int k = k0;
+        for (; k + 4 <= kend + 1; k += 4) {
+          cd yv[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) yv[m] = row[k + 1 + m];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            rot_cols(xv, yv[m], crot[k + m], srot[k + m]);
+            row[k + m] = xv;
+            xv = yv[m];
+          }
+        }
+        for (; k <= kend; ++k) {
+          cd yv = row[k + 1];
+          rot_cols(xv, yv, crot[k], srot[k]);
+          row[k] = xv;
+          xv = yv;
+        }
+        row[kend + 1] = xv;
+      }
+    }
+    __syncthreads();
+    if (acc) acc[1] += wall_clock64() - t_c0;
+    for (int e = tid; e < nb * nb; e += EIG_THREADS)
+      H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[(e / nb) * ld + e % nb];
+    const int nabove = b0 - l, nright = ihi - b1;
+    const int tot = nabove + nright;
+    for (int e = tid; e < tot; e += EIG_THREADS) {
+      if (e < nabove)
+        slide<false>(H + (long long)(l + e) * n, 1, kstart, kend + 1, crot, srot);
+      else
+        slide<true>(H + b1 + 1 + (e - nabove), n, kstart, kend + 1, crot, srot);
+    }
+    __syncthreads();
+    if (kend == ihi - 1) break;
+    kstart = kend + 1;
+    b0 = kstart - 1;
+  }
+}
+
 // Two shifts per sweep: two bulges, EIG_SEP positions apart (the leading one
 // with shift s0), chased through the same LDS blocks by one wavefront.  Each
 // step computes both rotations, then applies all row rotations (phase R) and
@@ -977,6 +1137,8 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       if (two)
         qr_sweep2_blocked<EIG_THREADS>(H, n, l, ihi, shift, shift2, part, crot, srot, crot2, srot2,
                                        tp ? acc_t : nullptr);
+      else if (flags & 4)
+        qr_sweep_reg<EIG_THREADS>(H, n, l, ihi, shift, part, crot, srot, tp ? acc_t : nullptr);
       else
         qr_sweep_blocked<EIG_THREADS>(H, Zt, n, l, ihi, shift, part, crot, srot, &s_xy[0][0], (flags & 1) != 0,
                                       tp ? acc_t : nullptr);
@@ -1210,7 +1372,10 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   // eigenproblems it needs about 1.5x more shifts than single-shift sweeps, which outweighs the
   // shared block overheads)
   const char* ts_env = getenv("KHX_EIG_TWO_SHIFT");
-  const int flags = ((cw_env && atoi(cw_env) == 0) ? 0 : 1) | ((ts_env && atoi(ts_env) != 0) ? 2 : 0);
+  // bit 2: register chase (qr_sweep_reg; default on, KHX_EIG_REG_CHASE=0 selects the LDS chase)
+  const char* rc_env = getenv("KHX_EIG_REG_CHASE");
+  const int flags = ((cw_env && atoi(cw_env) == 0) ? 0 : 1) | ((ts_env && atoi(ts_env) != 0) ? 2 : 0) |
+                    ((rc_env && atoi(rc_env) == 0) ? 0 : 4);
   const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
   KHX_REQUIRE(lds <= 150 * 1024, "zeig: n too large for the LDS work vectors");
   static size_t lds_limit[5] = {0, 0, 0, 0, 0};

@@ -307,7 +307,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
 // the bulk of the flops runs in 64 x 64 GEMM tiles spread over the whole GPU
 // instead of inside the one workgroup that owns the matrix.
 // GP: the panel stays in global memory (systems too large for an LDS panel)
-template <bool GP>
+template <int NB, bool GP>
 __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
                                                                cd* B, int ldb, long long sB, int* ipiv, int* info,
                                                                int keepL) {
@@ -315,20 +315,20 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
   __shared__ double rv[LB_THREADS / 64];
   __shared__ int ri[LB_THREADS / 64];
   __shared__ cd rc[LB_THREADS / 64];
-  __shared__ int perm[LB_NB];
+  __shared__ int perm[NB];
   const long long item = blockIdx.x;
   cd* a = A + item * sA;
   cd* bm = m > 0 ? B + item * sB : nullptr;
   const int tid = threadIdx.x;
   int first_zero = 0;
-  const int nb = min(LB_NB, n - j0);
+  const int nb = min(NB, n - j0);
   const int pr = n - j0;
-  // panel element (r, c): LDS copy (row stride LB_NB) or the matrix itself
+  // panel element (r, c): LDS copy (row stride NB) or the matrix itself
   cd* pan = GP ? a + (long long)j0 * lda + j0 : smem;
-  const long long pld = GP ? lda : LB_NB;
+  const long long pld = GP ? lda : NB;
   if (!GP) {
-    for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
-      const int r = e / LB_NB, c = e % LB_NB;
+    for (int e = tid; e < pr * NB; e += LB_THREADS) {
+      const int r = e / NB, c = e % NB;
       pan[e] = (c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
     }
   }
@@ -376,8 +376,8 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
     __syncthreads();
   }
   if (!GP) {
-    for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
-      const int r = e / LB_NB, c = e % LB_NB;
+    for (int e = tid; e < pr * NB; e += LB_THREADS) {
+      const int r = e / NB, c = e % NB;
       if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[e];
     }
   }
@@ -415,73 +415,76 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
       }
     }
     if (e >= nleft) {
-      cd x[LB_NB];
+      cd x[NB];
 #pragma unroll
-      for (int i = 0; i < LB_NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ld] : mkc(0.0, 0.0);
+      for (int i = 0; i < NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ld] : mkc(0.0, 0.0);
 #pragma unroll
-      for (int i = 1; i < LB_NB; ++i)
+      for (int i = 1; i < NB; ++i)
         if (i < nb) {  // the last panel may be narrower: rows past it are outside the matrix
 #pragma unroll
           for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], pan[i * pld + k], x[k]);
         }
 #pragma unroll
-      for (int i = 0; i < LB_NB; ++i)
+      for (int i = 0; i < NB; ++i)
         if (i < nb) col[(long long)(j0 + i) * ld] = x[i];
     }
   }
 }
 
 // X block = U11^-1 Y block for the rows j0 .. j0+nb of B (one thread per column)
+template <int NB>
 __global__ __launch_bounds__(LB_THREADS) void zlu_upper_block_kernel(int n, int m, int j0, const cd* A, int lda,
                                                                      long long sA, cd* B, int ldb, long long sB) {
-  __shared__ cd u[LB_NB * LB_NB];
+  __shared__ cd u[NB * NB];
   const long long item = blockIdx.y;
   const cd* a = A + item * sA;
   cd* bm = B + item * sB;
-  const int nb = min(LB_NB, n - j0);
-  for (int e = threadIdx.x; e < LB_NB * LB_NB; e += blockDim.x) {
-    const int r = e / LB_NB, c = e % LB_NB;
+  const int nb = min(NB, n - j0);
+  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+    const int r = e / NB, c = e % NB;
     u[e] = (r < nb && c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(r == c ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= m) return;
   cd* col = bm + c;
-  cd x[LB_NB];
+  cd x[NB];
 #pragma unroll
-  for (int i = 0; i < LB_NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ldb] : mkc(0.0, 0.0);
+  for (int i = 0; i < NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ldb] : mkc(0.0, 0.0);
 #pragma unroll
-  for (int i = LB_NB - 1; i >= 0; --i) {
+  for (int i = NB - 1; i >= 0; --i) {
 #pragma unroll
-    for (int k = i + 1; k < LB_NB; ++k) x[i] = fmsc(x[i], u[i * LB_NB + k], x[k]);
-    x[i] = divc(x[i], u[i * LB_NB + i]);
+    for (int k = i + 1; k < NB; ++k) x[i] = fmsc(x[i], u[i * NB + k], x[k]);
+    x[i] = divc(x[i], u[i * NB + i]);
   }
 #pragma unroll
-  for (int i = 0; i < LB_NB; ++i)
+  for (int i = 0; i < NB; ++i)
     if (i < nb) col[(long long)(j0 + i) * ldb] = x[i];
 }
 
-static size_t lu_panel_lds(int n) { return (size_t)n * LB_NB * sizeof(cd); }
+template <int NB>
+static size_t lu_panel_lds(int n) { return (size_t)n * NB * sizeof(cd); }
 
 bool lu_multi_fits(int n) { return n >= 1; }  // larger systems keep the panel in global memory
 
-int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL) {
+template <int NB>
+static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL) {
   if (batch <= 0 || n <= 0) return 0;
   KHX_REQUIRE(A.stride != 0 && (m == 0 || B.stride != 0) || batch == 1, "zlu_multi: operands cannot be broadcast");
   KHX_REQUIRE(batch <= 65535, "zlu_multi: at most 65535 systems per call");
-  const bool gp = lu_panel_lds(n) > 150 * 1024;
-  const size_t lds = gp ? 0 : lu_panel_lds(n);
+  const bool gp = lu_panel_lds<NB>(n) > 150 * 1024;
+  const size_t lds = gp ? 0 : lu_panel_lds<NB>(n);
   static size_t lds_limit = 0;
-  if (!gp) KHX_TRY(ensure_dynamic_lds((const void*)zlu_panel_kernel<false>, lds, lds_limit));
+  if (!gp) KHX_TRY(ensure_dynamic_lds((const void*)zlu_panel_kernel<NB, false>, lds, lds_limit));
   const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
   ProfScope ps(m > 0 ? "zgesv_kernel" : "zgetrf_kernel", s);
-  for (int j0 = 0; j0 < n; j0 += LB_NB) {
-    const int nb = min(LB_NB, n - j0);
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int nb = min(NB, n - j0);
     if (gp)
-      zlu_panel_kernel<true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
+      zlu_panel_kernel<NB, true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
                                                                       B.stride, ipiv, info, keepL ? 1 : 0);
     else
-      zlu_panel_kernel<false><<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
+      zlu_panel_kernel<NB, false><<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
                                                                          B.stride, ipiv, info, keepL ? 1 : 0);
     KHX_LAUNCH_CHECK();
     const int r1 = j0 + nb, rest = n - r1;
@@ -498,12 +501,12 @@ int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, i
     }
   }
   if (m <= 0) return 0;
-  // back substitution U X = Y, LB_NB rows at a time from the bottom
-  const int nblk = (n + LB_NB - 1) / LB_NB;
+  // back substitution U X = Y, NB rows at a time from the bottom
+  const int nblk = (n + NB - 1) / NB;
   for (int blk = nblk - 1; blk >= 0; --blk) {
-    const int j0 = blk * LB_NB;
-    const int nb = min(LB_NB, n - j0);
-    zlu_upper_block_kernel<<<dim3((m + LB_THREADS - 1) / LB_THREADS, batch), dim3(LB_THREADS), 0, s>>>(
+    const int j0 = blk * NB;
+    const int nb = min(NB, n - j0);
+    zlu_upper_block_kernel<NB><<<dim3((m + LB_THREADS - 1) / LB_THREADS, batch), dim3(LB_THREADS), 0, s>>>(
         n, m, j0, A.p, A.ld, A.stride, B.p, B.ld, B.stride);
     KHX_LAUNCH_CHECK();
     if (j0 > 0) {
@@ -513,6 +516,14 @@ int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, i
     }
   }
   return 0;
+}
+
+// panel width: KHX_LU_NB=16|32 forces it
+int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL) {
+  const char* env = getenv("KHX_LU_NB");
+  const int nb = env ? atoi(env) : 16;
+  if (nb == 32) return zlu_multi_nb<32>(s, n, m, A, B, batch, ipiv, info, keepL);
+  return zlu_multi_nb<16>(s, n, m, A, B, batch, ipiv, info, keepL);
 }
 
 // KHX_LU_PATH=0/1/2 forces a path; by default the multi-launch variant takes

@@ -36,8 +36,8 @@ def test_host_only_entry_points():
     assert lib.khx_halfspace(3, None, None, 25, 1, None, None, None, None, None, None) == -1
     assert b"side" in lib.khx_last_error()
     # GEMM tile table: bad tile ids and shapes are rejected, recording is host-side
-    assert lib.khx_gemm_force(7) == -1
-    assert lib.khx_gemm_tune_set(10, 10, 10, 0, 0, 1, 4) == -1
+    assert lib.khx_gemm_force(8) == -1
+    assert lib.khx_gemm_tune_set(10, 10, 10, 0, 0, 1, 8) == -1
     assert lib.khx_gemm_tune_set(10, 10, 10, 3, 0, 1, 0) == -1
     assert lib.khx_gemm_tune_set(10, 10, 10, 0, 0, 1, 2) == 0
     assert lib.khx_gemm_tune_set(10, 10, 10, 0, 0, 1, -1) == 0

@@ -13,9 +13,10 @@ def crand(rng, *shape):
     return rng.normal(size=shape) + 1j * rng.normal(size=shape)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("M,N,K,opa,opb", [(162, 162, 162, "N", "N"), (37, 101, 29, "T", "N"),
-                                           (70, 33, 65, "N", "C"), (5, 130, 81, "C", "T")])
+                                           (70, 33, 65, "N", "C"), (5, 130, 81, "C", "T"),
+                                           (64, 64, 32, "N", "N"), (1, 1, 1, "C", "N"), (100, 17, 3, "T", "T")])
 def test_every_tile_matches_numpy(gpu, tile, M, N, K, opa, opb):
     from khepri_amd import _lib, ops
     lib = _lib.load()

@@ -258,3 +258,29 @@ def test_eig_slots_fill_whole_cus(gpu):
     for n in (50, 162, 242):
         slots = ops.eig_slots(n)
         assert slots >= cus and slots % cus == 0, (n, slots, cus)
+
+
+@pytest.mark.parametrize("reg_chase", ["0", "1"])
+def test_eig_chase_variants(gpu, monkeypatch, reg_chase):
+    """Both QR chases (LDS block chase, register chase) on layer matrices with
+    windows that span several chase blocks and on random matrices."""
+    from khepri_amd import ops
+    monkeypatch.setenv("KHX_EIG_REG_CHASE", reg_chase)
+    pix = W.disc_pixmap()
+    C = rcwa.convolution_matrix(pix, (9, 9))
+    g = rcwa.g_vectors((9, 9), np.eye(2))
+    mats = []
+    for wl, kp in [(2.0, (0.1, 0.3)), (1.7, (0.0, 0.0)), (2.2, (0.4, -0.2))]:
+        kx, ky, _ = rcwa.k_vectors(g, kp, wl)
+        P, Q = rcwa.structured_pq(kx, ky, C)
+        mats.append(P @ Q)
+    rng = np.random.default_rng(9)
+    mats += [crand(rng, 162, 162), crand(rng, 100, 100)[:, :]]
+    for M in mats:
+        lam, Wm, info = ops.eig(dev(M[None], gpu))
+        assert int(info.cpu()[0]) == 0
+        lam, Wm = lam.cpu().numpy()[0], Wm.cpu().numpy()[0]
+        assert np.abs(M @ Wm - Wm * lam).max() / np.abs(M).max() < 1e-12
+        ref = np.linalg.eigvals(M)
+        d = np.abs(lam[:, None] - ref[None, :])
+        assert d.min(axis=1).max() < 1e-10 * np.abs(ref).max()
