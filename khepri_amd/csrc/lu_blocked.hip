@@ -306,7 +306,10 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
 // trailing update [A22 | B2] -= L21 [U12 | Y1] is a batched zgemm launch, so
 // the bulk of the flops runs in 64 x 64 GEMM tiles spread over the whole GPU
 // instead of inside the one workgroup that owns the matrix.
-// GP: the panel stays in global memory (systems too large for an LDS panel)
+// GP: the panel stays in global memory (systems too large for an LDS panel).
+// keepL bit 0: apply the interchanges to the factored columns left of the
+// panel too (zgetrf layout); bit 1: only interchange B's rows (its triangular
+// solves run after the factorization).
 template <int NB, bool GP>
 __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
                                                                cd* B, int ldb, long long sB, int* ipiv, int* info,
@@ -389,7 +392,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
   }
   // interchanges on the other columns and the triangular solve with L11,
   // one thread per column
-  const int nleft = keepL ? j0 : 0;
+  const int nleft = (keepL & 1) ? j0 : 0;
   const int nright = n - j0 - nb;
   for (int e = tid; e < nleft + nright + m; e += LB_THREADS) {
     cd* col;
@@ -414,7 +417,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
         *y = t;
       }
     }
-    if (e >= nleft) {
+    if (e >= nleft && (e < nleft + nright || !(keepL & 2))) {
       cd x[NB];
 #pragma unroll
       for (int i = 0; i < NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ld] : mkc(0.0, 0.0);
@@ -462,6 +465,91 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_upper_block_kernel(int n, int 
     if (i < nb) col[(long long)(j0 + i) * ldb] = x[i];
 }
 
+// X block = L11^-1 Y block (unit lower) for the rows j0 .. j0+nb of B (one thread per column)
+template <int NB>
+__global__ __launch_bounds__(LB_THREADS) void zlu_lower_block_kernel(int n, int m, int j0, const cd* A, int lda,
+                                                                     long long sA, cd* B, int ldb, long long sB) {
+  __shared__ cd lw[NB * NB];
+  const long long item = blockIdx.y;
+  const cd* a = A + item * sA;
+  cd* bm = B + item * sB;
+  const int nb = min(NB, n - j0);
+  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+    const int r = e / NB, c = e % NB;
+    lw[e] = (r < nb && c < r) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
+  }
+  __syncthreads();
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  cd* col = bm + c;
+  cd x[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ldb] : mkc(0.0, 0.0);
+#pragma unroll
+  for (int i = 1; i < NB; ++i)
+#pragma unroll
+    for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], lw[i * NB + k], x[k]);
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    if (i < nb) col[(long long)(j0 + i) * ldb] = x[i];
+}
+
+// Triangular solves with the factors of the multi-launch LU, split in halves
+// so that most of the work is one GEMM with a deep K (rows r0 .. r1 of B):
+//   L: X1 = L11^-1 B1;  B2 -= L21 X1;  X2 = L22^-1 B2
+//   U: X2 = U22^-1 B2;  B1 -= U12 X2;  X1 = U11^-1 B1
+// Ranges of at most 2 NB rows use the NB-row block kernels directly.
+template <int NB>
+static int trsm_lower_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int r0, int r1) {
+  const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
+  if (r1 - r0 <= 2 * NB) {
+    for (int j0 = r0; j0 < r1; j0 += NB) {
+      const int nb = min(NB, r1 - j0);
+      zlu_lower_block_kernel<NB><<<dim3((m + LB_THREADS - 1) / LB_THREADS, batch), dim3(LB_THREADS), 0, s>>>(
+          min(n, j0 + nb), m, j0, A.p, A.ld, A.stride, B.p, B.ld, B.stride);
+      KHX_LAUNCH_CHECK();
+      if (j0 + nb < r1)
+        KHX_TRY(zgemm(s, r1 - j0 - nb, m, nb, mone, Mat{A.p + (long long)(j0 + nb) * A.ld + j0, A.ld, A.stride},
+                      Mat{B.p + (long long)j0 * B.ld, B.ld, B.stride}, one,
+                      Mat{B.p + (long long)(j0 + nb) * B.ld, B.ld, B.stride}, batch));
+    }
+    return 0;
+  }
+  const int mid = r0 + ((r1 - r0) / 2 + NB - 1) / NB * NB;
+  KHX_TRY(trsm_lower_rec<NB>(s, n, m, A, B, batch, r0, mid));
+  KHX_TRY(zgemm(s, r1 - mid, m, mid - r0, mone, Mat{A.p + (long long)mid * A.ld + r0, A.ld, A.stride},
+                Mat{B.p + (long long)r0 * B.ld, B.ld, B.stride}, one, Mat{B.p + (long long)mid * B.ld, B.ld, B.stride},
+                batch));
+  return trsm_lower_rec<NB>(s, n, m, A, B, batch, mid, r1);
+}
+
+template <int NB>
+static int trsm_upper_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int r0, int r1) {
+  const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
+  if (r1 - r0 <= 2 * NB) {
+    // blocks of NB rows aligned to r0, from the bottom
+    const int nblk = (r1 - r0 + NB - 1) / NB;
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+      const int j0 = r0 + blk * NB;
+      const int nb = min(NB, r1 - j0);
+      zlu_upper_block_kernel<NB><<<dim3((m + LB_THREADS - 1) / LB_THREADS, batch), dim3(LB_THREADS), 0, s>>>(
+          min(n, j0 + nb), m, j0, A.p, A.ld, A.stride, B.p, B.ld, B.stride);
+      KHX_LAUNCH_CHECK();
+      if (j0 > r0)
+        KHX_TRY(zgemm(s, j0 - r0, m, nb, mone, Mat{A.p + (long long)r0 * A.ld + j0, A.ld, A.stride},
+                      Mat{B.p + (long long)j0 * B.ld, B.ld, B.stride}, one,
+                      Mat{B.p + (long long)r0 * B.ld, B.ld, B.stride}, batch));
+    }
+    return 0;
+  }
+  const int mid = r0 + ((r1 - r0) / 2 + NB - 1) / NB * NB;
+  KHX_TRY(trsm_upper_rec<NB>(s, n, m, A, B, batch, mid, r1));
+  KHX_TRY(zgemm(s, mid - r0, m, r1 - mid, mone, Mat{A.p + (long long)r0 * A.ld + mid, A.ld, A.stride},
+                Mat{B.p + (long long)mid * B.ld, B.ld, B.stride}, one, Mat{B.p + (long long)r0 * B.ld, B.ld, B.stride},
+                batch));
+  return trsm_upper_rec<NB>(s, n, m, A, B, batch, r0, mid);
+}
+
 template <int NB>
 static size_t lu_panel_lds(int n) { return (size_t)n * NB * sizeof(cd); }
 
@@ -478,14 +566,22 @@ static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, in
   if (!gp) KHX_TRY(ensure_dynamic_lds((const void*)zlu_panel_kernel<NB, false>, lds, lds_limit));
   const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
   ProfScope ps(m > 0 ? "zgesv_kernel" : "zgetrf_kernel", s);
+  // right-hand sides: with 64 or more of them the panels only interchange B's
+  // rows and the triangular solves run afterwards, split in halves
+  // (trsm_*_rec), which needs L in final row order (interchanges applied to
+  // the factored columns too); fewer right-hand sides (or KHX_LU_RECSOLVE=0)
+  // are solved panel by panel
+  const char* rs_env = getenv("KHX_LU_RECSOLVE");
+  const bool rec = m >= 64 && !(rs_env && atoi(rs_env) == 0);
+  const int mode = (keepL || rec ? 1 : 0) | (rec ? 2 : 0);
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int nb = min(NB, n - j0);
     if (gp)
       zlu_panel_kernel<NB, true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
-                                                                      B.stride, ipiv, info, keepL ? 1 : 0);
+                                                                      B.stride, ipiv, info, mode);
     else
       zlu_panel_kernel<NB, false><<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
-                                                                         B.stride, ipiv, info, keepL ? 1 : 0);
+                                                                         B.stride, ipiv, info, mode);
     KHX_LAUNCH_CHECK();
     const int r1 = j0 + nb, rest = n - r1;
     if (rest > 0) {
@@ -493,7 +589,7 @@ static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, in
       const Mat U12{A.p + (long long)j0 * A.ld + r1, A.ld, A.stride};
       const Mat A22{A.p + (long long)r1 * A.ld + r1, A.ld, A.stride};
       KHX_TRY(zgemm(s, rest, rest, nb, mone, L21, U12, one, A22, batch));
-      if (m > 0) {
+      if (m > 0 && !rec) {
         const Mat Y1{B.p + (long long)j0 * B.ld, B.ld, B.stride};
         const Mat B2{B.p + (long long)r1 * B.ld, B.ld, B.stride};
         KHX_TRY(zgemm(s, rest, m, nb, mone, L21, Y1, one, B2, batch));
@@ -501,6 +597,10 @@ static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, in
     }
   }
   if (m <= 0) return 0;
+  if (rec) {
+    KHX_TRY(trsm_lower_rec<NB>(s, n, m, A, B, batch, 0, n));
+    return trsm_upper_rec<NB>(s, n, m, A, B, batch, 0, n);
+  }
   // back substitution U X = Y, NB rows at a time from the bottom
   const int nblk = (n + NB - 1) / NB;
   for (int blk = nblk - 1; blk >= 0; --blk) {

@@ -31,7 +31,8 @@ def test_zgemm(gpu, M, N, K, batch):
     np.testing.assert_allclose(out.cpu().numpy(), A @ B[0], atol=1e-12 * np.abs(A @ B[0]).max())
 
 
-@pytest.mark.parametrize("n,m,batch", [(1, 1, 3), (50, 50, 4), (162, 324, 2), (97, 3, 3)])
+@pytest.mark.parametrize("n,m,batch", [(1, 1, 3), (50, 50, 4), (162, 324, 2), (97, 3, 3), (130, 64, 3),
+                                       (242, 100, 2), (339, 70, 1), (1250, 300, 1)])
 def test_zgesv(gpu, n, m, batch):
     from khepri_amd import ops
     rng = np.random.default_rng(n * 7 + m)
@@ -50,6 +51,14 @@ def test_zgesv_singular_reports_info(gpu):
     _, info = ops.solve(dev(A, gpu), dev(np.ones((2, 4, 1), dtype=complex), gpu))
     info = info.cpu().numpy()
     assert info[0] == 0 and info[1] > 0
+    # the same through the multi-launch LU with right-hand sides solved after the factorization
+    A = np.zeros((2, 100, 100), dtype=complex)
+    A[0] = np.eye(100)
+    A[1] = np.eye(100)
+    A[1, 40, 40] = 0.0
+    _, info = ops.solve(dev(A, gpu), dev(np.ones((2, 100, 64), dtype=complex), gpu))
+    info = info.cpu().numpy()
+    assert info[0] == 0 and info[1] == 41
 
 
 @pytest.mark.parametrize("n,batch", [(1, 2), (2, 3), (5, 4), (50, 6), (162, 2)])
@@ -183,7 +192,7 @@ def test_poynting_matches_oracle(gpu):
 
 
 @pytest.mark.parametrize("n,m,batch", [(1, 1, 2), (50, 1, 3), (162, 7, 2), (162, 1, 3), (242, 3, 2), (97, 4, 2),
-                                       (1250, 1, 1)])
+                                       (1250, 1, 1), (162, 162, 2), (131, 80, 2)])
 def test_lu_factor_solve(gpu, n, m, batch):
     from khepri_amd import ops
     rng = np.random.default_rng(n + 100 * m)
