@@ -309,11 +309,12 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
 // GP: the panel stays in global memory (systems too large for an LDS panel).
 // keepL bit 0: apply the interchanges to the factored columns left of the
 // panel too (zgetrf layout); bit 1: only interchange B's rows (its triangular
-// solves run after the factorization).
+// solves run after the factorization).  Columns right of the panel are all
+// interchanged; only those before cend are also solved with L11.
 template <int NB, bool GP>
 __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
                                                                cd* B, int ldb, long long sB, int* ipiv, int* info,
-                                                               int keepL) {
+                                                               int keepL, int cend) {
   extern __shared__ cd smem[];
   __shared__ double rv[LB_THREADS / 64];
   __shared__ int ri[LB_THREADS / 64];
@@ -417,7 +418,8 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
         *y = t;
       }
     }
-    if (e >= nleft && (e < nleft + nright || !(keepL & 2))) {
+    const bool trsm = (e >= nleft && e < nleft + nright) ? (j0 + nb + (e - nleft) < cend) : (e >= nleft && !(keepL & 2));
+    if (trsm) {
       cd x[NB];
 #pragma unroll
       for (int i = 0; i < NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ld] : mkc(0.0, 0.0);
@@ -553,6 +555,34 @@ static int trsm_upper_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, 
 template <int NB>
 static size_t lu_panel_lds(int n) { return (size_t)n * NB * sizeof(cd); }
 
+// Recursive LU of the columns c0 .. c1 (rows c0 .. n), all earlier columns
+// already applied:  factor the left half;  U12 = L11^-1 A12;
+// A22 -= L21 U12 (one GEMM with K = half the width);  factor the right half.
+// The NB-wide leaves are zlu_panel_kernel launches, which interchange every
+// column (and B) at once, so no pivot list has to be kept.
+template <int NB>
+static int lu_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, int mode, bool gp,
+                  size_t lds, int c0, int c1) {
+  if (c1 - c0 <= NB) {
+    if (gp)
+      zlu_panel_kernel<NB, true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, c0, A.p, A.ld, A.stride, B.p, B.ld,
+                                                                      B.stride, ipiv, info, mode, c1);
+    else
+      zlu_panel_kernel<NB, false><<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, c0, A.p, A.ld, A.stride, B.p,
+                                                                         B.ld, B.stride, ipiv, info, mode, c1);
+    KHX_LAUNCH_CHECK();
+    return 0;
+  }
+  const int mid = c0 + ((c1 - c0) / 2 + NB - 1) / NB * NB;
+  KHX_TRY(lu_rec<NB>(s, n, m, A, B, batch, ipiv, info, mode, gp, lds, c0, mid));
+  const Mat right{A.p + mid, A.ld, A.stride};
+  KHX_TRY(trsm_lower_rec<NB>(s, n, c1 - mid, A, right, batch, c0, mid));
+  KHX_TRY(zgemm(s, n - mid, c1 - mid, mid - c0, mkc(-1.0, 0.0), Mat{A.p + (long long)mid * A.ld + c0, A.ld, A.stride},
+                Mat{A.p + (long long)c0 * A.ld + mid, A.ld, A.stride}, mkc(1.0, 0.0),
+                Mat{A.p + (long long)mid * A.ld + mid, A.ld, A.stride}, batch));
+  return lu_rec<NB>(s, n, m, A, B, batch, ipiv, info, mode, gp, lds, mid, c1);
+}
+
 bool lu_multi_fits(int n) { return n >= 1; }  // larger systems keep the panel in global memory
 
 template <int NB>
@@ -574,14 +604,25 @@ static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, in
   const char* rs_env = getenv("KHX_LU_RECSOLVE");
   const bool rec = m >= 64 && !(rs_env && atoi(rs_env) == 0);
   const int mode = (keepL || rec ? 1 : 0) | (rec ? 2 : 0);
+  // the factorization itself recursively (deep-K GEMM updates) for n >= 200
+  // when B is absent or solved afterwards (KHX_LU_RECFACTOR=0/1 forces it)
+  const char* rf_env = getenv("KHX_LU_RECFACTOR");
+  const bool recf = rf_env ? atoi(rf_env) != 0 : n >= 200;  // measured: panel-by-panel is faster below
+  if ((m <= 0 || rec) && recf) {
+    // the TRSM / GEMM updates read L in final row order: interchanges always reach the factored columns
+    KHX_TRY(lu_rec<NB>(s, n, m, A, B, batch, ipiv, info, mode | 1, gp, lds, 0, n));
+    if (m <= 0) return 0;
+    KHX_TRY(trsm_lower_rec<NB>(s, n, m, A, B, batch, 0, n));
+    return trsm_upper_rec<NB>(s, n, m, A, B, batch, 0, n);
+  }
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int nb = min(NB, n - j0);
     if (gp)
       zlu_panel_kernel<NB, true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
-                                                                      B.stride, ipiv, info, mode);
+                                                                      B.stride, ipiv, info, mode, n);
     else
       zlu_panel_kernel<NB, false><<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
-                                                                         B.stride, ipiv, info, mode);
+                                                                         B.stride, ipiv, info, mode, n);
     KHX_LAUNCH_CHECK();
     const int r1 = j0 + nb, rest = n - r1;
     if (rest > 0) {

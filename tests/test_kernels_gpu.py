@@ -293,3 +293,27 @@ def test_eig_chase_variants(gpu, monkeypatch, reg_chase):
         ref = np.linalg.eigvals(M)
         d = np.abs(lam[:, None] - ref[None, :])
         assert d.min(axis=1).max() < 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n", [97, 130, 162, 250])
+@pytest.mark.parametrize("recfactor", ["0", "1"])
+def test_lu_recursive_paths(gpu, monkeypatch, n, recfactor):
+    """Multi-launch LU with the factorization recursive or panel by panel and
+    the triangular solves after it (right-hand sides >= 64)."""
+    from khepri_amd import ops
+    monkeypatch.setenv("KHX_LU_RECFACTOR", recfactor)
+    rng = np.random.default_rng(n + 7)
+    A, B = crand(rng, 2, n, n), crand(rng, 2, n, 64)
+    X, info = ops.solve(dev(A, gpu), dev(B, gpu))
+    assert (info.cpu().numpy() == 0).all()
+    ref = np.linalg.solve(A, B)
+    np.testing.assert_allclose(X.cpu().numpy(), ref, atol=1e-10 * np.abs(ref).max())
+    lu, piv, info = ops.lu_factor(dev(A, gpu))
+    lu, piv = lu.cpu().numpy(), piv.cpu().numpy()
+    for b in range(2):
+        L = np.tril(lu[b], -1) + np.eye(n)
+        U = np.triu(lu[b])
+        PA = A[b].copy()
+        for k in range(n):
+            PA[[k, piv[b, k]]] = PA[[piv[b, k], k]]
+        np.testing.assert_allclose(L @ U, PA, atol=1e-12 * np.abs(A[b]).max())
