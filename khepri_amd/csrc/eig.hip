@@ -141,16 +141,16 @@ __device__ __forceinline__ void rot_rows(cd& x, cd& y, double c, cd s) {
 // v[ihi*step] (a row of H above the window: step 1, column rotations; a
 // column right of it or of Zt: step n).  Loads go out eight at a time so the
 // sequential chain does not wait on each one.
-template <bool ROWS>
+template <bool ROWS, int PF = 8>
 __device__ __forceinline__ void slide(cd* v, int step, int l, int ihi, const double* crot, const cd* srot) {
   cd xv = v[l * step];
   int k = l;
-  for (; k + 8 <= ihi; k += 8) {
-    cd yv[8];
+  for (; k + PF <= ihi; k += PF) {
+    cd yv[PF];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) yv[m] = v[(k + 1 + m) * step];
+    for (int m = 0; m < PF; ++m) yv[m] = v[(k + 1 + m) * step];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < PF; ++m) {
       if (ROWS)
         rot_rows(xv, yv[m], crot[k + m], srot[k + m]);
       else
@@ -677,6 +677,9 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+#ifndef EIG_SLIDE_PF
+#define EIG_SLIDE_PF 8  // loads in flight per line in the in-window updates (16: register spills, slower)
+#endif
 template <int EIG_THREADS>
 __device__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, double* crot, cd* srot,
                              unsigned long long* acc) {
@@ -804,9 +807,9 @@ __device__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, dou
     const int tot = nabove + nright;
     for (int e = tid; e < tot; e += EIG_THREADS) {
       if (e < nabove)
-        slide<false>(H + (long long)(l + e) * n, 1, kstart, kend + 1, crot, srot);
+        slide<false, EIG_SLIDE_PF>(H + (long long)(l + e) * n, 1, kstart, kend + 1, crot, srot);
       else
-        slide<true>(H + b1 + 1 + (e - nabove), n, kstart, kend + 1, crot, srot);
+        slide<true, EIG_SLIDE_PF>(H + b1 + 1 + (e - nabove), n, kstart, kend + 1, crot, srot);
     }
     __syncthreads();
     if (kend == ihi - 1) break;
