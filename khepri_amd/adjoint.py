@@ -123,6 +123,25 @@ def flux_gradient(crystal, wT=1.0, wR=0.0, wrt=None, reduce=False, eta_rel=1e-12
     return R, T, grads
 
 
+def _layers_forward(cl, names, kx, ky, k0):
+    """_layer_forward for every named layer; the patterned ones in one fused
+    call when their eigenproblems would not fill the GPU on their own."""
+    pat = [nm for nm in names if cl.layers[nm].formulation == Formulation.FFT]
+    B, N = kx.shape
+    out = {}
+    if len(pat) >= 2 and B * len(pat) <= 4 * ops.eig_slots(2 * N) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1":
+        mats = [cl.layers[nm].convolution_matrices(kx.device) for nm in pat]
+        k0ds = [(k0 * cl.layers[nm].depth).contiguous() for nm in pat]
+        res = ops.patterned_layers_fused([m[0] for m in mats], [m[1] for m in mats], kx, ky, k0ds, modes=True)
+        for nm, (C, IC), k0d, (S, (W, V, lam), info) in zip(pat, mats, k0ds, res):
+            ops.raise_on_info(info, "patterned layer")
+            out[nm] = ("dense", S, dict(C=C, IC=IC, W=W, V=V, lam=lam, k0d=k0d))
+    for nm in names:
+        if nm not in out:
+            out[nm] = _layer_forward(cl.layers[nm], kx, ky, k0)
+    return out
+
+
 def _layer_forward(layer, kx, ky, k0):
     """("identity", None) | ("dense", S, extras) for one layer over a batch."""
     k0d = (k0 * layer.depth).contiguous()
@@ -151,7 +170,7 @@ def _gradient_chunk(cl, names, wrt, g, N, kp, wl, te, tm, wR, wT, eta_rel, reduc
     kx, ky = ops.kvectors(g, kp, wl)
     k0 = 2 * np.pi / wl
     # 1. layer S-matrices and modes (recomputed)
-    fwd = {nm: _layer_forward(cl.layers[nm], kx, ky, k0) for nm in names}
+    fwd = _layers_forward(cl, names, kx, ky, k0)
     seq = [nm for nm in cl.global_stacking if fwd[nm][0] == "dense"]
     c = ops.incident(N, kp, wl, te, tm, cl.epsi)
     # 2. chain products P_k = P_{k-1} * S_k
@@ -179,8 +198,23 @@ def _gradient_chunk(cl, names, wrt, g, N, kp, wl, te, tm, wR, wT, eta_rel, reduc
         _accumulate(Sbar, seq[k], SkBar)
         G = Pbar
     _accumulate(Sbar, seq[0], G)
-    # 5. layer adjoints -> Cbar -> pixels
+    # 5. layer adjoints -> Cbar -> pixels (the layers together in one call
+    #    when one layer's batch would leave the GPU mostly idle)
     out = {}
+    live = [nm for nm in wrt if nm in Sbar]
+    Cbars = {}
+    if len(live) >= 2 and Bc * len(live) <= 4 * ops.eig_slots(n) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1":
+        L = len(live)
+        exs = [fwd[nm][2] for nm in live]
+        Cf = torch.stack([ex["C"].expand(Bc, N, N) for ex in exs]).reshape(L * Bc, N, N)
+        ICf = torch.stack([ex["IC"].expand(Bc, N, N) for ex in exs]).reshape(L * Bc, N, N)
+        Cbar_all, info = ops.patterned_layer_bwd(
+            Cf, ICf, kx.repeat(L, 1), ky.repeat(L, 1), torch.cat([ex["k0d"].reshape(Bc) for ex in exs]),
+            torch.cat([ex["W"] for ex in exs]), torch.cat([ex["V"] for ex in exs]),
+            torch.cat([ex["lam"] for ex in exs]), torch.cat([Sbar[nm] for nm in live]), eta_rel)
+        ops.raise_on_info(info, "layer adjoint")
+        for l, nm in enumerate(live):
+            Cbars[nm] = Cbar_all[l * Bc:(l + 1) * Bc]
     for nm in wrt:
         ex = fwd[nm][2]
         if nm not in Sbar:
@@ -188,9 +222,12 @@ def _gradient_chunk(cl, names, wrt, g, N, kp, wl, te, tm, wR, wT, eta_rel, reduc
             out[nm] = torch.zeros(((Bc,) if not reduce else ()) + shape, device=dev,
                                   dtype=torch.complex128 if np.iscomplexobj(cl.layers[nm].epsilon) else torch.float64)
             continue
-        Cbar, info = ops.patterned_layer_bwd(ex["C"], ex["IC"], kx, ky, ex["k0d"], ex["W"], ex["V"], ex["lam"],
-                                             Sbar[nm], eta_rel)
-        ops.raise_on_info(info, "layer adjoint")
+        if nm in Cbars:
+            Cbar = Cbars[nm]
+        else:
+            Cbar, info = ops.patterned_layer_bwd(ex["C"], ex["IC"], kx, ky, ex["k0d"], ex["W"], ex["V"], ex["lam"],
+                                                 Sbar[nm], eta_rel)
+            ops.raise_on_info(info, "layer adjoint")
         eps = np.asarray(cl.layers[nm].epsilon)
         if reduce:
             ones = torch.ones((1, Bc), dtype=ops.CDT, device=dev)

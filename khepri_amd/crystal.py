@@ -211,8 +211,7 @@ class Crystal:
         self.S = None
         self._RT = None
         if not rt_only and chunk >= B:
-            for name in names:
-                self.layers[name].solve_batch(kp_all, wl_all)
+            Layer.solve_batch_many([self.layers[name] for name in names], kp_all, wl_all)
             self._check_layers(names)
             self.stacking_matrices, self.stacking_reverse_matrices, self.Stot = stack_layers(
                 self.expansion.pw, stacked, self.stack_retain_mask, reverse=want_reverse)
@@ -229,8 +228,7 @@ class Crystal:
         for c0 in range(0, B, chunk):
             sl = slice(c0, min(B, c0 + chunk))
             kp, wl = kp_all[sl], wl_all[sl]
-            for name in names:
-                self.layers[name].solve_batch(kp, wl, rt_only=rt_only)
+            Layer.solve_batch_many([self.layers[name] for name in names], kp, wl, rt_only=rt_only)
             self._check_layers(names)
             if rt_only:
                 c = ops.incident(N, kp, wl, te_all[sl], tm_all[sl], self.epsi)

@@ -1363,7 +1363,20 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   cd* X = Zt + (size_t)n * n * batch;
   const long long sZ = (long long)n * n;
   const char* env = getenv("KHX_EIG_THREADS");
-  const int threads = env ? atoi(env) : EIG_THREADS_DEFAULT;
+  // a batch that leaves CUs idle gets 512 threads per matrix (measured faster
+  // there: the blocked Hessenberg and the deferred passes spread over more lanes)
+  int threads = EIG_THREADS_DEFAULT;
+  if (env) {
+    threads = atoi(env);
+  } else {
+    static int cus[16] = {0};
+    int dev = 0;
+    KHX_CHECK_HIP(hipGetDevice(&dev));
+    if (dev >= 0 && dev < 16) {
+      if (cus[dev] == 0) KHX_CHECK_HIP(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+      if (batch <= cus[dev]) threads = 512;
+    }
+  }
   KHX_REQUIRE(threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024,
               "zeig: KHX_EIG_THREADS must be 64, 128, 256, 512 or 1024");
   // optional LDS padding: fewer resident workgroups per CU keep the working

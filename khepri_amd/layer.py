@@ -8,6 +8,8 @@ from __future__ import annotations
 
 from enum import IntEnum
 
+import os
+
 import numpy as np
 import torch
 
@@ -137,6 +139,39 @@ class Layer:
 
     def chain_item(self):
         return (self.skind, self.X if self.skind == "prop" else self.S)
+
+    @staticmethod
+    def solve_batch_many(layers, kp, wl, rt_only=False):
+        """solve_batch for several layers of one expansion.  When the batch is
+        smaller than one round of the eigen-solver (khx_eig_slots), the
+        patterned layers are solved together in one fused call so that their
+        eigenproblems share the GPU; everything else as solve_batch."""
+        pat = [ly for ly in layers if getattr(ly, "formulation", None) == Formulation.FFT]
+        B = kp.shape[0]
+        fuse = len(pat) >= 2 and B * len(pat) <= 4 * ops.eig_slots(2 * pat[0].expansion.N) and \
+            all(ly.expansion is pat[0].expansion for ly in pat) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1"
+        if not fuse:
+            for ly in layers:
+                ly.solve_batch(kp, wl, rt_only=rt_only)
+            return
+        dev = kp.device
+        kx, ky = ops.kvectors(pat[0].expansion.g_device(dev), kp, wl)
+        k0 = 2 * np.pi / wl
+        mats = [ly.convolution_matrices(dev) for ly in pat]
+        modes = any(bool(ly.fields) for ly in pat)
+        res = ops.patterned_layers_fused([m[0] for m in mats], [m[1] for m in mats], kx, ky,
+                                         [(k0 * ly.depth).contiguous() for ly in pat], modes=modes)
+        for ly, (C, IC), (S, (W, V, L), info) in zip(pat, mats, res):
+            ly.device = dev
+            ly.skind, ly.X = "dense", None
+            ly.S, ly.info, ly.C = S, info, C
+            keep = bool(ly.fields)
+            ly.IC = IC if keep else None
+            ly.W, ly.V, ly.L = (W, V, L) if keep else (None, None, None)
+            ly.kx, ly.ky = (kx, ky) if keep else (None, None)
+        for ly in layers:
+            if getattr(ly, "formulation", None) != Formulation.FFT:
+                ly.solve_batch(kp, wl, rt_only=rt_only)
 
     def solve_batch(self, kp, wl, ws=None, rt_only=False):
         """S-matrix for a batch.  With rt_only (R/T wanted, no fields), layers of

@@ -284,6 +284,27 @@ def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None):
     return S, (W, V, lam), info
 
 
+def patterned_layers_fused(Cs, ICs, kx, ky, k0ds, modes=False):
+    """patterned_layer for several layers over the same sources in one call:
+    item l*B + b is layer l at source b, so the eigen-solver and every other
+    stage runs once over L*B items (a small batch of one layer leaves most CUs
+    idle).  Returns one (S, (W, V, lam), info) per layer, as views."""
+    L = len(Cs)
+    B, N = kx.shape
+    C = torch.stack([c.expand(B, N, N) for c in Cs]).reshape(L * B, N, N)
+    IC = torch.stack([c.expand(B, N, N) for c in ICs]).reshape(L * B, N, N)
+    kxl = kx.repeat(L, 1)
+    kyl = ky.repeat(L, 1)
+    k0dl = torch.cat([k.reshape(B) for k in k0ds])
+    S, (W, V, lam), info = patterned_layer(C, IC, kxl, kyl, k0dl, modes=modes)
+    out = []
+    for l in range(L):
+        sl = slice(l * B, (l + 1) * B)
+        out.append((S[sl], (W[sl] if modes else None, V[sl] if modes else None, lam[sl] if modes else None),
+                    info[sl]))
+    return out
+
+
 def analytic_layer(kind, kx, ky, eps, k0d=None, modes=False):
     """kind 'uniform' | 'reflexion' | 'transmission'."""
     lib = _lib.load()
