@@ -252,6 +252,10 @@ int zgetrf_solve(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv
 
 int zgetrs(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch) {
   if (batch <= 0 || n <= 0 || m <= 0) return 0;
+  // large systems: the sequential row-by-row solves below leave the GPU idle
+  // (one wavefront per system); split the triangles into GEMMs instead
+  if (n >= 200 && batch <= 65535 && (A.stride != 0 || batch == 1) && (B.stride != 0 || batch == 1))
+    return zlu_solve_factored(s, n, m, A, ipiv, B, batch);
   if (m <= ZGETRS_VEC_MAX && (size_t)n * m * sizeof(cd) <= 64 * 1024) {
     ProfScope ps("zgetrs_kernel", s);
     zgetrs_vec_kernel<<<dim3(batch), dim3(64), (size_t)n * m * sizeof(cd), s>>>(n, m, A.p, A.ld, A.stride, ipiv,

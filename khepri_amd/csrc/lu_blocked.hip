@@ -20,6 +20,7 @@ namespace khx {
 constexpr int LB_THREADS = 256;
 constexpr int LB_NB = 16;  // panel width
 constexpr int LB_TW = 64;  // column tile of the mini-GEMM
+constexpr int LB_SG = 4;   // interchanges per independent group in the panel kernel (the code spells out 4)
 
 // C[r][c] -= sum_k L[r - r0][k] * U[k][c - cbase] for rows r in [r0, r1) and
 // tile columns [0, tw); L rows live in LDS with stride LB_NB, the U tile in
@@ -391,6 +392,22 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
     else if (info[item] == 0)
       info[item] = first_zero;
   }
+  // The interchanges go in groups of LB_SG: when the rows a group touches
+  // are distinct (no pivot row is another row of the group), its swaps are
+  // independent and each column loads them all before storing, instead of
+  // one dependent load / store round per interchange.
+  __shared__ int gfree[NB / LB_SG];
+  if (tid < NB / LB_SG) {
+    bool ok = true;
+    const int g0 = tid * LB_SG;
+    for (int i = g0; i < g0 + LB_SG && i < nb; ++i) {
+      const int pi = perm[i];
+      for (int k = g0; k < g0 + LB_SG && k < nb; ++k)
+        if (k != i && (pi == k || (pi != i && perm[k] == pi))) ok = false;
+    }
+    gfree[tid] = ok ? 1 : 0;
+  }
+  __syncthreads();
   // interchanges on the other columns and the triangular solve with L11,
   // one thread per column
   const int nleft = (keepL & 1) ? j0 : 0;
@@ -408,14 +425,32 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
       col = bm + (e - nleft - nright);
       ld = ldb;
     }
-    for (int jj = 0; jj < nb; ++jj) {
-      const int p = perm[jj];
-      if (p != jj) {
-        cd* x = col + (long long)(j0 + jj) * ld;
-        cd* y = col + (long long)(j0 + p) * ld;
-        const cd t = *x;
-        *x = *y;
-        *y = t;
+    cd* c0p = col + (long long)j0 * ld;
+    for (int g0 = 0; g0 < nb; g0 += LB_SG) {
+      if (gfree[g0 / LB_SG]) {
+        const cd z = mkc(0.0, 0.0);
+        cd xa0 = z, xa1 = z, xa2 = z, xa3 = z, xb0 = z, xb1 = z, xb2 = z, xb3 = z;
+        const int p0 = g0 < nb ? perm[g0] : g0, p1 = g0 + 1 < nb ? perm[g0 + 1] : g0 + 1;
+        const int p2 = g0 + 2 < nb ? perm[g0 + 2] : g0 + 2, p3 = g0 + 3 < nb ? perm[g0 + 3] : g0 + 3;
+        if (p0 != g0) { xa0 = c0p[(long long)g0 * ld]; xb0 = c0p[(long long)p0 * ld]; }
+        if (p1 != g0 + 1) { xa1 = c0p[(long long)(g0 + 1) * ld]; xb1 = c0p[(long long)p1 * ld]; }
+        if (p2 != g0 + 2) { xa2 = c0p[(long long)(g0 + 2) * ld]; xb2 = c0p[(long long)p2 * ld]; }
+        if (p3 != g0 + 3) { xa3 = c0p[(long long)(g0 + 3) * ld]; xb3 = c0p[(long long)p3 * ld]; }
+        if (p0 != g0) { c0p[(long long)g0 * ld] = xb0; c0p[(long long)p0 * ld] = xa0; }
+        if (p1 != g0 + 1) { c0p[(long long)(g0 + 1) * ld] = xb1; c0p[(long long)p1 * ld] = xa1; }
+        if (p2 != g0 + 2) { c0p[(long long)(g0 + 2) * ld] = xb2; c0p[(long long)p2 * ld] = xa2; }
+        if (p3 != g0 + 3) { c0p[(long long)(g0 + 3) * ld] = xb3; c0p[(long long)p3 * ld] = xa3; }
+      } else {
+        for (int jj = g0; jj < g0 + LB_SG && jj < nb; ++jj) {
+          const int p = perm[jj];
+          if (p != jj) {
+            cd* x = c0p + (long long)jj * ld;
+            cd* y = c0p + (long long)p * ld;
+            const cd t = *x;
+            *x = *y;
+            *y = t;
+          }
+        }
       }
     }
     const bool trsm = (e >= nleft && e < nleft + nright) ? (j0 + nb + (e - nleft) < cend) : (e >= nleft && !(keepL & 2));
@@ -552,8 +587,73 @@ static int trsm_upper_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, 
   return trsm_upper_rec<NB>(s, n, m, A, B, batch, r0, mid);
 }
 
+// Row interchanges of zgetrf (ipiv, 0-based, applied in order k = 0 .. n-1)
+// to the m columns of B.  One workgroup per system: B staged in LDS when it
+// fits, one thread per column walking the interchanges.
+__global__ __launch_bounds__(256) void zlaswp_kernel(int n, int m, const int* __restrict__ ipiv, cd* B, int ldb,
+                                                     long long sB, int staged) {
+  extern __shared__ cd sb[];
+  const long long item = blockIdx.x;
+  cd* bm = B + item * sB;
+  const int* pv = ipiv + item * n;
+  if (staged) {
+    for (int e = threadIdx.x; e < n * m; e += blockDim.x) sb[e] = bm[(long long)(e / m) * ldb + e % m];
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x)
+      for (int k = 0; k < n; ++k) {
+        const int p = pv[k];
+        if (p != k) {
+          const cd t = sb[k * m + j];
+          sb[k * m + j] = sb[p * m + j];
+          sb[p * m + j] = t;
+        }
+      }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * m; e += blockDim.x) bm[(long long)(e / m) * ldb + e % m] = sb[e];
+    return;
+  }
+  for (int j = threadIdx.x; j < m; j += blockDim.x)
+    for (int k = 0; k < n; ++k) {
+      const int p = pv[k];
+      if (p != k) {
+        const cd t = bm[(long long)k * ldb + j];
+        bm[(long long)k * ldb + j] = bm[(long long)p * ldb + j];
+        bm[(long long)p * ldb + j] = t;
+      }
+    }
+}
+
+// B <- A^-1 B from zgetrf factors (any path): interchanges, then the two
+// triangular solves split in halves into GEMMs (trsm_*_rec).
+int zlu_solve_factored(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch) {
+  if (batch <= 0 || n <= 0 || m <= 0) return 0;
+  KHX_REQUIRE(batch <= 65535, "zlu_solve_factored: at most 65535 systems per call");
+  ProfScope ps("zgetrs_kernel", s);
+  const size_t lds = (size_t)n * m * sizeof(cd);
+  const int staged = lds <= 64 * 1024 ? 1 : 0;
+  zlaswp_kernel<<<dim3(batch), dim3(256), staged ? lds : 0, s>>>(n, m, ipiv, B.p, B.ld, B.stride, staged);
+  KHX_LAUNCH_CHECK();
+  KHX_TRY(trsm_lower_rec<16>(s, n, m, A, B, batch, 0, n));
+  return trsm_upper_rec<16>(s, n, m, A, B, batch, 0, n);
+}
+
 template <int NB>
 static size_t lu_panel_lds(int n) { return (size_t)n * NB * sizeof(cd); }
+
+// One panel step (panel in LDS, or in global memory when it does not fit).
+template <int NB>
+static int launch_panel(hipStream_t s, int n, int m, int j0, Mat A, Mat B, int batch, int* ipiv, int* info, int mode,
+                        int cend, bool gp, size_t lds) {
+  if (gp) {
+    zlu_panel_kernel<NB, true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
+                                                                      B.stride, ipiv, info, mode, cend);
+  } else {
+    zlu_panel_kernel<NB, false><<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
+                                                                         B.stride, ipiv, info, mode, cend);
+  }
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
 
 // Recursive LU of the columns c0 .. c1 (rows c0 .. n), all earlier columns
 // already applied:  factor the left half;  U12 = L11^-1 A12;
@@ -563,16 +663,7 @@ static size_t lu_panel_lds(int n) { return (size_t)n * NB * sizeof(cd); }
 template <int NB>
 static int lu_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, int mode, bool gp,
                   size_t lds, int c0, int c1) {
-  if (c1 - c0 <= NB) {
-    if (gp)
-      zlu_panel_kernel<NB, true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, c0, A.p, A.ld, A.stride, B.p, B.ld,
-                                                                      B.stride, ipiv, info, mode, c1);
-    else
-      zlu_panel_kernel<NB, false><<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, c0, A.p, A.ld, A.stride, B.p,
-                                                                         B.ld, B.stride, ipiv, info, mode, c1);
-    KHX_LAUNCH_CHECK();
-    return 0;
-  }
+  if (c1 - c0 <= NB) return launch_panel<NB>(s, n, m, c0, A, B, batch, ipiv, info, mode, c1, gp, lds);
   const int mid = c0 + ((c1 - c0) / 2 + NB - 1) / NB * NB;
   KHX_TRY(lu_rec<NB>(s, n, m, A, B, batch, ipiv, info, mode, gp, lds, c0, mid));
   const Mat right{A.p + mid, A.ld, A.stride};
@@ -617,13 +708,7 @@ static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, in
   }
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int nb = min(NB, n - j0);
-    if (gp)
-      zlu_panel_kernel<NB, true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
-                                                                      B.stride, ipiv, info, mode, n);
-    else
-      zlu_panel_kernel<NB, false><<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
-                                                                         B.stride, ipiv, info, mode, n);
-    KHX_LAUNCH_CHECK();
+    KHX_TRY(launch_panel<NB>(s, n, m, j0, A, B, batch, ipiv, info, mode, n, gp, lds));
     const int r1 = j0 + nb, rest = n - r1;
     if (rest > 0) {
       const Mat L21{A.p + (long long)r1 * A.ld + j0, A.ld, A.stride};
