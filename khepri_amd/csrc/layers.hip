@@ -82,6 +82,51 @@ __global__ __launch_bounds__(256) void fourier_coef_kernel(const double* __restr
   }
 }
 
+// The same coefficients as a separable DFT, one workgroup per pixmap:
+// G[x][dv] = sum_y eps[x][y] e^{-2 pi i dv y / ny} (LDS), then
+// F[du][dv] = sum_x G[x][dv] e^{-2 pi i du x / nx} / (nx ny).  Phases are
+// reduced exactly in integers before sincospi.  (2Q-1) nx ny + (2P-1)(2Q-1) nx
+// terms instead of (2P-1)(2Q-1) nx ny.
+__global__ __launch_bounds__(256) void fourier_coef_sep_kernel(const double* __restrict__ eps, int cplx, int nx,
+                                                              int ny, int P, int Q, cd* __restrict__ F) {
+  extern __shared__ cd G[];  // nx x (2Q - 1)
+  const int ncu = 2 * P - 1, ncv = 2 * Q - 1;
+  const long long layer = blockIdx.x;
+  const double* e = eps + layer * (long long)nx * ny * (cplx ? 2 : 1);
+  for (int t = threadIdx.x; t < nx * ncv; t += blockDim.x) {
+    const int x = t / ncv, dv = t % ncv - (Q - 1);
+    const long long dvn = (((long long)dv % ny) + ny) % ny;
+    double sx = 0.0, sy = 0.0;
+    for (int y = 0; y < ny; ++y) {
+      const long long m = (dvn * y) % ny;
+      double sn, cs;
+      sincospi(-2.0 * (double)m / (double)ny, &sn, &cs);
+      const long long idx = (long long)x * ny + y;
+      const double er = cplx ? e[2 * idx] : e[idx];
+      const double ei = cplx ? e[2 * idx + 1] : 0.0;
+      sx += er * cs - ei * sn;
+      sy += er * sn + ei * cs;
+    }
+    G[t] = mkc(sx, sy);
+  }
+  __syncthreads();
+  const double inv = 1.0 / ((double)nx * (double)ny);
+  for (int t = threadIdx.x; t < ncu * ncv; t += blockDim.x) {
+    const int du = t / ncv - (P - 1), dvi = t % ncv;
+    const long long dun = (((long long)du % nx) + nx) % nx;
+    double sx = 0.0, sy = 0.0;
+    for (int x = 0; x < nx; ++x) {
+      const long long m = (dun * x) % nx;
+      double sn, cs;
+      sincospi(-2.0 * (double)m / (double)nx, &sn, &cs);
+      const cd g = G[x * ncv + dvi];
+      sx += g.x * cs - g.y * sn;
+      sy += g.x * sn + g.y * cs;
+    }
+    F[layer * ncu * ncv + t] = mkc(sx * inv, sy * inv);
+  }
+}
+
 // C[q*P+p][q'*P+p'] = eps_hat[p-p'][q-q']  (khepri/tools.py:38-56)
 __global__ void toeplitz_kernel(const cd* __restrict__ F, int P, int Q, cd* __restrict__ C) {
   const int N = P * Q, ncv = 2 * Q - 1;
@@ -104,7 +149,11 @@ int convmat(hipStream_t s, const double* eps, int cplx, int nx, int ny, int P, i
               "convmat: index out of bounds -- the pixmap is too small for the plane-wave count");
   const int ncoef = (2 * P - 1) * (2 * Q - 1);
   KHX_REQUIRE(count <= 65535, "convmat: at most 65535 pixmaps per call");
-  fourier_coef_kernel<<<dim3(ncoef, count), 256, 0, s>>>(eps, cplx, nx, ny, P, Q, Fws);
+  const size_t glds = (size_t)nx * (2 * Q - 1) * sizeof(cd);
+  if (glds <= 64 * 1024 && !getenv("KHX_DIRECT_DFT"))
+    fourier_coef_sep_kernel<<<dim3(count), 256, glds, s>>>(eps, cplx, nx, ny, P, Q, Fws);
+  else
+    fourier_coef_kernel<<<dim3(ncoef, count), 256, 0, s>>>(eps, cplx, nx, ny, P, Q, Fws);
   KHX_LAUNCH_CHECK();
   const int N = P * Q;
   int gx = (N * N + 255) / 256;

@@ -317,3 +317,17 @@ def test_lu_recursive_paths(gpu, monkeypatch, n, recfactor):
         for k in range(n):
             PA[[k, piv[b, k]]] = PA[[piv[b, k], k]]
         np.testing.assert_allclose(L @ U, PA, atol=1e-12 * np.abs(A[b]).max())
+
+
+@pytest.mark.parametrize("direct", ["0", "1"])
+@pytest.mark.parametrize("shape,pw", [((128, 128), (13, 13)), ((37, 52), (5, 7)), ((64, 64), (11, 11))])
+def test_convmat_dft_variants(gpu, monkeypatch, direct, shape, pw):
+    """Separable and direct DFT of the pixmap give the oracle's coefficients."""
+    from khepri_amd import ops
+    if direct == "1":
+        monkeypatch.setenv("KHX_DIRECT_DFT", "1")
+    rng = np.random.default_rng(sum(shape))
+    pix = rng.uniform(1.0, 12.0, size=shape) + 1j * rng.uniform(0.0, 0.2, size=shape)
+    C = ops.convmat(dev(pix, gpu), pw)[0].cpu().numpy()
+    ref = rcwa.convolution_matrix(pix, pw)
+    np.testing.assert_allclose(C, ref, atol=2e-15 * np.abs(ref).max() * 8)
