@@ -331,3 +331,59 @@ def test_convmat_dft_variants(gpu, monkeypatch, direct, shape, pw):
     C = ops.convmat(dev(pix, gpu), pw)[0].cpu().numpy()
     ref = rcwa.convolution_matrix(pix, pw)
     np.testing.assert_allclose(C, ref, atol=2e-15 * np.abs(ref).max() * 8)
+
+
+EIG_VARIANTS = [
+    {"KHX_EIG_TWO_SHIFT": "1", "KHX_EIG_REG_CHASE": "0"},
+    {"KHX_EIG_CHASE_WAVE": "0", "KHX_EIG_REG_CHASE": "0"},
+    {"KHX_EIG_THREADS": "64"}, {"KHX_EIG_THREADS": "128"}, {"KHX_EIG_THREADS": "512"},
+    {"KHX_EIG_THREADS": "1024"}, {"KHX_EIG_LDS_PAD": "8192"},
+]
+
+
+@pytest.mark.parametrize("env", EIG_VARIANTS, ids=lambda e: "-".join(f"{k[4:]}={v}" for k, v in e.items()))
+def test_eig_tuning_variants(gpu, monkeypatch, env):
+    """Every launch-shape / chase variant selectable through the environment
+    solves the layer eigenproblem (n = 50 in LDS, n = 162 in HBM)."""
+    from khepri_amd import ops
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    pix = W.disc_pixmap()
+    for pw in ((5, 5), (9, 9)):
+        C = rcwa.convolution_matrix(pix, pw)
+        g = rcwa.g_vectors(pw, np.eye(2))
+        mats = []
+        for wl, kp in [(2.0, (0.1, 0.3)), (1.7, (0.0, 0.0))]:
+            kx, ky, _ = rcwa.k_vectors(g, kp, wl)
+            P, Q = rcwa.structured_pq(kx, ky, C)
+            mats.append(P @ Q)
+        M = np.array(mats)
+        lam, Wm, info = ops.eig(dev(M, gpu))
+        assert (info.cpu().numpy() == 0).all()
+        lam, Wm = lam.cpu().numpy(), Wm.cpu().numpy()
+        for b in range(len(M)):
+            assert np.abs(M[b] @ Wm[b] - Wm[b] * lam[b]).max() / np.abs(M[b]).max() < 1e-12
+            ref = np.linalg.eigvals(M[b])
+            assert np.abs(lam[b][:, None] - ref[None, :]).min(axis=1).max() < 1e-10 * np.abs(ref).max()
+
+
+LU_VARIANTS = [{"KHX_LU_NB": "32"}, {"KHX_LU_PATH": "0"}, {"KHX_LU_PATH": "1"}, {"KHX_LU_PATH": "2"},
+               {"KHX_LU_UNBLOCKED": "1"}, {"KHX_LU_RECSOLVE": "0"}, {"KHX_LU_NB": "32", "KHX_LU_RECFACTOR": "1"}]
+
+
+@pytest.mark.parametrize("env", LU_VARIANTS, ids=lambda e: "-".join(f"{k[4:]}={v}" for k, v in e.items()))
+def test_lu_tuning_variants(gpu, monkeypatch, env):
+    """Every LU path selectable through the environment: solve and factors."""
+    from khepri_amd import ops
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    rng = np.random.default_rng(17)
+    for n, m in ((50, 70), (162, 5), (162, 100)):
+        A, B = crand(rng, 2, n, n), crand(rng, 2, n, m)
+        X, info = ops.solve(dev(A, gpu), dev(B, gpu))
+        assert (info.cpu().numpy() == 0).all()
+        ref = np.linalg.solve(A, B)
+        np.testing.assert_allclose(X.cpu().numpy(), ref, atol=1e-10 * np.abs(ref).max())
+        lu, piv, info = ops.lu_factor(dev(A, gpu))
+        Xs = ops.lu_solve(lu, piv, dev(B, gpu)).cpu().numpy()
+        np.testing.assert_allclose(Xs, ref, atol=1e-10 * np.abs(ref).max())
