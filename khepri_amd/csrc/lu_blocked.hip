@@ -19,11 +19,12 @@ namespace khx {
 
 constexpr int LB_THREADS = 256;
 constexpr int LB_NB = 16;  // panel width
+constexpr int LB_PS = LB_NB + 1;  // LDS row stride of a panel (padding: one row per thread hits distinct banks)
 constexpr int LB_TW = 64;  // column tile of the mini-GEMM
 constexpr int LB_SG = 4;   // interchanges per independent group in the panel kernel (the code spells out 4)
 
 // C[r][c] -= sum_k L[r - r0][k] * U[k][c - cbase] for rows r in [r0, r1) and
-// tile columns [0, tw); L rows live in LDS with stride LB_NB, the U tile in
+// tile columns [0, tw); L rows live in LDS with stride LB_PS, the U tile in
 // LDS with stride LB_TW.  colptr(t) gives the global address of (row 0,
 // tile column t); rows are ld apart.
 template <typename ColPtr>
@@ -42,7 +43,7 @@ __device__ __forceinline__ void mini_gemm_sub(const cd* Ls, int lrow0, const cd*
     for (int kk = 0; kk < nb; ++kk) {
       cd lv[4], uv[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) lv[i] = (rb + i < r1) ? Ls[(rb + i - lrow0) * LB_NB + kk] : mkc(0.0, 0.0);
+      for (int i = 0; i < 4; ++i) lv[i] = (rb + i < r1) ? Ls[(rb + i - lrow0) * LB_PS + kk] : mkc(0.0, 0.0);
 #pragma unroll
       for (int j = 0; j < 4; ++j) uv[j] = Us[kk * LB_TW + cbase + j];
 #pragma unroll
@@ -101,8 +102,8 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
                                                                  int ldb, long long sB, int* ipiv, int* info,
                                                                  int keepL) {
   extern __shared__ cd smem[];
-  cd* pan = smem;                // n x LB_NB
-  cd* ut = pan + n * LB_NB;      // LB_NB x LB_TW
+  cd* pan = smem;                // n x LB_PS
+  cd* ut = pan + n * LB_PS;      // LB_NB x LB_TW
   __shared__ double rv[LB_THREADS / 64];
   __shared__ int ri[LB_THREADS / 64];
   __shared__ cd rc[LB_THREADS / 64];
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
     const int pr = n - j0;
     for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
       const int r = e / LB_NB, c = e % LB_NB;
-      pan[e] = (c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
+      pan[r * LB_PS + c] = (c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
     }
     __syncthreads();
     // ---- factor the panel in LDS
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
       int bi = pr;
       cd bv = mkc(0.0, 0.0);
       for (int r = jj + tid; r < pr; r += LB_THREADS) {
-        const cd v = pan[r * LB_NB + jj];
+        const cd v = pan[r * LB_PS + jj];
         const double av = cabs1(v);
         if (av > best) {
           best = av;
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
       cd piv = bv;
       if (p >= pr) {  // NaN column: keep the diagonal
         p = jj;
-        piv = pan[jj * LB_NB + jj];
+        piv = pan[jj * LB_PS + jj];
       }
       const bool zero = (piv.x == 0.0 && piv.y == 0.0);
       if (zero && first_zero == 0) first_zero = j0 + jj + 1;
@@ -151,22 +152,22 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
       }
       __syncthreads();  // every thread has read the pivot column before the swap
       if (p != jj && tid < LB_NB) {
-        const cd t = pan[jj * LB_NB + tid];
-        pan[jj * LB_NB + tid] = pan[p * LB_NB + tid];
-        pan[p * LB_NB + tid] = t;
+        const cd t = pan[jj * LB_PS + tid];
+        pan[jj * LB_PS + tid] = pan[p * LB_PS + tid];
+        pan[p * LB_PS + tid] = t;
       }
       __syncthreads();
       for (int r = jj + 1 + tid; r < pr; r += LB_THREADS) {
-        cd* row = pan + r * LB_NB;
+        cd* row = pan + r * LB_PS;
         const cd l = row[jj] * rp;
         row[jj] = l;
-        for (int c = jj + 1; c < nb; ++c) row[c] = fmsc(row[c], l, pan[jj * LB_NB + c]);
+        for (int c = jj + 1; c < nb; ++c) row[c] = fmsc(row[c], l, pan[jj * LB_PS + c]);
       }
       __syncthreads();
     }
     for (int e = tid; e < pr * LB_NB; e += LB_THREADS) {
       const int r = e / LB_NB, c = e % LB_NB;
-      if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[e];
+      if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[r * LB_PS + c];
     }
     // ---- interchanges on the other columns (one thread per column)
     const int nleft = keepL ? j0 : 0;
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
 #pragma unroll
       for (int i = 1; i < LB_NB; ++i)
 #pragma unroll
-        for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], pan[i * LB_NB + k], x[k]);
+        for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], pan[i * LB_PS + k], x[k]);
 #pragma unroll
       for (int i = 0; i < LB_NB; ++i)
         if (i < nb) col[(long long)(j0 + i) * ld] = x[i];
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
     // U[0:j0+nb, j0:j0+nb] -> pan (rows 0..j0+nb-1)
     for (int e = tid; e < (j0 + nb) * LB_NB; e += LB_THREADS) {
       const int r = e / LB_NB, c = e % LB_NB;
-      pan[e] = (c < nb) ? a[(long long)r * lda + j0 + c] : mkc(0.0, 0.0);
+      pan[r * LB_PS + c] = (c < nb) ? a[(long long)r * lda + j0 + c] : mkc(0.0, 0.0);
     }
     __syncthreads();
     // X block = U11^-1 Y block (one thread per column of B)
@@ -276,8 +277,8 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
         if (i >= nb) continue;
 #pragma unroll
         for (int k = i + 1; k < LB_NB; ++k)
-          if (k < nb) x[i] = fmsc(x[i], pan[(j0 + i) * LB_NB + k], x[k]);
-        x[i] = divc(x[i], pan[(j0 + i) * LB_NB + i]);
+          if (k < nb) x[i] = fmsc(x[i], pan[(j0 + i) * LB_PS + k], x[k]);
+        x[i] = divc(x[i], pan[(j0 + i) * LB_PS + i]);
       }
 #pragma unroll
       for (int i = 0; i < LB_NB; ++i)
@@ -328,13 +329,14 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
   int first_zero = 0;
   const int nb = min(NB, n - j0);
   const int pr = n - j0;
-  // panel element (r, c): LDS copy (row stride NB) or the matrix itself
+  // panel element (r, c): LDS copy (row stride NB + 1) or the matrix itself
   cd* pan = GP ? a + (long long)j0 * lda + j0 : smem;
-  const long long pld = GP ? lda : NB;
+  // LDS rows padded by one element: a thread per row then hits distinct banks
+  const long long pld = GP ? lda : NB + 1;
   if (!GP) {
     for (int e = tid; e < pr * NB; e += LB_THREADS) {
       const int r = e / NB, c = e % NB;
-      pan[e] = (c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
+      pan[r * pld + c] = (c < nb) ? a[(long long)(j0 + r) * lda + j0 + c] : mkc(0.0, 0.0);
     }
   }
   __syncthreads();
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
   if (!GP) {
     for (int e = tid; e < pr * NB; e += LB_THREADS) {
       const int r = e / NB, c = e % NB;
-      if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[e];
+      if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[r * pld + c];
     }
   }
   if (tid == 0 && info) {
@@ -638,7 +640,7 @@ int zlu_solve_factored(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat 
 }
 
 template <int NB>
-static size_t lu_panel_lds(int n) { return (size_t)n * NB * sizeof(cd); }
+static size_t lu_panel_lds(int n) { return (size_t)n * (NB + 1) * sizeof(cd); }
 
 // One panel step (panel in LDS, or in global memory when it does not fit).
 template <int NB>
@@ -768,7 +770,7 @@ int lu_path(int n) {
   return 0;
 }
 
-static size_t lu_blocked_lds(int n) { return ((size_t)n * LB_NB + (size_t)LB_NB * LB_TW) * sizeof(cd); }
+static size_t lu_blocked_lds(int n) { return ((size_t)n * LB_PS + (size_t)LB_NB * LB_TW) * sizeof(cd); }
 
 bool lu_blocked_fits(int n) { return lu_blocked_lds(n) <= 150 * 1024; }
 
