@@ -213,9 +213,11 @@ def run_autotune(step, args):
     return {"classes": len(table), "changed": changed, "mfma": mfma, "seconds": round(time.perf_counter() - t0, 2)}
 
 
-def load_traffic(kernel, workload, matrices_per_launch):
+def load_traffic(kernel, workload, matrices_per_launch, avg_launch_ms):
     """HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE profile of
-    the same workload and launch size (None when the profile does not match)."""
+    the same workload and launch size (None when the profile does not match,
+    or when the kernel's launch time moved by more than 5 % since, i.e. the
+    profile is of a different build of the kernel)."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
     if not os.path.exists(path):
         return None
@@ -224,6 +226,9 @@ def load_traffic(kernel, workload, matrices_per_launch):
     if d.get("workload") != workload:
         return None
     if abs(d.get("matrices_per_launch", 0) - matrices_per_launch) > 0.01 * matrices_per_launch:
+        return None
+    ref_ms = d.get("avg_launch_ms")
+    if ref_ms is None or abs(ref_ms - avg_launch_ms) > 0.05 * ref_ms:
         return None
     return d.get("bytes_per_launch")
 
@@ -308,7 +313,7 @@ def main():
     workload = "disc_stack_bz pw(9,9) 65536 solves"
     roofline = dict(bound="mfma", kernel="zeig_schur_kernel", achieved=round(achieved, 4), peak=PEAK_FP64_TFLOPS,
                     unit="TFLOP/s", frac=round(achieved / PEAK_FP64_TFLOPS, 6),
-                    traffic=load_traffic("zeig_schur_kernel", workload, matrices_per_launch),
+                    traffic=load_traffic("zeig_schur_kernel", workload, matrices_per_launch, avg_ms),
                     avg_launch_ms=round(avg_ms, 4), matrices_per_launch=matrices_per_launch,
                     flops_per_matrix=eig_flops(n))
     share = {k: round(v[0] / (el * 1e3 * 1.0), 4) for k, v in kern.items()}

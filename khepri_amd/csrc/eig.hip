@@ -29,7 +29,10 @@ constexpr int EIG_THREADS_DEFAULT = 256;
 constexpr double ULP = 2.220446049250313e-16;
 constexpr double SAFMIN = 2.2250738585072014e-308;
 constexpr int KHX_EIG_LDS_N = 64;  // H resident in LDS up to this order
-constexpr int EIG_QB = 32;         // QR chase block held in LDS when H is in global memory
+#ifndef EIG_QB_SIZE
+#define EIG_QB_SIZE 40  // 40 x 41 padded block: the largest that keeps 4 workgroups per CU at n = 162
+#endif
+constexpr int EIG_QB = EIG_QB_SIZE;  // QR chase block held in LDS when H is in global memory
 
 constexpr int HB_NB = 8;          // panel width of the blocked Hessenberg reduction
 
@@ -429,7 +432,10 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
 // must follow the previous sweep's at p - 1 .. p + 1, so at step t the pass
 // applies sweep q at p = t - 2q: a line is read and written once for all the
 // logged sweeps, its live part (2 EIG_KLOG elements) staying in registers.
-constexpr int EIG_KLOG = 4;
+#ifndef EIG_KLOG_N
+#define EIG_KLOG_N 6  // sweeps per deferred pass (4: +3 % eigensolver time; 7, 8: same as 6)
+#endif
+constexpr int EIG_KLOG = EIG_KLOG_N;
 constexpr int EIG_PF = 4;  // loads in flight per line in the pipelined pass
 
 // Lines are processed EIG_THREADS at a time; the rotation parameters of the
