@@ -442,6 +442,10 @@ constexpr int EIG_PF = 4;  // loads in flight per line in the pipelined pass
 // next EIG_CH steps are staged in LDS (stage, at least 1.5 EIG_KLOG (EIG_CH + 6)
 // complex) so no step waits on a global load for them.
 constexpr int EIG_CH = 32;
+// the staged sines and cosines (K (EIG_CH + LAG) complex + as many doubles) live
+// in the chase-block region of LDS
+static_assert(3 * EIG_KLOG * (EIG_CH + 2 * (EIG_KLOG - 1)) <= 2 * EIG_QB * (EIG_QB + 1),
+              "rotation staging does not fit the chase-block LDS region");
 
 template <int EIG_THREADS>
 __device__ void flush_log(cd* H, cd* Zt, int n, int nsw, const int* sl, const int* sh, const double* lc,
