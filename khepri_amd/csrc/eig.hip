@@ -34,7 +34,10 @@ constexpr int KHX_EIG_LDS_N = 64;  // H resident in LDS up to this order
 #endif
 constexpr int EIG_QB = EIG_QB_SIZE;  // QR chase block held in LDS when H is in global memory
 
-constexpr int HB_NB = 8;          // panel width of the blocked Hessenberg reduction
+#ifndef HB_NB_SIZE
+#define HB_NB_SIZE 8
+#endif
+constexpr int HB_NB = HB_NB_SIZE;  // panel width of the blocked Hessenberg reduction
 
 // LDS layout: u (n), wv (2n, unblocked Hessenberg only), vh (n), part (the
 // Hessenberg partial sums / T factor, then the QR chase block), srot (n),
@@ -185,7 +188,10 @@ __device__ __forceinline__ void slide(cd* v, int step, int l, int ihi, const dou
 // each as one pass per column instead of one pass per reflector.
 // V and Y (n x HB_NB, row-major) live in global scratch; T, the partial sums
 // and the reflector in LDS.
-constexpr int HB_RC = 4;  // rows per staged chunk in the trailing passes
+#ifndef HB_RC_SIZE
+#define HB_RC_SIZE 4
+#endif
+constexpr int HB_RC = HB_RC_SIZE;  // rows per staged chunk in the trailing passes
 
 // One pass over the columns [c0, n) of M (rows r0..n-1), one thread per column:
 //   MODE 0:  M(i, col) -= sum_q F(i, q) conj(G(col, q))            (right update, F = Y, G = V)
