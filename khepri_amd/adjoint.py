@@ -129,7 +129,7 @@ def _layers_forward(cl, names, kx, ky, k0):
     pat = [nm for nm in names if cl.layers[nm].formulation == Formulation.FFT]
     B, N = kx.shape
     out = {}
-    if len(pat) >= 2 and B * len(pat) <= 4 * ops.eig_slots(2 * N) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1":
+    if len(pat) >= 2 and B * len(pat) <= ops.eig_slots(2 * N) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1":
         mats = [cl.layers[nm].convolution_matrices(kx.device) for nm in pat]
         k0ds = [(k0 * cl.layers[nm].depth).contiguous() for nm in pat]
         res = ops.patterned_layers_fused([m[0] for m in mats], [m[1] for m in mats], kx, ky, k0ds, modes=True)
@@ -203,7 +203,7 @@ def _gradient_chunk(cl, names, wrt, g, N, kp, wl, te, tm, wR, wT, eta_rel, reduc
     out = {}
     live = [nm for nm in wrt if nm in Sbar]
     Cbars = {}
-    if len(live) >= 2 and Bc * len(live) <= 4 * ops.eig_slots(n) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1":
+    if len(live) >= 2 and Bc * len(live) <= ops.eig_slots(n) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1":
         L = len(live)
         exs = [fwd[nm][2] for nm in live]
         Cf = torch.stack([ex["C"].expand(Bc, N, N) for ex in exs]).reshape(L * Bc, N, N)

@@ -142,13 +142,15 @@ class Layer:
 
     @staticmethod
     def solve_batch_many(layers, kp, wl, rt_only=False):
-        """solve_batch for several layers of one expansion.  When the batch is
-        smaller than one round of the eigen-solver (khx_eig_slots), the
-        patterned layers are solved together in one fused call so that their
-        eigenproblems share the GPU; everything else as solve_batch."""
+        """solve_batch for several layers of one expansion.  When all their
+        eigenproblems fit one round of the eigen-solver (khx_eig_slots), the
+        patterned layers are solved together in one fused call so that they
+        share the GPU (a single layer's small batch leaves most CUs idle); the
+        bound also keeps the fused workspace within one round's worth of
+        items.  Everything else as solve_batch."""
         pat = [ly for ly in layers if getattr(ly, "formulation", None) == Formulation.FFT]
         B = kp.shape[0]
-        fuse = len(pat) >= 2 and B * len(pat) <= 4 * ops.eig_slots(2 * pat[0].expansion.N) and \
+        fuse = len(pat) >= 2 and B * len(pat) <= ops.eig_slots(2 * pat[0].expansion.N) and \
             all(ly.expansion is pat[0].expansion for ly in pat) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1"
         if not fuse:
             for ly in layers:

@@ -116,8 +116,12 @@ def test_random_devices(gpu):
             np.testing.assert_allclose(to, g["T_orders"], atol=1e-10)
 
 
-def test_grad_grid_forward(gpu):
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_grad_grid_forward(gpu, monkeypatch, fuse):
+    """Four distinct patterned layers: solved in one fused call (default for
+    small batches) and one by one, both against the golden vectors."""
     from khepri_amd import Crystal
+    monkeypatch.setenv("KHX_FUSE_LAYERS", fuse)
     g = golden("grad_grid.npz")
     cfg = W.grad_grid()
     cl = Crystal((11, 11))
