@@ -13,7 +13,7 @@ import numpy as np
 
 from . import rcwa
 
-UNIFORM, FFT, HALF_INC, HALF_TRN = 0, 1, 3, 4
+UNIFORM, FFT, ANALYTICAL, HALF_INC, HALF_TRN = 0, 1, 2, 3, 4
 
 
 class Expansion:
@@ -63,6 +63,13 @@ class Layer:
         return cls(e, UNIFORM, eps, depth)
 
     @classmethod
+    def analytical(cls, e, islands, eps_host, depth):
+        """khepri/layer.py:112-119."""
+        layer = cls(e, ANALYTICAL, islands, depth)
+        layer.eps_host = eps_host
+        return layer
+
+    @classmethod
     def half_infinite(cls, e, side, eps):
         return cls(e, HALF_INC if side == "reflexion" else HALF_TRN, eps, 0)
 
@@ -70,10 +77,14 @@ class Layer:
         kx, ky, _ = self.expansion.k_vectors(kp, wl)
         W0, V0 = rcwa.free_space_eigenmodes(kx, ky)
         k0 = 2 * np.pi / wl
-        if self.kind == FFT:
-            # the reference recomputes C every solve (khepri/layer.py:157)
+        if self.kind in (FFT, ANALYTICAL):
+            # the reference recomputes C every solve (khepri/layer.py:157, 161-166)
             if self._C is None or not reuse_convmat:
-                self._C = rcwa.convolution_matrix(self.epsilon, self.expansion.pw)
+                if self.kind == FFT:
+                    self._C = rcwa.convolution_matrix(self.epsilon, self.expansion.pw)
+                else:
+                    self._C = rcwa.analytic_convolution_matrix(self.epsilon, self.eps_host, self.expansion.pw,
+                                                               self.expansion.lattice)
             C = self._C
             self.IC = np.linalg.inv(C)
             self.W, self.V, self.L = rcwa.structured_layer_modes(kx, ky, C)
@@ -153,6 +164,10 @@ class Crystal:
 
     def add_layer_uniform(self, name, eps, depth):
         self.layers[name] = Layer.uniform(self.expansion, eps, depth)
+
+    def add_layer_analytical(self, name, islands, eps_host, depth):
+        """khepri/crystal.py:119-123."""
+        self.layers[name] = Layer.analytical(self.expansion, islands, eps_host, depth)
 
     def add_layer_pixmap(self, name, eps, depth):
         self.layers[name] = Layer.pixmap(self.expansion, eps, depth)

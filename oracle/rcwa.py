@@ -135,6 +135,97 @@ def convolution_matrix(eps2d, pw):
 
 
 # --------------------------------------------------------------------------
+# analytical islands (closed-form Fourier coefficients)
+# --------------------------------------------------------------------------
+def unit_cell_area(lattice):
+    """|a1 x a2|.  khepri/tools.py:58-60."""
+    a1, a2 = np.asarray(lattice)
+    return abs(a1[0] * a2[1] - a1[1] * a2[0])
+
+
+def fexpz(z):
+    """(e^z - 1)/z, with the reference's truncated series for |z| <= 1e-2.
+    khepri/fourier.py:7-20."""
+    z = np.asarray(z, dtype=complex)
+    out = np.empty_like(z)
+    big = np.abs(z) > 1e-2
+    out[big] = (np.exp(z[big]) - 1.0) / z[big]
+    w = z[~big]
+    out[~big] = 1 + w / 2.0 * (1 + w / 3.0 * (1 + w / 4.0 * (1 + w / 5.0 * (1 + w / 6.0 * (1 + w / 7)))))
+    return out
+
+
+def transform_rectangle(ll0, ll1, ur0, ur1, Gx, Gy, sigma):
+    """Fourier coefficient of the indicator of [ll0, ur0] x [ll1, ur1].
+    khepri/fourier.py:28-39."""
+    a, b = ur0 - ll0, ur1 - ll1
+    return a * b / sigma * fexpz(-1j * Gx * a) * fexpz(-1j * Gy * b) * np.exp(-1j * (ll0 * Gx + ll1 * Gy))
+
+
+def transform_disc(c0, c1, radius, Gx, Gy, sigma):
+    """Fourier coefficient of the indicator of a disc: pi r^2 / sigma *
+    2 J1(|G| r) / (|G| r) * e^{-i G.c}, pi r^2 / sigma at G = 0.
+    khepri/fourier.py:42-58 (scipy.special.jv)."""
+    from scipy import special
+    Gx, Gy = np.asarray(Gx, dtype=float), np.asarray(Gy, dtype=float)
+    norm = np.sqrt(Gx * Gx + Gy * Gy) * radius
+    zero = np.isclose(Gx, 0.0) & np.isclose(Gy, 0.0)
+    out = np.zeros_like(Gx, dtype=complex)
+    nz = ~zero
+    out[nz] = (np.pi * radius ** 2 / sigma * 2 * np.exp(-1j * (c0 * Gx[nz] + c1 * Gy[nz]))
+               * special.jv(1.0, norm[nz]) / norm[nz])
+    out[zero] = np.pi * radius ** 2 / sigma
+    return out
+
+
+def island_transform(island, Gx, Gy, sigma):
+    """khepri/fourier.py:23-25 (dispatch on the island type)."""
+    kind, p = island["type"], island["params"]
+    if kind == "disc":
+        return transform_disc(*p, Gx, Gy, sigma)
+    if kind == "rectangle":
+        return transform_rectangle(*p, Gx, Gy, sigma)
+    if kind == "uniform":
+        return np.zeros_like(np.asarray(Gx), dtype=complex)
+    raise NotImplementedError(kind)
+
+
+def combine_fourier_masks(islands_data, eps_host, inverse=False):
+    """eps_g = eps_host delta_0 + sum_i (eps_i - eps_host) T_i.
+    khepri/fourier.py:145-159."""
+    length = islands_data[0][0].shape[0]
+    center = (length - 1) // 2
+    trsf = (lambda x: x, lambda x: 1 / x)[inverse]
+    eps_g = np.zeros((length,), dtype=complex)
+    eps_g[center] = trsf(eps_host)
+    for bf, island_eps in islands_data:
+        eps_g += (trsf(island_eps) - trsf(eps_host)) * bf
+    return eps_g
+
+
+def expanded_pw(pw, mul=3):
+    """khepri/expansion.py:86-92 (the plane-wave counts of the expanded grid)."""
+    return [e * mul if e > 1 else 1 for e in pw]
+
+
+def analytic_convolution_matrix(islands, eps_host, pw, lattice):
+    """C of an island layer: the island coefficients on the 3x-expanded
+    harmonic grid (m fastest), the flat array reshaped to the expanded
+    (P', Q') shape and gathered like a pixmap's spectrum.  The reshape pairs
+    the flat index i*Q' + j with (row i, column j), which for P = Q swaps the
+    roles of the two harmonic indices; this is reproduced as it is.
+    khepri/layer.py:161-166, khepri/expansion.py:86-92, khepri/tools.py:38-56."""
+    epw = expanded_pw(pw)
+    Gx, Gy = g_vectors(epw, lattice)
+    sigma = unit_cell_area(lattice)
+    data = [(island_transform(isl, Gx, Gy, sigma), isl["epsilon"]) for isl in islands]
+    if not data:
+        data = [(np.zeros(Gx.shape, dtype=complex), eps_host)]
+    fourier = combine_fourier_masks(data, eps_host).T
+    return toeplitz_from_fourier(fourier.reshape(epw), pw)
+
+
+# --------------------------------------------------------------------------
 # modes of the layers
 # --------------------------------------------------------------------------
 def _kz_masked(arg):

@@ -248,9 +248,42 @@ def gen_incident():
     save("incident.npz", cases=np.array(cases, dtype=float), vectors=np.array(out))
 
 
+from analytic_cases import ANALYTIC_CASES, ANALYTIC_SOURCES, GAP  # noqa: E402
+
+
+def gen_analytic():
+    """Island layers with closed-form coefficients (khepri/layer.py:161-166):
+    the convolution matrix and end-to-end R/T of [A, gap, A] stacks, plus the
+    reference test's own cylinder fixture (test/integration/test_cylinders.py:36-46)."""
+    from scipy.io import loadmat
+    fx = os.path.join("/root/reference", "test", "integration", "fixtures", "cylinder")
+    gv, sg, fo = (loadmat(os.path.join(fx, f)) for f in ("gvectors.mat", "sigma.mat", "fourier.mat"))
+    out = dict(cyl_Gx=np.asarray(gv["Gx"], dtype=float), cyl_Gy=np.asarray(gv["Gy"], dtype=float),
+               cyl_sigma=float(np.asarray(sg["sigma"]).ravel()[0]),
+               cyl_omega=np.array([[complex(np.asarray(v).ravel()[0]) for v in row] for row in fo["Omega_g"]]),
+               cyl_disc=np.array([0.5, 0.5, 0.2]) * 100e-9)
+    for c, (pw, lattice, islands, eps_host, depth) in enumerate(ANALYTIC_CASES):
+        R, T = [], []
+        for wl, te, tm, th, ph in ANALYTIC_SOURCES:
+            cl = Crystal(pw, lattice=lattice)
+            cl.add_layer_analytical("A", islands, eps_host, depth)
+            cl.add_layer_uniform("G", *GAP)
+            cl.set_device(["A", "G", "A"], False)
+            cl.set_source(wl, te, tm, th, ph)
+            cl.solve()
+            r, t = cl.poynting_flux_end()
+            R.append(r)
+            T.append(t)
+        out[f"C{c}"] = np.asarray(cl.layers["A"].C)
+        out[f"R{c}"] = np.array(R)
+        out[f"T{c}"] = np.array(T)
+    out["sources"] = np.array(ANALYTIC_SOURCES, dtype=float)
+    save("analytic.npz", **out)
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["disc_sweep", "disc_bz", "random_devices", "grad_grid",
-                             "twisted_small", "fields_patterned", "incident", "twisted"]
+                             "twisted_small", "fields_patterned", "incident", "twisted", "analytic"]
     for w in which:
         t0 = time.time()
         globals()["gen_" + w]()

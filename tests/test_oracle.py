@@ -2,6 +2,8 @@
 test (Fresnel thin film), and closed forms chosen so that a dropped term, a
 wrong sign or a transposed index fails at least one of them."""
 import hashlib
+import os
+import sys
 
 import numpy as np
 import pytest
@@ -395,3 +397,69 @@ def test_golden_grad_grid_forward():
     cl.solve(forward_only=True)
     r, t = cl.poynting_flux_end()
     assert abs(r - g["R"][0]) < 1e-9 and abs(t - g["T"][0]) < 1e-9
+
+
+# ---------------------------------------------------------------- island layers
+def _analytic_fixture():
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "analytic.npz"))
+
+
+def test_disc_transform_matches_reference_cylinder_fixture():
+    """The reference test's own fixture (test/integration/test_cylinders.py:36-46):
+    disc at (0.5a, 0.5a), r = 0.2a on its G grid."""
+    g = _analytic_fixture()
+    c0, c1, r = g["cyl_disc"]
+    om = rcwa.transform_disc(c0, c1, r, g["cyl_Gx"], g["cyl_Gy"], float(g["cyl_sigma"]))
+    np.testing.assert_allclose(om, g["cyl_omega"], rtol=1e-7)
+
+
+def test_island_transforms_closed_forms():
+    lat = np.eye(2)
+    Gx, Gy = rcwa.g_vectors((7, 7), lat)
+    # G = 0: filling fraction
+    z = rcwa.transform_disc(0.1, 0.2, 0.3, np.zeros(1), np.zeros(1), 1.0)
+    assert abs(z[0] - np.pi * 0.09) < 1e-15
+    r0 = rcwa.transform_rectangle(-0.2, -0.1, 0.3, 0.2, np.zeros(1), np.zeros(1), 1.0)
+    assert abs(r0[0] - 0.5 * 0.3) < 1e-15
+    # a rectangle covering the whole cell is the delta at G = 0
+    full = rcwa.transform_rectangle(-0.5, -0.5, 0.5, 0.5, Gx, Gy, 1.0)
+    expect = np.where((np.abs(Gx) < 1e-9) & (np.abs(Gy) < 1e-9), 1.0, 0.0)
+    np.testing.assert_allclose(full, expect, atol=1e-14)
+    # translating an island multiplies its coefficients by e^{-i G.d}
+    d = np.array([0.13, -0.07])
+    t0 = rcwa.transform_disc(0.0, 0.0, 0.25, Gx, Gy, 1.0)
+    t1 = rcwa.transform_disc(d[0], d[1], 0.25, Gx, Gy, 1.0)
+    np.testing.assert_allclose(t1, t0 * np.exp(-1j * (Gx * d[0] + Gy * d[1])), atol=1e-15)
+    # the disc's coefficients against a fine rasterised disc's (pixel DFT, 2048^2)
+    n = 2048
+    x = (np.arange(n) + 0.5) / n - 0.5
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    mask = ((X - 0.1) ** 2 + (Y + 0.05) ** 2 < 0.3 ** 2).astype(float)
+    for (m, k) in [(0, 0), (1, 0), (0, 2), (-1, 1), (2, -3)]:
+        gx, gy = 2 * np.pi * m, 2 * np.pi * k
+        num = (mask * np.exp(-1j * (gx * X + gy * Y))).mean()
+        ana = rcwa.transform_disc(0.1, -0.05, 0.3, np.array([gx]), np.array([gy]), 1.0)[0]
+        assert abs(num - ana) < 2e-4, (m, k, num, ana)
+    # the fexpz series and the closed form agree where they meet
+    w = np.array([0.0099j, -0.0099, 0.0101j, 0.007 + 0.007j])
+    np.testing.assert_allclose(rcwa.fexpz(w), np.where(w == 0, 1, (np.exp(w) - 1) / w), rtol=1e-14)
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_analytic_layer_matches_reference(case):
+    """Island-layer convolution matrix and [A, gap, A] R/T against the reference."""
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from analytic_cases import ANALYTIC_CASES, ANALYTIC_SOURCES, GAP
+    g = _analytic_fixture()
+    pw, lattice, islands, eps_host, depth = ANALYTIC_CASES[case]
+    C = rcwa.analytic_convolution_matrix(islands, eps_host, pw, rcwa.lattice_matrix(lattice))
+    np.testing.assert_allclose(C, g[f"C{case}"], atol=1e-14)
+    for s, (wl, te, tm, th, ph) in enumerate(ANALYTIC_SOURCES):
+        cl = OC.Crystal(pw, lattice=lattice)
+        cl.add_layer_analytical("A", islands, eps_host, depth)
+        cl.add_layer_uniform("G", *GAP)
+        cl.set_device(["A", "G", "A"], False)
+        cl.set_source(wl, te, tm, th, ph)
+        cl.solve()
+        r, t = cl.poynting_flux_end()
+        assert abs(r - g[f"R{case}"][s]) < 1e-11 and abs(t - g[f"T{case}"][s]) < 1e-11
