@@ -48,6 +48,17 @@ int khx_kvectors(const double* g, int N, const double* kp, const double* wl, int
  * eps: (count, nx, ny) real (is_complex = 0) or complex (is_complex = 1),
  * indexed eps[x][y].  C: complex (count, N, N).  Fails (-1) when the pixmap
  * is too small for the harmonics, where the reference raises IndexError.   */
+/* Convolution matrix of an island layer (replaces the Formulation.ANALYTICAL
+ * branch of Layer.solve, khepri/layer.py:161-166, with the island transforms
+ * of khepri/fourier.py:7-58 and combine_fourier_masks :145-159).  Islands on
+ * the device: kinds[i] 0 = disc (params c0, c1, r, unused), 1 = rectangle
+ * (ll0, ll1, ur0, ur1), 2 = uniform (no term); eps complex per island;
+ * eps_host complex (host pointer, 2 doubles); recip = b1x, b1y, b2x, b2y
+ * (host, 4 doubles); sigma = unit-cell area.  C (P Q x P Q complex) on the
+ * device; ws = khx_convmat_ws_bytes(P, Q, 1) bytes of device memory.      */
+int khx_convmat_islands(const int* kinds, const double* params, const double* eps, int nisl,
+                        const double* eps_host, const double* recip, double sigma, int P, int Q, double* C, void* ws,
+                        void* stream);
 size_t khx_convmat_ws_bytes(int P, int Q, int count);
 int khx_convmat(const double* eps, int is_complex, int nx, int ny, int P, int Q, int count, double* C,
                 void* ws, void* stream);

@@ -67,6 +67,7 @@ SIGNATURES = {
                                      _vp, _vp]),
     "khx_redheffer_bwd_ws_bytes": (_sz, [_i, _i]),
     "khx_redheffer_bwd": (_i, [_dp, _dp, _i, _i, _dp, _dp, _dp, _ip, _vp, _vp]),
+    "khx_convmat_islands": (_i, [_vp, _vp, _vp, _i, _vp, _vp, C.c_double, _i, _i, _vp, _vp, _vp]),
     "khx_eig_slots": (_i, [_i]),
     "khx_gemm_record": (_i, [_i, _vp, _i]),
     "khx_gemm_force": (_i, [_i]),

@@ -126,7 +126,7 @@ def flux_gradient(crystal, wT=1.0, wR=0.0, wrt=None, reduce=False, eta_rel=1e-12
 def _layers_forward(cl, names, kx, ky, k0):
     """_layer_forward for every named layer; the patterned ones in one fused
     call when their eigenproblems would not fill the GPU on their own."""
-    pat = [nm for nm in names if cl.layers[nm].formulation == Formulation.FFT]
+    pat = [nm for nm in names if cl.layers[nm].patterned]
     B, N = kx.shape
     out = {}
     if len(pat) >= 2 and B * len(pat) <= ops.eig_slots(2 * N) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1":
@@ -146,7 +146,7 @@ def _layer_forward(layer, kx, ky, k0):
     """("identity", None) | ("dense", S, extras) for one layer over a batch."""
     k0d = (k0 * layer.depth).contiguous()
     f = layer.formulation
-    if f == Formulation.FFT:
+    if f in (Formulation.FFT, Formulation.ANALYTICAL):
         C, IC = layer.convolution_matrices(kx.device)
         S, (W, V, lam), info = ops.patterned_layer(C, IC, kx, ky, k0d, modes=True)
         ops.raise_on_info(info, "patterned layer")

@@ -93,7 +93,7 @@ class CrystalBatch:
         wl = torch.as_tensor(np.array([self.crystals[i]._wl[0] for i, _ in inst]), device=dev)
         S_all = torch.empty((M, 2, 2, n, n), dtype=ops.CDT, device=dev)
         # ---- patterned instances: per-item convolution matrices, chunked
-        pat = [k for k, (_, L) in enumerate(inst) if L.formulation == Formulation.FFT]
+        pat = [k for k, (_, L) in enumerate(inst) if L.formulation in (Formulation.FFT, Formulation.ANALYTICAL)]
         for c0 in range(0, len(pat), self.chunk):
             ks = pat[c0:c0 + self.chunk]
             C, IC = self._convmats([inst[k][1] for k in ks], e.pw, dev)
@@ -155,11 +155,15 @@ class CrystalBatch:
 
     @staticmethod
     def _convmats(layers, pw, dev):
-        """(count, N, N) C and C^-1 of the pixmap layers (one convmat launch per pixmap shape)."""
+        """(count, N, N) C and C^-1 of the pixmap layers (one convmat launch per pixmap shape) and
+        island layers."""
         N = pw[0] * pw[1]
         C = torch.empty((len(layers), N, N), dtype=ops.CDT, device=dev)
         by_shape = defaultdict(list)
         for k, L in enumerate(layers):
+            if L.formulation == Formulation.ANALYTICAL:   # closed-form island coefficients
+                C[k] = L.convolution_matrices(dev)[0]
+                continue
             eps = np.asarray(L.epsilon)
             by_shape[(eps.shape, np.iscomplexobj(eps))].append(k)
         for (shape, cplx), ks in by_shape.items():

@@ -19,7 +19,7 @@ from . import chain, ops
 from .device import default_device
 from .expansion import Expansion, compute_kplanar
 from .extension import ExtendedLayer
-from .layer import Formulation, Layer, stack_layers
+from .layer import Layer, stack_layers
 
 
 def _bytes_per_item(n, unique, length):
@@ -176,7 +176,7 @@ class Crystal:
         return chunk
 
     def _has_patterned(self):
-        return any(getattr(self.layers[name], "formulation", None) == Formulation.FFT
+        return any(getattr(self.layers[name], "patterned", False)
                    for name in set(self.global_stacking) if name in self.layers)
 
     def solve(self, rt_only=None):

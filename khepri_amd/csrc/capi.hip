@@ -85,6 +85,16 @@ int khx_convmat(const double* eps, int is_complex, int nx, int ny, int P, int Q,
   return khx::convmat(as_stream(stream), eps, is_complex, nx, ny, P, Q, count, as_cd(C), static_cast<cd*>(ws));
 }
 
+int khx_convmat_islands(const int* kinds, const double* params, const double* eps, int nisl,
+                        const double* eps_host, const double* recip, double sigma, int P, int Q, double* C, void* ws,
+                        void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(eps_host && recip && C && ws && (nisl == 0 || (kinds && params && eps)),
+              "khx_convmat_islands: null argument");
+  return khx::convmat_islands(as_stream(stream), kinds, params, reinterpret_cast<const cd*>(eps), nisl,
+                              mkc(eps_host[0], eps_host[1]), recip, sigma, P, Q, as_cd(C), static_cast<cd*>(ws));
+}
+
 int khx_zgesv(int n, int nrhs, double* A, double* B, int batch, int* info, void* stream) {
   KHX_ENTRY_BEGIN
   KHX_REQUIRE(n > 0 && nrhs > 0 && batch >= 0, "khx_zgesv: bad sizes");

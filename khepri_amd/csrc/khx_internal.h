@@ -40,6 +40,9 @@ bool lu_multi_fits(int n);
 int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL);
 // LU path selection: 0 = unblocked, 1 = one-workgroup blocked, 2 = multi-launch blocked
 int lu_path(int n);
+// island layers: closed-form coefficients gathered into C (Fws: (2P-1)(2Q-1) complex)
+int convmat_islands(hipStream_t s, const int* kinds, const double* params, const cd* eps, int nisl, cd eps_host,
+                    const double* recip, double sigma, int P, int Q, cd* C, cd* Fws);
 // B <- A^-1 B from zgetrf factors: interchanges, then blocked triangular solves
 int zlu_solve_factored(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch);
 int zlu_blocked(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL);

@@ -127,6 +127,60 @@ __global__ __launch_bounds__(256) void fourier_coef_sep_kernel(const double* __r
   }
 }
 
+// ---- island layers: closed-form Fourier coefficients (khepri/fourier.py:7-58,
+// 145-159; khepri/layer.py:161-166).  The coefficients live on the
+// 3x-expanded harmonic grid (P' = 3P, Q' = 3Q, or 1 for a count of 1; m
+// fastest); the reference reshapes that flat array to (P', Q') and gathers
+// element [g0x + dp][g0y + dq] for harmonic differences (dp, dq), i.e. flat
+// index (g0x + dp) Q' + (g0y + dq).  One thread evaluates one entry of the
+// (2P-1) x (2Q-1) table that toeplitz_kernel gathers from.
+__device__ __forceinline__ cd fexpz_d(cd z) {
+  if (absc(z) > 1e-2) return divc(expc(z) - mkc(1.0, 0.0), z);
+  // (e^z - 1)/z = 1 + z/2 (1 + z/3 (1 + z/4 (1 + z/5 (1 + z/6 (1 + z/7)))))
+  cd t = mkc(1.0, 0.0) + (1.0 / 7.0) * z;
+  t = mkc(1.0, 0.0) + (1.0 / 6.0) * (z * t);
+  t = mkc(1.0, 0.0) + (1.0 / 5.0) * (z * t);
+  t = mkc(1.0, 0.0) + (1.0 / 4.0) * (z * t);
+  t = mkc(1.0, 0.0) + (1.0 / 3.0) * (z * t);
+  return mkc(1.0, 0.0) + 0.5 * (z * t);
+}
+
+__global__ void island_coef_kernel(const int* __restrict__ kinds, const double* __restrict__ params,
+                                   const cd* __restrict__ eps, int nisl, cd eps_host, double b1x, double b1y,
+                                   double b2x, double b2y, double sigma, int P, int Q, cd* __restrict__ F) {
+  const int ncu = 2 * P - 1, ncv = 2 * Q - 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncu * ncv) return;
+  const int dp = t / ncv - (P - 1), dq = t % ncv - (Q - 1);
+  const int e0 = P > 1 ? 3 * P : 1, e1 = Q > 1 ? 3 * Q : 1;
+  const int flat = (e0 / 2 + dp) * e1 + (e1 / 2 + dq);
+  const int m = flat % e0 - e0 / 2, n = flat / e0 - e1 / 2;
+  const double gx = m * b1x + n * b2x, gy = m * b1y + n * b2y;
+  cd acc = (flat == (e0 * e1 - 1) / 2) ? eps_host : mkc(0.0, 0.0);
+  for (int i = 0; i < nisl; ++i) {
+    const double* pr = params + 4 * i;
+    cd tr = mkc(0.0, 0.0);
+    if (kinds[i] == 0) {  // disc (c0, c1, r)
+      const double r = pr[2];
+      if (fabs(gx) <= 1e-8 && fabs(gy) <= 1e-8) {
+        tr = mkc(M_PI * r * r / sigma, 0.0);
+      } else {
+        const double nrm = sqrt(gx * gx + gy * gy) * r;
+        double sn, cs;
+        sincos(-(pr[0] * gx + pr[1] * gy), &sn, &cs);
+        tr = (M_PI * r * r / sigma * 2.0 * j1(nrm) / nrm) * mkc(cs, sn);
+      }
+    } else if (kinds[i] == 1) {  // rectangle (ll0, ll1, ur0, ur1)
+      const double a = pr[2] - pr[0], b = pr[3] - pr[1];
+      double sn, cs;
+      sincos(-(pr[0] * gx + pr[1] * gy), &sn, &cs);
+      tr = (a * b / sigma) * (fexpz_d(mkc(0.0, -gx * a)) * fexpz_d(mkc(0.0, -gy * b))) * mkc(cs, sn);
+    }
+    acc = acc + (eps[i] - eps_host) * tr;
+  }
+  F[t] = acc;
+}
+
 // C[q*P+p][q'*P+p'] = eps_hat[p-p'][q-q']  (khepri/tools.py:38-56)
 __global__ void toeplitz_kernel(const cd* __restrict__ F, int P, int Q, cd* __restrict__ C) {
   const int N = P * Q, ncv = 2 * Q - 1;
@@ -138,6 +192,20 @@ __global__ void toeplitz_kernel(const cd* __restrict__ F, int P, int Q, cd* __re
     const int dp = r % P - col % P, dq = r / P - col / P;
     c[e] = f[(dp + P - 1) * ncv + (dq + Q - 1)];
   }
+}
+
+int convmat_islands(hipStream_t s, const int* kinds, const double* params, const cd* eps, int nisl, cd eps_host,
+                    const double* recip, double sigma, int P, int Q, cd* C, cd* Fws) {
+  KHX_REQUIRE(P >= 1 && Q >= 1 && P % 2 == 1 && Q % 2 == 1, "convmat_islands: plane-wave counts must be odd");
+  KHX_REQUIRE(nisl >= 0 && sigma > 0.0, "convmat_islands: bad island count or cell area");
+  const int ncoef = (2 * P - 1) * (2 * Q - 1);
+  island_coef_kernel<<<dim3((ncoef + 255) / 256), 256, 0, s>>>(kinds, params, eps, nisl, eps_host, recip[0], recip[1],
+                                                               recip[2], recip[3], sigma, P, Q, Fws);
+  KHX_LAUNCH_CHECK();
+  const int N = P * Q;
+  toeplitz_kernel<<<dim3((N * N + 255) / 256, 1), 256, 0, s>>>(Fws, P, Q, C);
+  KHX_LAUNCH_CHECK();
+  return 0;
 }
 
 int convmat(hipStream_t s, const double* eps, int cplx, int nx, int ny, int P, int Q, int count, cd* C,

@@ -112,12 +112,34 @@ class Layer:
 
     @classmethod
     def analytical(cls, expansion, islands_description, eps_host, depth):
-        raise NotImplementedError("analytical island layers are not part of the GPU path; "
-                                  "rasterise the islands into a pixmap (Layer.pixmap)")
+        """Island layer with closed-form Fourier coefficients
+        (khepri/layer.py:112-119): islands_description is a list of
+        {"type": "disc" | "rectangle", "params": [...], "epsilon": eps}."""
+        layer = cls()
+        layer.expansion = expansion
+        layer.formulation = Formulation.ANALYTICAL
+        layer.epsilon = islands_description
+        layer.eps_host = eps_host
+        layer.depth = depth
+        return layer
+
+    @property
+    def patterned(self):
+        """True when the layer's modes come from the eigen-solver."""
+        return self.formulation in (Formulation.FFT, Formulation.ANALYTICAL)
 
     # ------------------------------------------------------------- solve
     def convolution_matrices(self, device):
-        """C and C^-1 of a pixmap layer, computed once per pixmap content."""
+        """C and C^-1 of a pixmap or island layer, computed once per content."""
+        if self.formulation == Formulation.ANALYTICAL:
+            key = (str(device), repr(self.epsilon), complex(self.eps_host))
+            if self._cm_key != key:
+                e = self.expansion
+                C = ops.convmat_islands(self.epsilon, self.eps_host, e.pw, e.reciprocal, e.sigma, device)
+                IC, info = ops.inverse(C[None])
+                ops.raise_on_info(info, "inverse of the convolution matrix")
+                self._C, self._IC, self._cm_key = C, IC[0], key
+            return self._C, self._IC
         eps = np.asarray(self.epsilon)
         key = (str(device), eps.shape, eps.dtype.str, hash(eps.tobytes()))
         if self._cm_key != key:
@@ -148,7 +170,7 @@ class Layer:
         share the GPU (a single layer's small batch leaves most CUs idle); the
         bound also keeps the fused workspace within one round's worth of
         items.  Everything else as solve_batch."""
-        pat = [ly for ly in layers if getattr(ly, "formulation", None) == Formulation.FFT]
+        pat = [ly for ly in layers if getattr(ly, "patterned", False)]
         B = kp.shape[0]
         fuse = len(pat) >= 2 and B * len(pat) <= ops.eig_slots(2 * pat[0].expansion.N) and \
             all(ly.expansion is pat[0].expansion for ly in pat) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1"
@@ -172,7 +194,7 @@ class Layer:
             ly.W, ly.V, ly.L = (W, V, L) if keep else (None, None, None)
             ly.kx, ly.ky = (kx, ky) if keep else (None, None)
         for ly in layers:
-            if getattr(ly, "formulation", None) != Formulation.FFT:
+            if not getattr(ly, "patterned", False):
                 ly.solve_batch(kp, wl, rt_only=rt_only)
 
     def solve_batch(self, kp, wl, ws=None, rt_only=False):
@@ -200,7 +222,7 @@ class Layer:
                 self.skind, self.S = "identity", None
                 self.W = self.V = self.L = self.IC = None
                 return
-        if self.formulation == Formulation.FFT:
+        if self.patterned:
             C, IC = self.convolution_matrices(dev)
             k0d = (k0 * self.depth).contiguous()
             self.S, (W, V, L), self.info = ops.patterned_layer(C, IC, kx, ky, k0d, modes=modes, ws=ws)

@@ -284,6 +284,38 @@ def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None):
     return S, (W, V, lam), info
 
 
+_ISLAND_KINDS = {"disc": 0, "rectangle": 1, "uniform": 2}
+
+
+def convmat_islands(islands, eps_host, pw, reciprocal, sigma, device):
+    """(N, N) convolution matrix of an island layer: islands = [{"type":
+    "disc" | "rectangle" | "uniform", "params": [...], "epsilon": eps}, ...]
+    as in khepri/layer.py:112-119, 161-166."""
+    lib = _lib.load()
+    P, Q = pw
+    nisl = len(islands)
+    kinds = np.zeros(max(nisl, 1), dtype=np.int32)
+    params = np.zeros((max(nisl, 1), 4), dtype=np.float64)
+    eps = np.zeros(max(nisl, 1), dtype=np.complex128)
+    for i, isl in enumerate(islands):
+        if isl["type"] not in _ISLAND_KINDS:
+            raise NotImplementedError(f"island type {isl['type']!r}")
+        kinds[i] = _ISLAND_KINDS[isl["type"]]
+        pr = list(isl.get("params", []))
+        params[i, :len(pr)] = pr
+        eps[i] = complex(isl["epsilon"])
+    kd = torch.as_tensor(kinds, device=device)
+    pd = torch.as_tensor(params, device=device)
+    ed = torch.as_tensor(eps, device=device)
+    C = torch.empty((P * Q, P * Q), dtype=CDT, device=device)
+    ws = _ws(lib.khx_convmat_ws_bytes(P, Q, 1), device)
+    eh = np.array([complex(eps_host).real, complex(eps_host).imag])
+    rec = np.ascontiguousarray(np.asarray(reciprocal, dtype=np.float64).reshape(4))
+    check(lib.khx_convmat_islands(_p(kd), _p(pd), _p(ed), nisl, eh.ctypes.data, rec.ctypes.data, float(sigma), P, Q,
+                                  _p(C), _p(ws), _stream()), "khx_convmat_islands")
+    return C
+
+
 def patterned_layers_fused(Cs, ICs, kx, ky, k0ds, modes=False):
     """patterned_layer for several layers over the same sources in one call:
     item l*B + b is layer l at source b, so the eigen-solver and every other

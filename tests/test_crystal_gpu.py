@@ -203,3 +203,81 @@ def test_rt_only_random_devices_orders(gpu):
     assert abs(r - g["R"][k]) <= TOL and abs(t - g["T"][k]) <= TOL
     np.testing.assert_allclose(ro, g["R_orders"], atol=1e-10)
     np.testing.assert_allclose(to, g["T_orders"], atol=1e-10)
+
+
+# ---------------------------------------------------------------- island layers
+def _analytic_cases():
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from analytic_cases import ANALYTIC_CASES, ANALYTIC_SOURCES, GAP
+    return ANALYTIC_CASES, ANALYTIC_SOURCES, GAP
+
+
+def test_island_convmat_matches_oracle(gpu):
+    from oracle import rcwa
+    from khepri_amd import ops
+    from khepri_amd.expansion import Expansion
+    cases, _, _ = _analytic_cases()
+    extra = [((7, 3), "square", [{"type": "rectangle", "params": [-0.5, -0.5, 0.5, 0.5], "epsilon": 9.0}], 1.0, 0.1),
+             ((5, 5), "square", [], 2.5, 0.1),
+             ((1, 5), "square", [{"type": "uniform", "params": [], "epsilon": 3.0},
+                                 {"type": "disc", "params": [0.3, 0.0, 0.45], "epsilon": 1.0 + 2j}], 4.0, 0.1)]
+    for pw, lattice, islands, eps_host, _ in list(cases) + extra:
+        lat = rcwa.lattice_matrix(lattice)
+        e = Expansion(pw, lat)
+        C = ops.convmat_islands(islands, eps_host, pw, e.reciprocal, e.sigma, gpu).cpu().numpy()
+        ref = rcwa.analytic_convolution_matrix(islands, eps_host, pw, lat)
+        np.testing.assert_allclose(C, ref, atol=1e-13 * max(1.0, np.abs(ref).max()))
+
+
+def test_island_layers_golden(gpu):
+    """Island-layer stacks [A, gap, A] against the reference's R/T."""
+    from khepri_amd import Crystal
+    g = golden("analytic.npz")
+    cases, sources, gap = _analytic_cases()
+    for c, (pw, lattice, islands, eps_host, depth) in enumerate(cases):
+        for s, (wl, te, tm, th, ph) in enumerate(sources):
+            cl = Crystal(pw, lattice=lattice)
+            cl.add_layer_analytical("A", islands, eps_host, depth)
+            cl.add_layer_uniform("G", *gap)
+            cl.set_device(["A", "G", "A"], False)
+            cl.set_source(wl, te, tm, th, ph)
+            cl.solve()
+            r, t = cl.poynting_flux_end()
+            assert abs(r - g[f"R{c}"][s]) <= TOL and abs(t - g[f"T{c}"][s]) <= TOL, (c, s, r, t)
+        np.testing.assert_allclose(cl.layers["A"].convolution_matrices(gpu)[0].cpu().numpy(), g[f"C{c}"],
+                                   atol=1e-13)
+
+
+def test_island_layers_batched_and_mixed(gpu):
+    """A batch of sources through an island + pixmap stack equals per-source
+    solves, and CrystalBatch handles island layers."""
+    from khepri_amd import Crystal
+    from khepri_amd.batch import CrystalBatch
+    cases, _, gap = _analytic_cases()
+    pw, lattice, islands, eps_host, depth = cases[0]
+    pix = W.disc_pixmap()
+    wls = np.array([1.5, 1.8, 2.1, 2.4])
+
+    def make(wl):
+        cl = Crystal(pw, lattice=lattice)
+        cl.add_layer_analytical("A", islands, eps_host, depth)
+        cl.add_layer_pixmap("P", pix, 0.3)
+        cl.add_layer_uniform("G", *gap)
+        cl.set_device(["A", "G", "P"], False)
+        cl.set_source(wl, 1.0, 0.0, 5.0, 10.0)
+        return cl
+    cl = make(wls)
+    cl.solve()
+    R, T = cl.poynting_flux_end()
+    singles = []
+    for k, wl in enumerate(wls):
+        c1 = make(wl)
+        c1.solve()
+        r, t = c1.poynting_flux_end()
+        singles.append((r, t))
+        assert abs(r - R[k]) < 1e-11 and abs(t - T[k]) < 1e-11
+    Rb, Tb = CrystalBatch([make(wl) for wl in wls]).solve()
+    np.testing.assert_allclose(Rb, [s[0] for s in singles], atol=1e-11)
+    np.testing.assert_allclose(Tb, [s[1] for s in singles], atol=1e-11)
