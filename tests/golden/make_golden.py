@@ -281,9 +281,39 @@ def gen_analytic():
     save("analytic.npz", **out)
 
 
+def gen_beams():
+    """Beam / Brillouin-zone front end (khepri/beams.py): rotation matrices,
+    rotated and shifted paraxial Gaussian and Laguerre-Gauss fields on a grid,
+    and their Fourier amplitudes over 2 x 2 cells."""
+    from functools import partial
+    import contextlib
+    import io
+    from khepri import beams
+    angles = [(0.0, 0.0, 0.0), (0.3, 0.0, 0.0), (0.2, 1.1, 0.4), (1.0, -0.7, 2.0)]
+    rots = np.array([beams.rotation_matrix(*a) for a in angles])
+    NS, bzs, a = 6, (2, 2), 1.0
+    xv = np.linspace(0.0, a * bzs[0], NS * bzs[0])
+    yv = np.linspace(0.0, a * bzs[1], NS * bzs[1])
+    X, Y = np.meshgrid(xv, yv)
+    Z = np.zeros_like(X) + 0.1
+    out = dict(rotations=rots, angles=np.array(angles), X=X, Y=Y, Z=Z)
+    cases = [("gauss", partial(beams._paraxial_gaussian_field_fn, beam_waist=0.7), (1.0, 1.0, -0.5, 0.2, 0.3, 0.1)),
+             ("lg", partial(beams._paraxial_laguerre_gaussian_field_fn, w0=0.8, p=1, l=2),
+              (0.9, 1.1, -0.3, 0.0, 0.0, 0.5))]
+    e = Expansion((3, 3))
+    for name, fn, (ox, oy, oz, pol, az, pz) in cases:
+        with contextlib.redirect_stdout(io.StringIO()):   # the reference prints array shapes here
+            f = np.asarray(beams.shifted_rotated_fields(fn, X, Y, Z, 0.63, ox, oy, oz, pol, az, pz))
+        out[f"{name}_fields"] = f
+        fr = np.swapaxes(np.swapaxes(f[..., 0], 0, 2), 1, 3)   # (rows, cols, 2, 3), cf. examples/bzi/vortex.py:61-63
+        out[f"{name}_amps"] = beams.amplitudes_from_fields(fr, e, 0.63, (0.4, -0.2), X, Y, bzs)
+    out["g_vectors"] = np.asarray(e.g_vectors)
+    save("beams.npz", **out)
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["disc_sweep", "disc_bz", "random_devices", "grad_grid",
-                             "twisted_small", "fields_patterned", "incident", "twisted", "analytic"]
+                             "twisted_small", "fields_patterned", "incident", "twisted", "analytic", "beams"]
     for w in which:
         t0 = time.time()
         globals()["gen_" + w]()

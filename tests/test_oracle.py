@@ -463,3 +463,51 @@ def test_analytic_layer_matches_reference(case):
         cl.solve()
         r, t = cl.poynting_flux_end()
         assert abs(r - g[f"R{case}"][s]) < 1e-11 and abs(t - g[f"T{case}"][s]) < 1e-11
+
+
+# ---------------------------------------------------------------- beam front end
+def test_beam_oracle_matches_reference_golden():
+    from functools import partial
+    from oracle import beams as OB
+    g = golden("beams.npz")
+    for ang, R in zip(g["angles"], g["rotations"]):
+        np.testing.assert_allclose(OB.rotation_matrix(*ang), R, atol=1e-15)
+    X, Y, Z = g["X"], g["Y"], g["Z"]
+    cases = [("gauss", partial(OB.paraxial_gaussian_field_fn, beam_waist=0.7), (1.0, 1.0, -0.5, 0.2, 0.3, 0.1)),
+             ("lg", partial(OB.paraxial_laguerre_gaussian_field_fn, w0=0.8, p=1, l=2), (0.9, 1.1, -0.3, 0.0, 0.0, 0.5))]
+    for name, fn, (ox, oy, oz, pol, az, pz) in cases:
+        f = OB.shifted_rotated_fields(fn, X, Y, Z, 0.63, ox, oy, oz, pol, az, pz)
+        assert f.shape == g[f"{name}_fields"].shape
+        np.testing.assert_allclose(f, g[f"{name}_fields"], atol=1e-13)
+        fr = np.swapaxes(np.swapaxes(f[..., 0], 0, 2), 1, 3)
+        amps = OB.amplitudes_from_fields(fr, g["g_vectors"], (3, 3), (0.4, -0.2), X, Y, (2, 2))
+        np.testing.assert_allclose(amps, g[f"{name}_amps"], atol=1e-13)
+
+
+def test_beam_oracle_closed_forms():
+    from oracle import beams as OB
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        R = OB.rotation_matrix(*rng.uniform(-3, 3, size=3))
+        np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-14)
+        assert abs(np.linalg.det(R) - 1) < 1e-14
+    # polar angle t rotates z towards x; the polarisation turns about the beam axis
+    t = 0.4
+    np.testing.assert_allclose(OB.rotation_matrix(t, 0, 0) @ [0, 0, 1], [np.sin(t), 0, np.cos(t)], atol=1e-15)
+    np.testing.assert_allclose(OB.rotation_matrix(0, 0, t) @ [1, 0, 0], [np.cos(t), np.sin(t), 0], atol=1e-15)
+    # Gaussian beam at its waist plane: real Gaussian profile, H = E / sqrt(er)
+    x = np.linspace(-1, 1, 9)
+    (ex, _, _), (_, hy, _) = OB.paraxial_gaussian_field_fn(x, 0 * x, 0 * x, 0.63, beam_waist=0.5, er=4.0)
+    np.testing.assert_allclose(ex, np.exp(-x ** 2 / 0.25), atol=1e-15)
+    np.testing.assert_allclose(hy, ex / 2, atol=1e-15)
+    # LG with p = l = 0 is the Gaussian beam
+    X, Y = np.meshgrid(x, x)
+    Zs = 0.3 + 0 * X
+    (eg, _, _), _ = OB.paraxial_gaussian_field_fn(X, Y, Zs, 0.63, beam_waist=0.5)
+    (el, _, _), _ = OB.paraxial_laguerre_gaussian_field_fn(X, Y, Zs, 0.63, w0=0.5, p=0, l=0)
+    np.testing.assert_allclose(el, eg, atol=1e-15)
+    # the point DFT of one plane wave picks its own harmonic
+    xs, ys = rng.uniform(0, 1, 50), rng.uniform(0, 1, 50)
+    f = np.exp(1j * (2 * np.pi * xs - 4 * np.pi * ys))
+    v = OB.slow_dft(f, xs, ys, np.array([2 * np.pi, 0.0]), np.array([-4 * np.pi, 0.0]))
+    assert abs(v[0] - 50) < 1e-12
