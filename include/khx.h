@@ -48,6 +48,22 @@ int khx_kvectors(const double* g, int N, const double* kp, const double* wl, int
  * eps: (count, nx, ny) real (is_complex = 0) or complex (is_complex = 1),
  * indexed eps[x][y].  C: complex (count, N, N).  Fails (-1) when the pixmap
  * is too small for the harmonics, where the reference raises IndexError.   */
+/* Beam front end (khepri/beams.py).  khx_beam_fields: E, H (each 3 x npts
+ * complex, component-major) of an x-polarised paraxial beam, kind 0 =
+ * Gaussian (w0 = beam waist), 1 = Laguerre-Gauss LG_pl, evaluated at the
+ * device points (x, y, z) taken to the beam frame; geom (device, 21 doubles)
+ * = rotation matrix M (row-major), M^-1, M^-1 o (o = beam origin); replaces
+ * shifted_rotated_fields with _paraxial_gaussian_field_fn /
+ * _paraxial_laguerre_gaussian_field_fn (khepri/beams.py:50-160).
+ * khx_points_dft: out[b][k] = scale sum_p f[b][p] exp(-i (kx[k] x[b][p] +
+ * ky[k] y[b][p])) for batch point sets of npts points (slow_dft,
+ * khepri/fourier.py:94-104, as used by amplitudes_from_fields,
+ * khepri/beams.py:164-190).                                                 */
+int khx_beam_fields(int kind, const double* x, const double* y, const double* z, long long npts, double wl,
+                    double w0, double er, int p, int l, const double* geom, double* E, double* H, void* stream);
+int khx_points_dft(const double* f, const double* x, const double* y, long long npts, const double* kx,
+                   const double* ky, int nk, int batch, double scale, double* out, void* stream);
+
 /* Convolution matrix of an island layer (replaces the Formulation.ANALYTICAL
  * branch of Layer.solve, khepri/layer.py:161-166, with the island transforms
  * of khepri/fourier.py:7-58 and combine_fourier_masks :145-159).  Islands on

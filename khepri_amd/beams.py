@@ -1,0 +1,165 @@
+"""Beam / Brillouin-zone integration front end (khepri/beams.py) on the GPU.
+
+A finite beam is decomposed over Bloch waves: its fields are sampled on a
+grid covering several unit cells (``shifted_rotated_fields``), their
+Fourier amplitudes per Bloch vector are taken (``amplitudes_from_fields``)
+and fed to ``Crystal.fields_volume(incident_fields=...)`` for every k-point
+of the Brillouin-zone grid (``gen_bzi_grid``).
+
+Field evaluation of the built-in paraxial beams (Gaussian, Laguerre-Gauss)
+and the point DFTs run in libkhx (``khx_beam_fields``, ``khx_points_dft``);
+the 3 x 3 beam rotation and the grid bookkeeping are host geometry.  A field
+function written by the caller (any other callable) is evaluated by that
+callable on host arrays, as in the reference, and only rotated here.
+Signatures, argument meaning and output shapes follow the reference,
+including the trailing length-1 axis of ``shifted_rotated_fields``.
+"""
+from __future__ import annotations
+
+import functools
+from math import prod
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+from .device import default_device
+
+
+def rotation_matrix(polar_angle, azimuthal_angle, polarization_angle):
+    """Beam frame -> lab frame: rotation by the polarization angle about the
+    beam axis after R_z(azimuthal) R_y(polar).  khepri/beams.py:6-47."""
+    cp, sp = np.cos(polar_angle), np.sin(polar_angle)
+    ca, sa = np.cos(azimuthal_angle), np.sin(azimuthal_angle)
+    cq, sq = np.cos(polarization_angle), np.sin(polarization_angle)
+    ry = np.array([[cp, 0.0, sp], [0.0, 1.0, 0.0], [-sp, 0.0, cp]])
+    rz = np.array([[ca, -sa, 0.0], [sa, ca, 0.0], [0.0, 0.0, 1.0]])
+    u = np.array([ca * sp, sa * sp, cp])                  # beam axis
+    cross = np.array([[0.0, -u[2], u[1]], [u[2], 0.0, -u[0]], [-u[1], u[0], 0.0]])
+    rp = cq * np.eye(3) + (1 - cq) * np.outer(u, u) + sq * cross   # Rodrigues' formula
+    return rp @ rz @ ry
+
+
+def gen_bzi_grid(shape, a=1, reciproc=None):
+    """Cell-centred Bloch vectors, (2, *shape).  khepri/beams.py:193-207."""
+    if reciproc is None:
+        b1, b2 = [2 * np.pi / a, 0], [0, 2 * np.pi / a]
+    else:
+        b1, b2 = reciproc
+    si, sj = 1 / shape[0], 1 / shape[1]
+    i, j = np.meshgrid(np.arange(-0.5 + si / 2, 0.5, si), np.arange(-0.5 + sj / 2, 0.5, sj), indexing="ij")
+    return np.stack([b1[0] * i + b2[0] * j, b1[1] * i + b2[1] * j])
+
+
+def _device():
+    return default_device()
+
+
+def _beam(kind, x, y, z, wl, geom, w0, er=1, p=0, l=0):
+    """Run khx_beam_fields on host arrays; returns E, H as (3, *shape) numpy."""
+    lib = _lib.load()
+    dev = _device()
+    x, y, z = np.broadcast_arrays(np.asarray(x, dtype=float), np.asarray(y, dtype=float),
+                                  np.asarray(z, dtype=float))
+    shape = x.shape
+    xd, yd, zd = (torch.as_tensor(np.ascontiguousarray(v).ravel(), device=dev) for v in (x, y, z))
+    n = xd.numel()
+    g = torch.as_tensor(np.ascontiguousarray(geom, dtype=np.float64), device=dev)
+    E = torch.empty((3, n), dtype=torch.complex128, device=dev)
+    H = torch.empty_like(E)
+    check(lib.khx_beam_fields(kind, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n, float(wl), float(w0),
+                              float(er), int(p), int(l), g.data_ptr(), E.data_ptr(), H.data_ptr(),
+                              torch.cuda.current_stream().cuda_stream), "khx_beam_fields")
+    return E.cpu().numpy().reshape(3, *shape), H.cpu().numpy().reshape(3, *shape)
+
+
+_IDENTITY_GEOM = np.concatenate([np.eye(3).ravel(), np.eye(3).ravel(), np.zeros(3)])
+
+
+def _paraxial_gaussian_field_fn(x, y, z, wl, beam_waist=1, er=1):
+    """x-polarised paraxial Gaussian beam in its own frame.  khepri/beams.py:135-160."""
+    E, H = _beam(0, x, y, z, wl, _IDENTITY_GEOM, beam_waist, er)
+    return (E[0], E[1], E[2]), (H[0], H[1], H[2])
+
+
+def _paraxial_laguerre_gaussian_field_fn(x, y, z, wl, w0=1, er=1, p=0, l=0):
+    """x-polarised paraxial Laguerre-Gauss beam LG_pl in its own frame.  khepri/beams.py:104-133."""
+    E, H = _beam(1, x, y, z, wl, _IDENTITY_GEOM, w0, er, p, l)
+    return (E[0], E[1], E[2]), (H[0], H[1], H[2])
+
+
+def _builtin(field_fn, kwargs):
+    """(kind, parameters) when field_fn is a built-in beam (possibly a functools.partial)."""
+    kw = dict(kwargs)
+    fn = field_fn
+    while isinstance(fn, functools.partial):
+        kw = {**fn.keywords, **kw}
+        if fn.args:
+            return None
+        fn = fn.func
+    if fn is _paraxial_gaussian_field_fn and set(kw) <= {"beam_waist", "er"}:
+        return 0, dict(w0=kw.get("beam_waist", 1), er=kw.get("er", 1))
+    if fn is _paraxial_laguerre_gaussian_field_fn and set(kw) <= {"w0", "er", "p", "l"}:
+        return 1, dict(w0=kw.get("w0", 1), er=kw.get("er", 1), p=kw.get("p", 0), l=kw.get("l", 0))
+    return None
+
+
+def shifted_rotated_fields(field_fn, x, y, z, wavelength, beam_origin_x, beam_origin_y, beam_origin_z,
+                           polar_angle, azimuthal_angle, polarization_angle, **kwargs):
+    """Fields of a beam defined in its own frame, placed at the origin o and
+    rotated by rotation_matrix(...): field_fn is evaluated at M^-1 r - M^-1 o
+    and E, H are rotated back by M.  Returns (2, 3, *x.shape, 1) like the
+    reference.  khepri/beams.py:50-101."""
+    x, y, z = np.broadcast_arrays(np.asarray(x, dtype=float), np.asarray(y, dtype=float),
+                                  np.asarray(z, dtype=float))
+    mat = rotation_matrix(polar_angle, azimuthal_angle, polarization_angle)
+    minv = np.linalg.inv(mat)
+    r0 = np.linalg.solve(mat, np.array([beam_origin_x, beam_origin_y, beam_origin_z], dtype=float))
+    b = _builtin(field_fn, kwargs)
+    if b is not None:
+        kind, prm = b
+        E, H = _beam(kind, x, y, z, wavelength, np.concatenate([mat.ravel(), minv.ravel(), r0]), **prm)
+    else:   # a caller's own field function: evaluated by it on host arrays
+        coords = np.stack([x, y, z], axis=-1)
+        r = np.einsum("ij,...j->...i", minv, coords) - r0
+        (exr, eyr, ezr), (hxr, hyr, hzr) = field_fn(r[..., 0], r[..., 1], r[..., 2], wavelength, **kwargs)
+        Er = np.stack(np.broadcast_arrays(exr, eyr, ezr), axis=0)
+        Hr = np.stack(np.broadcast_arrays(hxr, hyr, hzr), axis=0)
+        E = np.einsum("ij,j...->i...", mat, Er)
+        H = np.einsum("ij,j...->i...", mat, Hr)
+    return np.asarray([E, H])[..., np.newaxis]
+
+
+def amplitudes_from_fields(fields, e, wl, kp, x, y, bzs, a=1):
+    """Fourier amplitudes (Ex, Ey, Hx, Hy) x harmonics of fields sampled on
+    bzs[0] x bzs[1] unit cells (fields (rows, cols, 2, 3), x, y (rows, cols)),
+    after removing the Bloch phase of kp; normalised by (number of cells x
+    points per cell side) like the reference.  The cells of the fields and of
+    the coordinates are paired in the order the reference splits them.
+    khepri/beams.py:164-190."""
+    lib = _lib.load()
+    dev = _device()
+    fields = np.asarray(fields)
+    x, y = np.asarray(x, dtype=float), np.asarray(y, dtype=float)
+    kxi, kyi = kp
+    F = fields / np.exp(1j * (kxi * x + kyi * y))[..., np.newaxis, np.newaxis]
+    F = np.swapaxes(np.asarray(np.split(np.asarray(np.split(F, bzs[0], axis=1)), bzs[1], axis=1)), 0, 1)
+    NS = F.shape[2]
+    xs = np.asarray(np.split(np.asarray(np.split(x, bzs[0], axis=1)), bzs[1], axis=1)).reshape(-1, NS, NS)
+    ys = np.asarray(np.split(np.asarray(np.split(y, bzs[0], axis=1)), bzs[1], axis=1)).reshape(-1, NS, NS)
+    ncell = prod(bzs)
+    F = F.reshape(-1, NS, NS, 2, 3)[..., :2].reshape(ncell, NS * NS, 4)
+    # the per-cell sums of the reference add up to one sum over all points
+    f = torch.as_tensor(np.ascontiguousarray(np.moveaxis(F, -1, 0).reshape(4, -1)), dtype=torch.complex128,
+                        device=dev)
+    xp = torch.as_tensor(np.tile(xs.reshape(-1), 4), device=dev)
+    yp = torch.as_tensor(np.tile(ys.reshape(-1), 4), device=dev)
+    kx, ky = (torch.as_tensor(np.ascontiguousarray(np.ravel(v), dtype=np.float64), device=dev) for v in e.g_vectors)
+    nk = kx.numel()
+    out = torch.empty((4, nk), dtype=torch.complex128, device=dev)
+    npts = ncell * NS * NS
+    check(lib.khx_points_dft(f.data_ptr(), xp.data_ptr(), yp.data_ptr(), npts, kx.data_ptr(), ky.data_ptr(), nk, 4,
+                             1.0 / (ncell * NS), out.data_ptr(), torch.cuda.current_stream().cuda_stream),
+          "khx_points_dft")
+    return out.cpu().numpy()

@@ -85,6 +85,21 @@ int khx_convmat(const double* eps, int is_complex, int nx, int ny, int P, int Q,
   return khx::convmat(as_stream(stream), eps, is_complex, nx, ny, P, Q, count, as_cd(C), static_cast<cd*>(ws));
 }
 
+int khx_beam_fields(int kind, const double* x, const double* y, const double* z, long long npts, double wl,
+                    double w0, double er, int p, int l, const double* geom, double* E, double* H, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(x && y && z && geom && E && H, "khx_beam_fields: null argument");
+  return khx::beam_fields(as_stream(stream), kind, x, y, z, npts, wl, w0, er, p, l, geom, as_cd(E), as_cd(H));
+}
+
+int khx_points_dft(const double* f, const double* x, const double* y, long long npts, const double* kx,
+                   const double* ky, int nk, int batch, double scale, double* out, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(f && x && y && kx && ky && out, "khx_points_dft: null argument");
+  return khx::points_dft(as_stream(stream), reinterpret_cast<const cd*>(f), x, y, npts, kx, ky, nk, batch, scale,
+                         as_cd(out));
+}
+
 int khx_convmat_islands(const int* kinds, const double* params, const double* eps, int nisl,
                         const double* eps_host, const double* recip, double sigma, int P, int Q, double* C, void* ws,
                         void* stream) {

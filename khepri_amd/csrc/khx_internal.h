@@ -40,6 +40,11 @@ bool lu_multi_fits(int n);
 int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL);
 // LU path selection: 0 = unblocked, 1 = one-workgroup blocked, 2 = multi-launch blocked
 int lu_path(int n);
+// beam front end: paraxial beam fields on points, point DFT amplitudes
+int beam_fields(hipStream_t s, int kind, const double* x, const double* y, const double* z, long long npts,
+                double wl, double w0, double er, int p, int l, const double* geom, cd* E, cd* H);
+int points_dft(hipStream_t s, const cd* f, const double* x, const double* y, long long npts, const double* kx,
+               const double* ky, int nk, int batch, double scale, cd* out);
 // island layers: closed-form coefficients gathered into C (Fws: (2P-1)(2Q-1) complex)
 int convmat_islands(hipStream_t s, const int* kinds, const double* params, const cd* eps, int nisl, cd eps_host,
                     const double* recip, double sigma, int P, int Q, cd* C, cd* Fws);
