@@ -64,6 +64,33 @@ int khx_beam_fields(int kind, const double* x, const double* y, const double* z,
 int khx_points_dft(const double* f, const double* x, const double* y, long long npts, const double* kx,
                    const double* ky, int nk, int batch, double scale, double* out, void* stream);
 
+/* Band-structure tool (khepri/eigentricks.py:5-55).  S (batch, 4ng, 4ng)
+ * complex on the device in the (+, -) block layout: p = rows/columns 0:2ng,
+ * m = 2ng:4ng, S[p,p] forward transmission, S[m,p] reflection from the left,
+ * S[p,m] reflection from the right, S[m,m] backward transmission.
+ * khx_scattering_splitlr: Sl = [[Spp, 0], [Smp, -I]], Sr = [[I, -Spm],
+ * [0, -Smm]] (scattering_splitlr, khepri/eigentricks.py:5-22).
+ * khx_scattering_eig: Bloch factors w (batch, 4ng) and unit-norm modes v
+ * (batch, 4ng, 4ng, column j for w[j]) of the pencil Sl v = w Sr v
+ * (scattering_eigenvalues, khepri/eigentricks.py:29-40, which calls scipy's
+ * QZ eig(Sl, Sr)), computed as the eigenpairs of the Cayley transform
+ * (Sl - s Sr)^-1 (Sl + s Sr); s = (sigma_re, sigma_im) on the unit circle must
+ * not be a Bloch factor.  info[b]: 0 ok, > 0 eigensolver did not converge,
+ * < 0 s is a Bloch factor of item b (call again with another s),
+ * KHX_INFO_NAN_INPUT S[b] holds a NaN (the reference returns None); w and v
+ * of items with info < 0 or KHX_INFO_NAN_INPUT are NaN.
+ * khx_scattering_det: det[b] = det(Spp - Spm Smm^-1 Smp) det(Smm)
+ * (scattering_det, khepri/eigentricks.py:24-28), formed as sign * exp(sum
+ * log|u_ii|) like numpy's det; info[b] = k+1 when Smm is singular at pivot k
+ * (numpy's inv raises LinAlgError).                                         */
+#define KHX_INFO_NAN_INPUT 0x40000000
+int khx_scattering_splitlr(int ng, const double* S, int batch, double* Sl, double* Sr, void* stream);
+size_t khx_scattering_eig_ws_bytes(int ng, int batch);
+int khx_scattering_eig(int ng, const double* S, int batch, double sigma_re, double sigma_im, double* w, double* v,
+                       int* info, void* ws, void* stream);
+size_t khx_scattering_det_ws_bytes(int ng, int batch);
+int khx_scattering_det(int ng, const double* S, int batch, double* det, int* info, void* ws, void* stream);
+
 /* Convolution matrix of an island layer (replaces the Formulation.ANALYTICAL
  * branch of Layer.solve, khepri/layer.py:161-166, with the island transforms
  * of khepri/fourier.py:7-58 and combine_fourier_masks :145-159).  Islands on

@@ -70,6 +70,11 @@ SIGNATURES = {
     "khx_beam_fields": (_i, [_i, _vp, _vp, _vp, C.c_longlong, C.c_double, C.c_double, C.c_double, _i, _i, _vp, _vp,
                              _vp, _vp]),
     "khx_points_dft": (_i, [_vp, _vp, _vp, C.c_longlong, _vp, _vp, _i, _i, C.c_double, _vp, _vp]),
+    "khx_scattering_splitlr": (_i, [_i, _vp, _i, _vp, _vp, _vp]),
+    "khx_scattering_eig_ws_bytes": (_sz, [_i, _i]),
+    "khx_scattering_eig": (_i, [_i, _vp, _i, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp]),
+    "khx_scattering_det_ws_bytes": (_sz, [_i, _i]),
+    "khx_scattering_det": (_i, [_i, _vp, _i, _vp, _vp, _vp, _vp]),
     "khx_convmat_islands": (_i, [_vp, _vp, _vp, _i, _vp, _vp, C.c_double, _i, _i, _vp, _vp, _vp]),
     "khx_eig_slots": (_i, [_i]),
     "khx_gemm_record": (_i, [_i, _vp, _i]),

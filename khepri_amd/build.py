@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkhx.so")
-SOURCES = ["blas.hip", "lu.hip", "lu_blocked.hip", "eig.hip", "layers.hip", "star.hip", "flux.hip", "fields.hip", "adjoint.hip", "beams.hip", "prof.hip", "capi.hip"]
+SOURCES = ["blas.hip", "lu.hip", "lu_blocked.hip", "eig.hip", "layers.hip", "star.hip", "flux.hip", "fields.hip", "adjoint.hip", "beams.hip", "eigentricks.hip", "prof.hip", "capi.hip"]
 
 ARCH = os.environ.get("KHX_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

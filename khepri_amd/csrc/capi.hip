@@ -100,6 +100,34 @@ int khx_points_dft(const double* f, const double* x, const double* y, long long 
                          as_cd(out));
 }
 
+int khx_scattering_splitlr(int ng, const double* S, int batch, double* Sl, double* Sr, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(S && Sl && Sr, "khx_scattering_splitlr: null argument");
+  return khx::scattering_splitlr(as_stream(stream), ng, reinterpret_cast<const cd*>(S), batch, as_cd(Sl), as_cd(Sr));
+}
+
+size_t khx_scattering_eig_ws_bytes(int ng, int batch) {
+  return ng < 1 || batch < 0 ? 0 : khx::scattering_eig_ws_bytes(ng, batch);
+}
+
+int khx_scattering_eig(int ng, const double* S, int batch, double sigma_re, double sigma_im, double* w, double* v,
+                       int* info, void* ws, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(S && w && v && info && ws, "khx_scattering_eig: null argument");
+  return khx::scattering_eig(as_stream(stream), ng, reinterpret_cast<const cd*>(S), batch, mkc(sigma_re, sigma_im),
+                             as_cd(w), as_cd(v), info, ws);
+}
+
+size_t khx_scattering_det_ws_bytes(int ng, int batch) {
+  return ng < 1 || batch < 0 ? 0 : khx::scattering_det_ws_bytes(ng, batch);
+}
+
+int khx_scattering_det(int ng, const double* S, int batch, double* det, int* info, void* ws, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(S && det && info && ws, "khx_scattering_det: null argument");
+  return khx::scattering_det(as_stream(stream), ng, reinterpret_cast<const cd*>(S), batch, as_cd(det), info, ws);
+}
+
 int khx_convmat_islands(const int* kinds, const double* params, const double* eps, int nisl,
                         const double* eps_host, const double* recip, double sigma, int P, int Q, double* C, void* ws,
                         void* stream) {

@@ -46,6 +46,12 @@ int beam_fields(hipStream_t s, int kind, const double* x, const double* y, const
 int points_dft(hipStream_t s, const cd* f, const double* x, const double* y, long long npts, const double* kx,
                const double* ky, int nk, int batch, double scale, cd* out);
 // island layers: closed-form coefficients gathered into C (Fws: (2P-1)(2Q-1) complex)
+// band-structure tool: S-matrix pencil split, Bloch eigenpairs, determinant
+int scattering_splitlr(hipStream_t s, int ng, const cd* S, int batch, cd* Sl, cd* Sr);
+size_t scattering_eig_ws_bytes(int ng, int batch);
+int scattering_eig(hipStream_t s, int ng, const cd* S, int batch, cd sg, cd* w, cd* v, int* info, void* ws);
+size_t scattering_det_ws_bytes(int ng, int batch);
+int scattering_det(hipStream_t s, int ng, const cd* S, int batch, cd* det, int* info, void* ws);
 int convmat_islands(hipStream_t s, const int* kinds, const double* params, const cd* eps, int nisl, cd eps_host,
                     const double* recip, double sigma, int P, int Q, cd* C, cd* Fws);
 // B <- A^-1 B from zgetrf factors: interchanges, then blocked triangular solves
