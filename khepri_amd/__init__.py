@@ -28,4 +28,7 @@ def __getattr__(name):
     if name == "ExtendedLayer":
         from .extension import ExtendedLayer
         return ExtendedLayer
+    if name in ("save", "load"):
+        from . import state
+        return getattr(state, name)
     raise AttributeError(name)

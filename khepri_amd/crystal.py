@@ -11,6 +11,7 @@ from __future__ import annotations
 import cmath
 import os
 from copy import copy
+from types import SimpleNamespace
 
 import numpy as np
 import torch
@@ -138,7 +139,7 @@ class Crystal:
         kpa = np.broadcast_to(kpa, (B, 2)).copy()
         te_a = np.broadcast_to(np.asarray(te, dtype=np.complex128), (B,)).copy()
         tm_a = np.broadcast_to(np.asarray(tm, dtype=np.complex128), (B,)).copy()
-        self.source = type("Source", (), {})()
+        self.source = SimpleNamespace()
         self.source.te, self.source.tm, self.source.theta, self.source.phi = te, tm, theta, phi
         self.source.wavelength = wavelength
         self._wl, self._kp, self._te, self._tm = wl, kpa, te_a, tm_a
