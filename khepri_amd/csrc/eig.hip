@@ -696,141 +696,183 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 #ifndef EIG_SLIDE_PF
 #define EIG_SLIDE_PF 8  // loads in flight per line in the in-window updates (16: register spills, slower)
 #endif
+// One block of the register chase on wavefront 0 (lane j): the rotations
+// kstart..kend of a single-shift sweep inside the LDS copy hb (nb x nb, row
+// stride nb + 1) of H[b0.., b0..], recorded in crot / srot; the block rows
+// above the core take their column rotations at the end.
+__device__ __forceinline__ void reg_chase_block(cd* hb, int nb, int b0, int l, int ihi, int kstart, int kend,
+                                                cd shift, double* crot, cd* srot, int j) {
+  const int ld = nb + 1;
+  int q = kstart - b0;  // block index of the current rotation: 0 in the first block, 1 after
+  cd x, y;
+  if (kstart == l) {
+    x = hb[0] - shift;
+    y = hb[ld];
+  } else {
+    x = hb[q * ld + q - 1];
+    y = hb[(q + 1) * ld + q - 1];
+  }
+  cd a00 = hb[q * ld + q], a01 = hb[q * ld + q + 1];
+  cd a10 = hb[(q + 1) * ld + q], a11 = hb[(q + 1) * ld + q + 1];
+  const bool mine0 = (j >= q + 2 && j < nb);
+  cd v = mine0 ? hb[q * ld + j] : mkc(0.0, 0.0);
+  cd yj = mine0 ? hb[(q + 1) * ld + j] : mkc(0.0, 0.0);
+  cd wq = (q + 2 < nb) ? hb[(q + 2) * ld + q + 1] : mkc(0.0, 0.0);
+  cd dq = (q + 2 < nb) ? hb[(q + 2) * ld + q + 2] : mkc(0.0, 0.0);
+  for (int k = kstart; k <= kend; ++k, ++q) {
+    double c;
+    cd s, r;
+    givens(x, y, c, s, r);
+    // prefetch the next step's untouched inputs
+    const bool mine = (j >= q + 2 && j < nb);
+    const bool more = (k < kend);
+    cd yj_n = mkc(0.0, 0.0), wq_n = mkc(0.0, 0.0), dq_n = mkc(0.0, 0.0);
+    if (more) {
+      if (j >= q + 3 && j < nb) yj_n = hb[(q + 2) * ld + j];
+      if (q + 3 < nb) {
+        wq_n = hb[(q + 3) * ld + q + 2];
+        dq_n = hb[(q + 3) * ld + q + 3];
+      }
+    }
+    if (mine) {  // row rotation of the columns right of the core
+      hb[q * ld + j] = fmac(c * v, s, yj);
+      v = fmsc(c * yj, conjc(s), v);
+    }
+    rot_rows(a00, a10, c, s);
+    rot_rows(a01, a11, c, s);
+    rot_cols(a00, a01, c, s);
+    rot_cols(a10, a11, c, s);
+    // row k+2: (k+2, k) was zero, (k+2, k+1) is untouched so far
+    const cd bulge = conjc(s) * wq;
+    const cd w2 = c * wq;
+    if (j == 0) {
+      crot[k] = c;
+      srot[k] = s;
+      if (k > l) {
+        hb[q * ld + q - 1] = r;
+        hb[(q + 1) * ld + q - 1] = mkc(0.0, 0.0);
+      }
+      hb[q * ld + q] = a00;
+      hb[q * ld + q + 1] = a01;
+    }
+    if (!more) {
+      if (j == 0) {
+        hb[(q + 1) * ld + q] = a10;
+        hb[(q + 1) * ld + q + 1] = a11;
+        if (q + 2 < nb) {
+          hb[(q + 2) * ld + q] = bulge;
+          hb[(q + 2) * ld + q + 1] = w2;
+        }
+      }
+      if (mine) hb[(q + 1) * ld + j] = v;
+      break;
+    }
+    x = a10;
+    y = bulge;
+    a00 = a11;
+    a01 = mkc(readlane_d(v.x, q + 2), readlane_d(v.y, q + 2));
+    a10 = w2;
+    a11 = dq;
+    yj = yj_n;
+    wq = wq_n;
+    dq = dq_n;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // column rotations of the block rows above the core: row i takes the
+  // rotations k >= b0 + i + 1 (columns k - b0, k - b0 + 1)
+  const int k0 = max(kstart, b0 + j + 1);
+  if (j < nb && k0 <= kend) {
+    cd* row = hb + j * ld - b0;  // row[col] = block entry (j, col - b0)
+    cd xv = row[k0];
+    int k = k0;
+    for (; k + 4 <= kend + 1; k += 4) {
+      cd yv[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) yv[m] = row[k + 1 + m];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        rot_cols(xv, yv[m], crot[k + m], srot[k + m]);
+        row[k + m] = xv;
+        xv = yv[m];
+      }
+    }
+    for (; k <= kend; ++k) {
+      cd yv = row[k + 1];
+      rot_cols(xv, yv, crot[k], srot[k]);
+      row[k] = xv;
+      xv = yv;
+    }
+    row[kend + 1] = xv;
+  }
+}
+
+// Single-shift sweep over H[l..ihi] through LDS blocks of EIG_QB rows.
+// Wavefront 0 chases the bulge through block b + 1 while the other
+// wavefronts apply the rotations of block b to the rest of the window (the
+// rows above it and the columns right of block b + 1); the few columns that
+// block b + 1 holds get block b's row rotations first.  Every element sees
+// the rotations in sweep order: a block's rest update finishes (barrier)
+// before the next block's rest update or load touches the same lines.
 template <int EIG_THREADS>
 __device__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, double* crot, cd* srot,
                              unsigned long long* acc) {
   const int tid = threadIdx.x;
   int kstart = l, b0 = l;
-  while (true) {
-    const int b1 = min(b0 + EIG_QB - 1, ihi);
-    const int nb = b1 - b0 + 1;
-    const int ld = nb + 1;
-    const int kend = (b1 == ihi) ? ihi - 1 : b1 - 2;  // last rotation whose bulge stays in the block
+  int b1 = min(b0 + EIG_QB - 1, ihi);
+  int kend = (b1 == ihi) ? ihi - 1 : b1 - 2;  // last rotation whose bulge stays in the block
+  {
+    const int nb = b1 - b0 + 1, ld = nb + 1;
     for (int e = tid; e < nb * nb; e += EIG_THREADS)
       hb[(e / nb) * ld + e % nb] = H[(long long)(b0 + e / nb) * n + b0 + e % nb];
     __syncthreads();
     unsigned long long t_c0 = acc ? wall_clock64() : 0;
-    if (tid < 64) {
-      const int j = tid;  // block column (row rotations), then block row (column rotations)
-      int q = kstart - b0;  // block index of the current rotation: 0 in the first block, 1 after
-      cd x, y;
-      if (kstart == l) {
-        x = hb[0] - shift;
-        y = hb[ld];
-      } else {
-        x = hb[q * ld + q - 1];
-        y = hb[(q + 1) * ld + q - 1];
-      }
-      cd a00 = hb[q * ld + q], a01 = hb[q * ld + q + 1];
-      cd a10 = hb[(q + 1) * ld + q], a11 = hb[(q + 1) * ld + q + 1];
-      const bool mine0 = (j >= q + 2 && j < nb);
-      cd v = mine0 ? hb[q * ld + j] : mkc(0.0, 0.0);
-      cd yj = mine0 ? hb[(q + 1) * ld + j] : mkc(0.0, 0.0);
-      cd wq = (q + 2 < nb) ? hb[(q + 2) * ld + q + 1] : mkc(0.0, 0.0);
-      cd dq = (q + 2 < nb) ? hb[(q + 2) * ld + q + 2] : mkc(0.0, 0.0);
-      for (int k = kstart; k <= kend; ++k, ++q) {
-        double c;
-        cd s, r;
-        givens(x, y, c, s, r);
-        // prefetch the next step's untouched inputs
-        const bool mine = (j >= q + 2 && j < nb);
-        const bool more = (k < kend);
-        cd yj_n = mkc(0.0, 0.0), wq_n = mkc(0.0, 0.0), dq_n = mkc(0.0, 0.0);
-        if (more) {
-          if (j >= q + 3 && j < nb) yj_n = hb[(q + 2) * ld + j];
-          if (q + 3 < nb) {
-            wq_n = hb[(q + 3) * ld + q + 2];
-            dq_n = hb[(q + 3) * ld + q + 3];
-          }
-        }
-        if (mine) {  // row rotation of the columns right of the core
-          hb[q * ld + j] = fmac(c * v, s, yj);
-          v = fmsc(c * yj, conjc(s), v);
-        }
-        rot_rows(a00, a10, c, s);
-        rot_rows(a01, a11, c, s);
-        rot_cols(a00, a01, c, s);
-        rot_cols(a10, a11, c, s);
-        // row k+2: (k+2, k) was zero, (k+2, k+1) is untouched so far
-        const cd bulge = conjc(s) * wq;
-        const cd w2 = c * wq;
-        if (j == 0) {
-          crot[k] = c;
-          srot[k] = s;
-          if (k > l) {
-            hb[q * ld + q - 1] = r;
-            hb[(q + 1) * ld + q - 1] = mkc(0.0, 0.0);
-          }
-          hb[q * ld + q] = a00;
-          hb[q * ld + q + 1] = a01;
-        }
-        if (!more) {
-          if (j == 0) {
-            hb[(q + 1) * ld + q] = a10;
-            hb[(q + 1) * ld + q + 1] = a11;
-            if (q + 2 < nb) {
-              hb[(q + 2) * ld + q] = bulge;
-              hb[(q + 2) * ld + q + 1] = w2;
-            }
-          }
-          if (mine) hb[(q + 1) * ld + j] = v;
-          break;
-        }
-        x = a10;
-        y = bulge;
-        a00 = a11;
-        a01 = mkc(readlane_d(v.x, q + 2), readlane_d(v.y, q + 2));
-        a10 = w2;
-        a11 = dq;
-        yj = yj_n;
-        wq = wq_n;
-        dq = dq_n;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      // column rotations of the block rows above the core: row i takes the
-      // rotations k >= b0 + i + 1 (columns k - b0, k - b0 + 1)
-      const int k0 = max(kstart, b0 + j + 1);
-      if (j < nb && k0 <= kend) {
-        cd* row = hb + j * ld - b0;  // row[col] = block entry (j, col - b0)
-        cd xv = row[k0];
-        int k = k0;
-        for (; k + 4 <= kend + 1; k += 4) {
-          cd yv[4];
-#pragma unroll
-          for (int m = 0; m < 4; ++m) yv[m] = row[k + 1 + m];
-#pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            rot_cols(xv, yv[m], crot[k + m], srot[k + m]);
-            row[k + m] = xv;
-            xv = yv[m];
-          }
-        }
-        for (; k <= kend; ++k) {
-          cd yv = row[k + 1];
-          rot_cols(xv, yv, crot[k], srot[k]);
-          row[k] = xv;
-          xv = yv;
-        }
-        row[kend + 1] = xv;
+    if (tid < 64) reg_chase_block(hb, nb, b0, l, ihi, kstart, kend, shift, crot, srot, tid);
+    __syncthreads();
+    if (acc) acc[1] += wall_clock64() - t_c0;
+  }
+  while (true) {
+    const int nb = b1 - b0 + 1, ld = nb + 1;
+    for (int e = tid; e < nb * nb; e += EIG_THREADS)
+      H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[(e / nb) * ld + e % nb];
+    const int nabove = b0 - l;
+    if (kend == ihi - 1) {  // last block (b1 == ihi): only the rows above remain
+      for (int e = tid; e < nabove; e += EIG_THREADS)
+        slide<false, EIG_SLIDE_PF>(H + (long long)(l + e) * n, 1, kstart, kend + 1, crot, srot);
+      __syncthreads();
+      break;
+    }
+    // next block: rows / columns b0n..b1n
+    const int b0n = kend, b1n = min(b0n + EIG_QB - 1, ihi);
+    const int kstartn = kend + 1, kendn = (b1n == ihi) ? ihi - 1 : b1n - 2;
+    // columns b1+1..b1n (held by the next block) take this block's row rotations now
+    for (int e = tid; e < b1n - b1; e += EIG_THREADS)
+      slide<true, EIG_SLIDE_PF>(H + b1 + 1 + e, n, kstart, kend + 1, crot, srot);
+    __syncthreads();
+    const int nbn = b1n - b0n + 1, ldn = nbn + 1;
+    for (int e = tid; e < nbn * nbn; e += EIG_THREADS)
+      hb[(e / nbn) * ldn + e % nbn] = H[(long long)(b0n + e / nbn) * n + b0n + e % nbn];
+    __syncthreads();
+    unsigned long long t_c0 = acc ? wall_clock64() : 0;
+    if (tid < 64) reg_chase_block(hb, nbn, b0n, l, ihi, kstartn, kendn, shift, crot, srot, tid);
+    if (EIG_THREADS == 64 || tid >= 64) {
+      // rest of this block: rows l..b0-1 (column rotations) and columns b1n+1..ihi (row rotations);
+      // a one-wavefront workgroup does it after the chase
+      constexpr int RW = EIG_THREADS == 64 ? 64 : EIG_THREADS - 64;
+      const int nright = ihi - b1n, tot = nabove + nright;
+      for (int e = EIG_THREADS == 64 ? tid : tid - 64; e < tot; e += RW) {
+        if (e < nabove)
+          slide<false, EIG_SLIDE_PF>(H + (long long)(l + e) * n, 1, kstart, kend + 1, crot, srot);
+        else
+          slide<true, EIG_SLIDE_PF>(H + b1n + 1 + (e - nabove), n, kstart, kend + 1, crot, srot);
       }
     }
     __syncthreads();
     if (acc) acc[1] += wall_clock64() - t_c0;
-    for (int e = tid; e < nb * nb; e += EIG_THREADS)
-      H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[(e / nb) * ld + e % nb];
-    const int nabove = b0 - l, nright = ihi - b1;
-    const int tot = nabove + nright;
-    for (int e = tid; e < tot; e += EIG_THREADS) {
-      if (e < nabove)
-        slide<false, EIG_SLIDE_PF>(H + (long long)(l + e) * n, 1, kstart, kend + 1, crot, srot);
-      else
-        slide<true, EIG_SLIDE_PF>(H + b1 + 1 + (e - nabove), n, kstart, kend + 1, crot, srot);
-    }
-    __syncthreads();
-    if (kend == ihi - 1) break;
-    kstart = kend + 1;
-    b0 = kstart - 1;
+    kstart = kstartn;
+    kend = kendn;
+    b0 = b0n;
+    b1 = b1n;
   }
 }
 
