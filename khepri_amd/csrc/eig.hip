@@ -693,6 +693,9 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+#ifndef EIG_HESS_LOAD
+#define EIG_HESS_LOAD 1  // the chase blocks move only the Hessenberg band (column >= row - 2) between H and LDS (0: whole blocks)
+#endif
 #ifndef EIG_SLIDE_PF
 #define EIG_SLIDE_PF 8  // loads in flight per line in the in-window updates (16: register spills, slower)
 #endif
@@ -824,7 +827,7 @@ __device__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, dou
   {
     const int nb = b1 - b0 + 1, ld = nb + 1;
     for (int e = tid; e < nb * nb; e += EIG_THREADS)
-      hb[(e / nb) * ld + e % nb] = H[(long long)(b0 + e / nb) * n + b0 + e % nb];
+      if (EIG_HESS_LOAD == 0 || e % nb >= e / nb - 2) hb[(e / nb) * ld + e % nb] = H[(long long)(b0 + e / nb) * n + b0 + e % nb];
     __syncthreads();
     unsigned long long t_c0 = acc ? wall_clock64() : 0;
     if (tid < 64) reg_chase_block(hb, nb, b0, l, ihi, kstart, kend, shift, crot, srot, tid);
@@ -834,7 +837,7 @@ __device__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, dou
   while (true) {
     const int nb = b1 - b0 + 1, ld = nb + 1;
     for (int e = tid; e < nb * nb; e += EIG_THREADS)
-      H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[(e / nb) * ld + e % nb];
+      if (EIG_HESS_LOAD == 0 || e % nb >= e / nb - 2) H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[(e / nb) * ld + e % nb];
     const int nabove = b0 - l;
     if (kend == ihi - 1) {  // last block (b1 == ihi): only the rows above remain
       for (int e = tid; e < nabove; e += EIG_THREADS)
@@ -851,7 +854,8 @@ __device__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, dou
     __syncthreads();
     const int nbn = b1n - b0n + 1, ldn = nbn + 1;
     for (int e = tid; e < nbn * nbn; e += EIG_THREADS)
-      hb[(e / nbn) * ldn + e % nbn] = H[(long long)(b0n + e / nbn) * n + b0n + e % nbn];
+      if (EIG_HESS_LOAD == 0 || e % nbn >= e / nbn - 2)
+        hb[(e / nbn) * ldn + e % nbn] = H[(long long)(b0n + e / nbn) * n + b0n + e % nbn];
     __syncthreads();
     unsigned long long t_c0 = acc ? wall_clock64() : 0;
     if (tid < 64) reg_chase_block(hb, nbn, b0n, l, ihi, kstartn, kendn, shift, crot, srot, tid);
