@@ -328,7 +328,8 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
       const int c = k0 + j;
       if (j > 0) {
         // (1) right update of column c by the panel's previous reflectors: a -= Y V(c, :)^H
-        for (int i = tid; i < n; i += EIG_THREADS) {
+        // (rows k0+1..; the rows above the panel get it from Y's top rows after the panel)
+        for (int i = k0 + 1 + tid; i < n; i += EIG_THREADS) {
           const cd* yr = Y + (long long)i * HB_NB;
           const cd* vc = V + (long long)c * HB_NB;
           cd a = H[(long long)i * n + c];
@@ -374,7 +375,7 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
       __syncthreads();
       // (4) vh = H(:, c+1:) u  (panel-start columns): each wavefront takes four
       // rows at a time (four independent loads in flight per lane)
-      for (int i0 = wid * 4; i0 < n; i0 += EIG_WAVES * 4) {
+      for (int i0 = k0 + 1 + wid * 4; i0 < n; i0 += EIG_WAVES * 4) {
         double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
         for (int q = lane; q < L; q += 64) {
           const cd uq = u[q];
@@ -402,7 +403,7 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
       } else if (tid == j) {
         Ts[j * HB_NB + j] = mkc(skip ? 0.0 : 1.0, 0.0);
       }
-      for (int i = tid; i < n; i += EIG_THREADS) {
+      for (int i = k0 + 1 + tid; i < n; i += EIG_THREADS) {
         const cd* yr = Y + (long long)i * HB_NB;
         cd a = vh[i];
         for (int q = 0; q < j; ++q) a = fmsc(a, yr[q], wv[q]);
@@ -421,6 +422,26 @@ __device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, c
         for (int p = 0; p <= q; ++p) t = fmac(t, vr[p], Ts[p * HB_NB + q]);
         tr[q] = t;
       }
+    }
+    __syncthreads();
+    // Y's rows 0..k0 in one pass (zlahr2's Y(1:K) = A(1:K, K+1:N) V T): the
+    // columns past k0 are still at their panel-start values in those rows
+    for (int e = tid; e < (k0 + 1) * HB_NB; e += EIG_THREADS) {
+      const int i = e / HB_NB, q = e % HB_NB;
+      const cd* hr = H + (long long)i * n;
+      cd acc = mkc(0.0, 0.0);
+      for (int m = k0 + 1; m < n; ++m) acc = fmac(acc, hr[m], Vt[(long long)m * HB_NB + q]);
+      Y[(long long)i * HB_NB + q] = acc;
+    }
+    __syncthreads();
+    // rows 0..k0 of the panel's later columns: a -= Y V(c, :)^H
+    for (int e = tid; e < (k0 + 1) * (nbp - 1); e += EIG_THREADS) {
+      const int i = e / (nbp - 1), c = k0 + 1 + e % (nbp - 1);
+      const cd* yr = Y + (long long)i * HB_NB;
+      const cd* vc = V + (long long)c * HB_NB;
+      cd a = H[(long long)i * n + c];
+      for (int q = 0; q < HB_NB; ++q) a = fmsc(a, yr[q], conjc(vc[q]));
+      H[(long long)i * n + c] = a;
     }
     __syncthreads();
     // (6)-(8) trailing columns A <- (I - V Vt^H)(A - Y V^H) and Zt <- Zt - conj(V) Vt^T Zt,
