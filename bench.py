@@ -181,6 +181,94 @@ def extra_workload(name, args):
             # forward (R, T) + backward: one dT/d eps map per frequency for all 4 x 64 x 64 pixels
             return flux_gradient(cl, wT=1.0)
         units, unit = len(cfg["freqs"]), "grad evals/s"
+    elif name == "islands":
+        # analytical island layer (disc + rectangle, closed-form coefficients) in the c2 stack, pw (9, 9),
+        # 64 frequencies x 8 x 8 Bloch vectors
+        isl = [{"type": "disc", "params": [0.5, 0.5, 0.3], "epsilon": 1.0},
+               {"type": "rectangle", "params": [0.05, 0.05, 0.25, 0.9], "epsilon": 4.0}]
+        fr = np.repeat(np.linspace(0.45, 0.65, 64), 64)
+        kp = np.tile(workloads.bz_grid((8, 8)).reshape(2, -1).T, (64, 1))
+
+        def build(cls_, pw=(9, 9)):
+            c = cls_(pw)
+            c.add_layer_uniform("S1", 1.0, 1.1)
+            c.add_layer_analytical("Sisl", isl, 12.0, 0.55)
+            c.set_device(["Sisl", "S1", "Sisl"], False)
+            return c
+        cl = build(Crystal)
+        cl.set_source(1 / fr, 1.0, 0.0, kp=kp)
+
+        def step():
+            cl.solve()
+            return cl.poynting_flux_end()
+        units, unit = len(fr), "solves/s"
+
+        def cpu(nsample):
+            from oracle import crystal as OC
+            oc = build(OC.Crystal)
+            pos = np.linspace(0, len(fr) - 1, nsample).astype(int)
+            t0 = time.perf_counter()
+            for p_ in pos:
+                oc.set_source(1 / fr[p_], 1.0, 0.0, kp=tuple(kp[p_]))
+                oc.solve()
+                oc.poynting_flux_end()
+            return nsample / (time.perf_counter() - t0), f"{nsample} solves of the same workload"
+    elif name == "bands":
+        # Bloch eigenpairs of 2 048 period S-matrices (the c2 disc period, pw (5, 5): 4ng = 100),
+        # 32 frequencies x 8 x 8 Bloch vectors, solved in one batch
+        from khepri_amd import eigentricks as E
+        cfg = workloads.disc_stack_bz()
+        fr = np.repeat(np.linspace(0.45, 0.65, 32), 64)
+        kp = np.tile(workloads.bz_grid((8, 8)).reshape(2, -1).T, (32, 1))
+        cl = Crystal((5, 5))
+        cl.add_layer_uniform("S1", cfg["eps_gap"], cfg["depth_gap"])
+        cl.add_layer_pixmap("Scyl", cfg["pixmap"], cfg["depth_disc"])
+        cl.set_device(["Scyl", "S1"], False)
+        cl.set_source(1 / fr, 1.0, 0.0, kp=kp)
+        cl.solve(rt_only=False)
+        S = E.scattering_matrix(cl)
+        del cl
+
+        def step():
+            return E.scattering_eigenvalues(S)
+        units, unit = S.shape[0], "S-matrices/s"
+
+        def cpu(nsample):
+            from oracle import eigentricks as OE
+            Sn = S[np.linspace(0, S.shape[0] - 1, nsample).astype(int)].cpu().numpy()
+            t0 = time.perf_counter()
+            for b in range(nsample):
+                OE.scattering_eigenvalues(Sn[b])
+            return nsample / (time.perf_counter() - t0), f"{nsample} S-matrices of the same batch, scipy QZ eig(Sl, Sr)"
+    elif name == "beams":
+        # Gaussian beam (tilted, shifted) sampled on 8 x 8 unit cells of 32 x 32 points, then its Bloch
+        # amplitudes (pw (9, 9)) at 16 Bloch vectors: one step = fields + 16 amplitude sets
+        from functools import partial
+        from khepri_amd import beams as B
+        from oracle import beams as OB
+        ns, bzs = 32, (8, 8)
+        xs = np.arange(ns * bzs[0]) / ns
+        X, Y = np.meshgrid(xs, xs)
+        Z = np.zeros_like(X)
+        geo = (4.0, 4.0, -0.5, 0.2, 0.3, 0.1)
+        kps = workloads.bz_grid((4, 4)).reshape(2, -1).T
+        e = Expansion((9, 9))
+
+        def step():
+            f = B.shifted_rotated_fields(partial(B._paraxial_gaussian_field_fn, beam_waist=2.0), X, Y, Z, 0.63, *geo)
+            fr_ = np.swapaxes(np.swapaxes(f[..., 0], 0, 2), 1, 3)
+            return [B.amplitudes_from_fields(fr_, e, 0.63, tuple(k), X, Y, bzs) for k in kps]
+        units, unit = len(kps), "k-points/s"
+
+        def cpu(nsample):
+            from khepri_amd.expansion import Expansion as PE
+            gv = PE((9, 9)).g_vectors
+            t0 = time.perf_counter()
+            f = OB.shifted_rotated_fields(partial(OB.paraxial_gaussian_field_fn, beam_waist=2.0), X, Y, Z, 0.63, *geo)
+            fr_ = np.swapaxes(np.swapaxes(f[..., 0], 0, 2), 1, 3)
+            for k in kps[:nsample]:
+                OB.amplitudes_from_fields(fr_, gv, (9, 9), tuple(k), X, Y, bzs)
+            return nsample / (time.perf_counter() - t0), f"fields + {nsample} amplitude sets of the same workload"
     else:
         raise SystemExit(f"unknown workload {name}")
     tuned = run_autotune(step, args)
@@ -192,10 +280,17 @@ def extra_workload(name, args):
         step()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    return {"metric": f"{name} throughput", "value": round(units * args.steps / el, 3), "unit": unit,
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
-            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": name, "units_per_step": units}, "gemm_autotune": tuned}
+    out = {"metric": f"{name} throughput", "value": round(units * args.steps / el, 3), "unit": unit,
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": name, "units_per_step": units}, "gemm_autotune": tuned}
+    if "cpu" in locals() and args.cpu_sample > 0:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=1), np.errstate(all="ignore"):
+            v, what = cpu(min(args.cpu_sample, {"islands": 10, "bands": 10, "beams": 2}[name]))
+        out["cpu_baseline"] = {"value": round(v, 3), "unit": unit, "cores": 1, "kind": "port",
+                               "sample": f"oracle, 1 BLAS thread: {what}"}
+    return out
 
 
 def run_autotune(step, args):
@@ -244,7 +339,8 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one plain device copy of this many bytes before the workload (byte-counter calibration)")
     ap.add_argument("--workload", default="disc_stack_bz",
-                    help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted | grad_grid")
+                    help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted | grad_grid | "
+                         "islands | bands | beams")
     ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
     ap.add_argument("--freqs", type=int, default=32, help="twisted: number of frequencies")
     ap.add_argument("--angles", type=int, default=8, help="twisted: number of twist angles")
