@@ -112,6 +112,7 @@ def extra_workload(name, args):
     import torch
     from khepri_amd import Crystal, Expansion, Layer, workloads
     from khepri_amd.batch import CrystalBatch
+    cpu = None  # oracle timing of a bounded sample, for the workloads that have one
     if name == "disc_sweep":
         cfg = workloads.disc_stack_sweep()
         cl = Crystal(cfg["pw"])
@@ -284,7 +285,7 @@ def extra_workload(name, args):
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
            "config": {"workload": name, "units_per_step": units}, "gemm_autotune": tuned}
-    if "cpu" in locals() and args.cpu_sample > 0:
+    if cpu is not None and args.cpu_sample > 0:
         from threadpoolctl import threadpool_limits
         with threadpool_limits(limits=1), np.errstate(all="ignore"):
             v, what = cpu(min(args.cpu_sample, {"islands": 10, "bands": 10, "beams": 2}[name]))
