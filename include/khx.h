@@ -59,6 +59,16 @@ int khx_kvectors(const double* g, int N, const double* kp, const double* wl, int
  * ky[k] y[b][p])) for batch point sets of npts points (slow_dft,
  * khepri/fourier.py:94-104, as used by amplitudes_from_fields,
  * khepri/beams.py:164-190).                                                 */
+/* khx_bloch_amplitudes: the whole of amplitudes_from_fields
+ * (khepri/beams.py:164-190) on the device: fields (R, C, 2, 3) complex and
+ * coordinates x, y (R, C) sampled on bz0 x bz1 square cells of ns x ns points
+ * (R = ns bz1, C = ns bz0); the Bloch phase of (kpx, kpy) is removed at each
+ * field point and out (4 x nk complex: Ex, Ey, Hx, Hy) holds the sums over
+ * every cell of the field cells' DFTs at the harmonics (gx, gy), with the
+ * reference's pairing of field cells and coordinate cells, / (bz0 bz1 ns).  */
+int khx_bloch_amplitudes(const double* fields, const double* x, const double* y, int ns, int bz0, int bz1,
+                         double kpx, double kpy, const double* gx, const double* gy, int nk, double* out,
+                         void* stream);
 int khx_beam_fields(int kind, const double* x, const double* y, const double* z, long long npts, double wl,
                     double w0, double er, int p, int l, const double* geom, double* E, double* H, void* stream);
 int khx_points_dft(const double* f, const double* x, const double* y, long long npts, const double* kx,

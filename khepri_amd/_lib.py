@@ -69,6 +69,7 @@ SIGNATURES = {
     "khx_redheffer_bwd": (_i, [_dp, _dp, _i, _i, _dp, _dp, _dp, _ip, _vp, _vp]),
     "khx_beam_fields": (_i, [_i, _vp, _vp, _vp, C.c_longlong, C.c_double, C.c_double, C.c_double, _i, _i, _vp, _vp,
                              _vp, _vp]),
+    "khx_bloch_amplitudes": (_i, [_vp, _vp, _vp, _i, _i, _i, C.c_double, C.c_double, _vp, _vp, _i, _vp, _vp]),
     "khx_points_dft": (_i, [_vp, _vp, _vp, C.c_longlong, _vp, _vp, _i, _i, C.c_double, _vp, _vp]),
     "khx_scattering_splitlr": (_i, [_i, _vp, _i, _vp, _vp, _vp]),
     "khx_scattering_eig_ws_bytes": (_sz, [_i, _i]),

@@ -7,7 +7,9 @@ and fed to ``Crystal.fields_volume(incident_fields=...)`` for every k-point
 of the Brillouin-zone grid (``gen_bzi_grid``).
 
 Field evaluation of the built-in paraxial beams (Gaussian, Laguerre-Gauss)
-and the point DFTs run in libkhx (``khx_beam_fields``, ``khx_points_dft``);
+and the Bloch amplitudes run in libkhx (``khx_beam_fields``,
+``khx_bloch_amplitudes``; ``points_dft`` binds the plain point DFT,
+``khx_points_dft``);
 the 3 x 3 beam rotation and the grid bookkeeping are host geometry.  A field
 function written by the caller (any other callable) is evaluated by that
 callable on host arrays, as in the reference, and only rotated here.
@@ -17,7 +19,6 @@ including the trailing length-1 axis of ``shifted_rotated_fields``.
 from __future__ import annotations
 
 import functools
-from math import prod
 
 import numpy as np
 import torch
@@ -133,33 +134,53 @@ def shifted_rotated_fields(field_fn, x, y, z, wavelength, beam_origin_x, beam_or
 
 def amplitudes_from_fields(fields, e, wl, kp, x, y, bzs, a=1):
     """Fourier amplitudes (Ex, Ey, Hx, Hy) x harmonics of fields sampled on
-    bzs[0] x bzs[1] unit cells (fields (rows, cols, 2, 3), x, y (rows, cols)),
-    after removing the Bloch phase of kp; normalised by (number of cells x
-    points per cell side) like the reference.  The cells of the fields and of
-    the coordinates are paired in the order the reference splits them.
-    khepri/beams.py:164-190."""
+    bzs[0] x bzs[1] unit cells (fields (rows, cols, 2, 3), x, y (rows, cols),
+    rows = ns bzs[1], cols = ns bzs[0]), after removing the Bloch phase of kp;
+    normalised by (number of cells x points per cell side) like the
+    reference.  The cells of the fields and of the coordinates are paired in
+    the orders the reference splits them.  The whole computation, phase
+    removal included, is one libkhx launch (khx_bloch_amplitudes); numpy
+    arrays or device tensors are accepted.  khepri/beams.py:164-190."""
     lib = _lib.load()
     dev = _device()
-    fields = np.asarray(fields)
-    x, y = np.asarray(x, dtype=float), np.asarray(y, dtype=float)
-    kxi, kyi = kp
-    F = fields / np.exp(1j * (kxi * x + kyi * y))[..., np.newaxis, np.newaxis]
-    F = np.swapaxes(np.asarray(np.split(np.asarray(np.split(F, bzs[0], axis=1)), bzs[1], axis=1)), 0, 1)
-    NS = F.shape[2]
-    xs = np.asarray(np.split(np.asarray(np.split(x, bzs[0], axis=1)), bzs[1], axis=1)).reshape(-1, NS, NS)
-    ys = np.asarray(np.split(np.asarray(np.split(y, bzs[0], axis=1)), bzs[1], axis=1)).reshape(-1, NS, NS)
-    ncell = prod(bzs)
-    F = F.reshape(-1, NS, NS, 2, 3)[..., :2].reshape(ncell, NS * NS, 4)
-    # the per-cell sums of the reference add up to one sum over all points
-    f = torch.as_tensor(np.ascontiguousarray(np.moveaxis(F, -1, 0).reshape(4, -1)), dtype=torch.complex128,
-                        device=dev)
-    xp = torch.as_tensor(np.tile(xs.reshape(-1), 4), device=dev)
-    yp = torch.as_tensor(np.tile(ys.reshape(-1), 4), device=dev)
+
+    def on_dev(v, dtype):
+        if isinstance(v, torch.Tensor):
+            return v.to(device=dev, dtype=dtype).contiguous()
+        return torch.as_tensor(np.ascontiguousarray(v), dtype=dtype, device=dev)
+    f = on_dev(fields, torch.complex128)
+    xd, yd = on_dev(x, torch.float64), on_dev(y, torch.float64)
+    rows, cols = xd.shape
+    if tuple(f.shape) != (rows, cols, 2, 3) or tuple(yd.shape) != (rows, cols):
+        raise ValueError(f"fields must be (rows, cols, 2, 3) and x, y (rows, cols); got {tuple(f.shape)}, "
+                         f"{tuple(xd.shape)}, {tuple(yd.shape)}")
+    if rows % bzs[1] or cols % bzs[0] or rows // bzs[1] != cols // bzs[0]:
+        raise ValueError(f"{rows} x {cols} samples do not split into {bzs[0]} x {bzs[1]} square cells")
+    ns = rows // bzs[1]
     kx, ky = (torch.as_tensor(np.ascontiguousarray(np.ravel(v), dtype=np.float64), device=dev) for v in e.g_vectors)
     nk = kx.numel()
     out = torch.empty((4, nk), dtype=torch.complex128, device=dev)
-    npts = ncell * NS * NS
-    check(lib.khx_points_dft(f.data_ptr(), xp.data_ptr(), yp.data_ptr(), npts, kx.data_ptr(), ky.data_ptr(), nk, 4,
-                             1.0 / (ncell * NS), out.data_ptr(), torch.cuda.current_stream().cuda_stream),
-          "khx_points_dft")
+    check(lib.khx_bloch_amplitudes(f.data_ptr(), xd.data_ptr(), yd.data_ptr(), ns, int(bzs[0]), int(bzs[1]),
+                                   float(np.real(kp[0])), float(np.real(kp[1])), kx.data_ptr(), ky.data_ptr(), nk,
+                                   out.data_ptr(), torch.cuda.current_stream().cuda_stream), "khx_bloch_amplitudes")
     return out.cpu().numpy()
+
+
+def points_dft(f, x, y, kx, ky):
+    """sum_p f[..., p] exp(-i (kx x_p + ky y_p)) for each (kx, ky) on the
+    device; f (..., npts) with x, y (npts,) shared.  khepri/fourier.py:94-104."""
+    lib = _lib.load()
+    dev = _device()
+    f = np.asarray(f, dtype=np.complex128)
+    lead = f.shape[:-1]
+    fb = torch.as_tensor(np.ascontiguousarray(f.reshape(-1, f.shape[-1])), device=dev)
+    batch, npts = fb.shape
+    xb = torch.as_tensor(np.tile(np.asarray(x, dtype=np.float64).ravel(), batch), device=dev)
+    yb = torch.as_tensor(np.tile(np.asarray(y, dtype=np.float64).ravel(), batch), device=dev)
+    kxd = torch.as_tensor(np.ascontiguousarray(np.ravel(kx), dtype=np.float64), device=dev)
+    kyd = torch.as_tensor(np.ascontiguousarray(np.ravel(ky), dtype=np.float64), device=dev)
+    out = torch.empty((batch, kxd.numel()), dtype=torch.complex128, device=dev)
+    check(lib.khx_points_dft(fb.data_ptr(), xb.data_ptr(), yb.data_ptr(), npts, kxd.data_ptr(), kyd.data_ptr(),
+                             kxd.numel(), batch, 1.0, out.data_ptr(), torch.cuda.current_stream().cuda_stream),
+          "khx_points_dft")
+    return out.cpu().numpy().reshape(*lead, kxd.numel())

@@ -104,6 +104,65 @@ __global__ __launch_bounds__(256) void points_dft_kernel(const cd* __restrict__ 
   }
 }
 
+// Bloch amplitudes of fields sampled on bz0 x bz1 square cells of ns x ns
+// points (khepri/beams.py:164-190): fields (R, C, 2, 3) complex and x, y
+// (R, C) with R = ns bz1, C = ns bz0.  out[j][k] (j = Ex, Ey, Hx, Hy) =
+// scale sum over cells c and cell points (a, b) of
+//   F_j(c, a, b) exp(-i (gx[k] xs(c, a, b) + gy[k] ys(c, a, b))),
+// F = fields exp(-i (kpx x + kpy y)) taken at the field's own point.  The
+// reference splits the fields and the coordinates into cells in two orders:
+// field cell c is cell (i0, i1) = (c / bz1, c % bz1) of the column / row
+// split, coordinate cell c is (c % bz0, c / bz0); the same pairing is kept.
+__global__ __launch_bounds__(256) void bloch_amplitudes_kernel(const cd* __restrict__ fields,
+                                                               const double* __restrict__ x,
+                                                               const double* __restrict__ y, int ns, int bz0, int bz1,
+                                                               double kpx, double kpy, const double* __restrict__ gx,
+                                                               const double* __restrict__ gy, int nk, double scale,
+                                                               cd* __restrict__ out) {
+  const int kk = blockIdx.x, j = blockIdx.y;
+  const int C = ns * bz0;
+  const long long per = (long long)ns * ns, total = per * bz0 * bz1;
+  const double g0 = gx[kk], g1 = gy[kk];
+  double sr = 0.0, si = 0.0;
+  for (long long t = threadIdx.x; t < total; t += blockDim.x) {
+    const int c = (int)(t / per), a = (int)((t / ns) % ns), b = (int)(t % ns);
+    const long long pf = (long long)((c % bz1) * ns + a) * C + (c / bz1) * ns + b;
+    const long long px = (long long)((c / bz0) * ns + a) * C + (c % bz0) * ns + b;
+    const cd v = fields[pf * 6 + (j >> 1) * 3 + (j & 1)];
+    double sn, cs;
+    sincos(-(kpx * x[pf] + kpy * y[pf] + g0 * x[px] + g1 * y[px]), &sn, &cs);
+    sr += v.x * cs - v.y * sn;
+    si += v.x * sn + v.y * cs;
+  }
+  __shared__ double rr[4], ri[4];
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    rr[wid] = sr;
+    ri[wid] = si;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tr = 0.0, ti = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      tr += rr[q];
+      ti += ri[q];
+    }
+    out[(long long)j * nk + kk] = mkc(tr * scale, ti * scale);
+  }
+}
+
+int bloch_amplitudes(hipStream_t s, const cd* fields, const double* x, const double* y, int ns, int bz0, int bz1,
+                     double kpx, double kpy, const double* gx, const double* gy, int nk, cd* out) {
+  KHX_REQUIRE(ns >= 1 && bz0 >= 1 && bz1 >= 1 && nk >= 0, "bloch_amplitudes: bad sizes");
+  if (nk == 0) return 0;
+  const double scale = 1.0 / ((double)bz0 * bz1 * ns);
+  bloch_amplitudes_kernel<<<dim3(nk, 4), 256, 0, s>>>(fields, x, y, ns, bz0, bz1, kpx, kpy, gx, gy, nk, scale, out);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
 int beam_fields(hipStream_t s, int kind, const double* x, const double* y, const double* z, long long npts,
                 double wl, double w0, double er, int p, int l, const double* geom, cd* E, cd* H) {
   KHX_REQUIRE(kind == 0 || kind == 1, "beam_fields: kind must be 0 (Gaussian) or 1 (Laguerre-Gauss)");

@@ -92,6 +92,15 @@ int khx_beam_fields(int kind, const double* x, const double* y, const double* z,
   return khx::beam_fields(as_stream(stream), kind, x, y, z, npts, wl, w0, er, p, l, geom, as_cd(E), as_cd(H));
 }
 
+int khx_bloch_amplitudes(const double* fields, const double* x, const double* y, int ns, int bz0, int bz1,
+                         double kpx, double kpy, const double* gx, const double* gy, int nk, double* out,
+                         void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(fields && x && y && gx && gy && out, "khx_bloch_amplitudes: null argument");
+  return khx::bloch_amplitudes(as_stream(stream), reinterpret_cast<const cd*>(fields), x, y, ns, bz0, bz1, kpx, kpy,
+                               gx, gy, nk, as_cd(out));
+}
+
 int khx_points_dft(const double* f, const double* x, const double* y, long long npts, const double* kx,
                    const double* ky, int nk, int batch, double scale, double* out, void* stream) {
   KHX_ENTRY_BEGIN

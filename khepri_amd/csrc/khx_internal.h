@@ -43,6 +43,8 @@ int lu_path(int n);
 // beam front end: paraxial beam fields on points, point DFT amplitudes
 int beam_fields(hipStream_t s, int kind, const double* x, const double* y, const double* z, long long npts,
                 double wl, double w0, double er, int p, int l, const double* geom, cd* E, cd* H);
+int bloch_amplitudes(hipStream_t s, const cd* fields, const double* x, const double* y, int ns, int bz0, int bz1,
+                     double kpx, double kpy, const double* gx, const double* gy, int nk, cd* out);
 int points_dft(hipStream_t s, const cd* f, const double* x, const double* y, long long npts, const double* kx,
                const double* ky, int nk, int batch, double scale, cd* out);
 // island layers: closed-form coefficients gathered into C (Fws: (2P-1)(2Q-1) complex)

@@ -67,3 +67,40 @@ def test_beam_functions_direct_and_caller_callable(gpu):
     got = B.shifted_rotated_fields(plane, X, Y, Z, 0.7, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, kz=2.0)
     ref = OB.shifted_rotated_fields(plane, X, Y, Z, 0.7, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, kz=2.0)
     np.testing.assert_allclose(got, ref, atol=1e-13)
+
+
+def test_amplitudes_device_inputs_and_cell_checks(gpu):
+    """Device-tensor inputs give the numpy-input result; a grid that does not
+    split into square cells is rejected; non-square cell counts (bzs 3 x 2)
+    follow the oracle's cell pairing."""
+    import torch
+    from khepri_amd import beams as B
+    from khepri_amd.expansion import Expansion
+    from oracle import beams as OB
+    rng = np.random.default_rng(11)
+    ns, bzs = 5, (3, 2)
+    X, Y = np.meshgrid(np.arange(ns * bzs[0]) / ns, np.arange(ns * bzs[1]) / ns + 0.1)
+    F = rng.standard_normal((*X.shape, 2, 3)) + 1j * rng.standard_normal((*X.shape, 2, 3))
+    e = Expansion((3, 5))
+    got = B.amplitudes_from_fields(F, e, 0.7, (0.3, -0.4), X, Y, bzs)
+    ref = OB.amplitudes_from_fields(F, e.g_vectors, (3, 5), (0.3, -0.4), X, Y, bzs)
+    np.testing.assert_allclose(got, ref, atol=1e-12 * np.abs(ref).max())
+    dev = torch.device("cuda")
+    got2 = B.amplitudes_from_fields(torch.as_tensor(F, device=dev), e, 0.7, (0.3, -0.4), torch.as_tensor(X, device=dev),
+                                    torch.as_tensor(Y, device=dev), bzs)
+    np.testing.assert_array_equal(got2, got)
+    with pytest.raises(ValueError):
+        B.amplitudes_from_fields(F[:, :-1], e, 0.7, (0.3, -0.4), X[:, :-1], Y[:, :-1], bzs)
+
+
+def test_points_dft_vs_oracle(gpu):
+    from khepri_amd import beams as B
+    from oracle import beams as OB
+    rng = np.random.default_rng(12)
+    x, y = rng.uniform(-1, 1, 300), rng.uniform(-1, 1, 300)
+    f = rng.standard_normal((2, 300)) + 1j * rng.standard_normal((2, 300))
+    kx, ky = rng.uniform(-9, 9, 25), rng.uniform(-9, 9, 25)
+    got = B.points_dft(f, x, y, kx, ky)
+    for b in range(2):
+        ref = OB.slow_dft(f[b], x, y, kx, ky)
+        np.testing.assert_allclose(got[b], ref, atol=1e-12 * np.abs(ref).max())
