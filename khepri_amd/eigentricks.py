@@ -42,7 +42,7 @@ def _to_dev(S):
     if not batched:
         t = t[None]
     if t.dim() != 3 or t.shape[1] != t.shape[2] or t.shape[1] % 4 != 0:
-        raise ValueError(f"S must be (4ng, 4ng) or (B, 4ng, 4ng), got {tuple(S.shape)}")
+        raise ValueError(f"S must be (4ng, 4ng) or (B, 4ng, 4ng), got {tuple(np.shape(S))}")
     return t.contiguous(), batched, numpy_in
 
 
