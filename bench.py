@@ -167,6 +167,34 @@ def extra_workload(name, args):
                 out.append(cl.poynting_flux_end())
             return out
         units, unit = len(fr) * len(cls), "solves/s"
+    elif name == "twisted_fields":
+        # the twisted bilayer's field output: for every seeded angle at the middle frequency, solve with
+        # the full chain and evaluate E and H on a 64 x 64 x 32 volume (x, y over the cell, z over 0..1)
+        cfg = workloads.twisted_bilayer()
+        fmid = cfg["freqs"][len(cfg["freqs"]) // 2]
+        xs = np.linspace(0, 1, 64)
+        X, Y = np.meshgrid(xs, xs)
+        Zs = np.linspace(0, 1.0, 32)
+        cls = []
+        for ang in cfg["angles_deg"][:args.angles]:
+            e1, e2 = Expansion(cfg["pw"]), Expansion(cfg["pw"])
+            e2.rotate(np.deg2rad(ang))
+            etw = e1 + e2
+            cl = Crystal.from_expansion(etw)
+            cl.add_layer("L1", Layer.pixmap(e1, cfg["pixmap"], cfg["depth"]), extended=True)
+            cl.add_layer("gap", Layer.uniform(etw, 1.0, cfg["gap"]))
+            cl.add_layer("L2", Layer.pixmap(e2, cfg["pixmap"], cfg["depth"]), extended=True)
+            cl.set_device(["L1", "gap", "L2"], True)
+            cl.set_source(1 / fmid, 1.0, 0.0)
+            cls.append(cl)
+
+        def step():
+            out = []
+            for cl in cls:
+                cl.solve()
+                out.append(cl.fields_volume(X, Y, Zs))
+            return out
+        units, unit = len(cls), "field volumes/s"
     elif name == "grad_grid":
         from khepri_amd.adjoint import flux_gradient
         cfg = workloads.grad_grid()
@@ -341,7 +369,7 @@ def main():
                     help="run one plain device copy of this many bytes before the workload (byte-counter calibration)")
     ap.add_argument("--workload", default="disc_stack_bz",
                     help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted | grad_grid | "
-                         "islands | bands | beams")
+                         "twisted_fields | islands | bands | beams")
     ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
     ap.add_argument("--freqs", type=int, default=32, help="twisted: number of frequencies")
     ap.add_argument("--angles", type=int, default=8, help="twisted: number of twist angles")

@@ -44,3 +44,21 @@ def test_twisted_625_harmonics_rt(gpu):
     cl.solve(rt_only=True)
     r, t = cl.poynting_flux_end()
     assert abs(r - g["R"][0]) <= TOL and abs(t - g["T"][0]) <= TOL
+
+
+def test_twisted_625_harmonics_fields(gpu):
+    """fields_volume of the 625-harmonic twisted bilayer (first seeded angle,
+    middle frequency) on an 8 x 8 x 3 grid against the reference's golden
+    E and H: relative error ||E - E_ref|| / ||E_ref|| <= 1e-8."""
+    g = golden("twisted.npz")
+    cfg = W.twisted_bilayer()
+    cl = twisted((5, 5), np.deg2rad(cfg["angles_deg"][0]), cfg["pixmap"], fields=True)
+    cl.set_source(1 / float(g["freq"]), 1.0, 0.0)
+    cl.solve()
+    E, H = cl.fields_volume(g["x"], g["y"], g["z"])
+    for got, ref in ((E, g["E"]), (H, g["H"])):
+        got = np.asarray(got)
+        assert got.shape == ref.shape
+        rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        print(f"field rel. error {rel:.2e}")
+        assert rel <= 1e-8
