@@ -37,7 +37,10 @@ class CrystalBatch:
     def __init__(self, crystals, device=None, chunk=None):
         self.crystals = list(crystals)
         self.device = torch.device(device) if device is not None else None
-        self.chunk = chunk or int(os.environ.get("KHX_BATCH_CHUNK", "512"))
+        env = os.environ.get("KHX_BATCH_CHUNK")
+        # patterned layers per launch; default: the eigensolver's resident
+        # slots at the group's order, so every launch is one full round
+        self.chunk = chunk or (int(env) if env else None)
 
     def _dev(self):
         return self.device or default_device()
@@ -94,8 +97,9 @@ class CrystalBatch:
         S_all = torch.empty((M, 2, 2, n, n), dtype=ops.CDT, device=dev)
         # ---- patterned instances: per-item convolution matrices, chunked
         pat = [k for k, (_, L) in enumerate(inst) if L.formulation in (Formulation.FFT, Formulation.ANALYTICAL)]
-        for c0 in range(0, len(pat), self.chunk):
-            ks = pat[c0:c0 + self.chunk]
+        chunk = self.chunk or ops.eig_slots(n)
+        for c0 in range(0, len(pat), chunk):
+            ks = pat[c0:c0 + chunk]
             C, IC = self._convmats([inst[k][1] for k in ks], e.pw, dev)
             kt = torch.as_tensor(ks, device=dev)
             kx, ky = ops.kvectors(g, kp[kt].contiguous(), wl[kt].contiguous())
