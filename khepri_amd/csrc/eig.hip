@@ -78,6 +78,16 @@ __device__ __forceinline__ int block_max_int(int v, int* red) {
   return t;
 }
 
+// 1/sqrt(x) for x in the normal range: the hardware estimate refined by the
+// second-order step y + y e (1/2 + 3/8 e), e = 1 - x y^2 (the series of
+// (1 - e)^(-1/2)); no zero / infinity / NaN handling, which the caller's range
+// test makes unnecessary.
+__device__ __forceinline__ double rsqrt_normal(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 // Complex Givens rotation G = [[c, s], [-conj(s), c]] with G [x; y] = [r; 0].
 // Fast path: two independent reciprocal square roots instead of the chain of
 // hypot calls and divisions; squares stay far from over/underflow there.
@@ -85,7 +95,7 @@ __device__ __forceinline__ void givens(cd x, cd y, double& c, cd& s, cd& r) {
   const double ax2 = abs2c(x), ay2 = abs2c(y);
   const double nr2 = ax2 + ay2;
   if (ax2 > 1e-280 && nr2 < 1e280 && ay2 > 0.0) {
-    const double iax = rsqrt(ax2), inr = rsqrt(nr2);
+    const double iax = rsqrt_normal(ax2), inr = rsqrt_normal(nr2);
     const double ax = ax2 * iax, nr = nr2 * inr;
     c = ax * inr;
     s = (iax * inr) * (x * conjc(y));
