@@ -90,3 +90,24 @@ def test_state_round_trip_solved(gpu, tmp_path):
     c2.solve()
     R3, T3 = c2.poynting_flux_end()
     assert np.abs(R3 - R).max() < 1e-13 and np.abs(T3 - T).max() < 1e-13
+
+
+def test_runner_command_line(gpu, tmp_path):
+    """python -m khepri_amd.ez -s STUDY -p PROJECT.json writes the study's CSV."""
+    import json
+    import subprocess
+    import sys
+    proj = copy.deepcopy(PROJECT)
+    prm = proj["studies"]["spec"]["parameters"]
+    prm["output_data"] = os.path.join(tmp_path, "cli.csv")
+    path = os.path.join(tmp_path, "project.json")
+    with open(path, "w") as f:
+        json.dump(proj, f)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-m", "khepri_amd.ez", "-s", "spec", "-p", path], cwd=root,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rows = np.loadtxt(prm["output_data"], delimiter=",", skiprows=1)
+    assert rows.shape == (8, 7)
+    from khepri_amd import ez
+    np.testing.assert_allclose(rows, ez.run(PROJECT, "spec"), rtol=0, atol=1e-13)
