@@ -316,7 +316,7 @@ def extra_workload(name, args):
     if cpu is not None and args.cpu_sample > 0:
         from threadpoolctl import threadpool_limits
         with threadpool_limits(limits=1), np.errstate(all="ignore"):
-            v, what = cpu(min(args.cpu_sample, {"islands": 10, "bands": 10, "beams": 2}[name]))
+            v, what = cpu(min(args.cpu_sample, {"islands": 30, "bands": 200, "beams": 4}[name]))
         out["cpu_baseline"] = {"value": round(v, 3), "unit": unit, "cores": 1, "kind": "port",
                                "sample": f"oracle, 1 BLAS thread: {what}"}
     return out
@@ -363,7 +363,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--solves", type=int, default=65536, help="subset of the workload (default: all)")
-    ap.add_argument("--cpu-sample", type=int, default=40, help="oracle solves timed for cpu_baseline (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=100, help="oracle solves timed for cpu_baseline (0: skip; 100 = about 10 s of CPU work)")
     ap.add_argument("--b1-sample", type=int, default=24, help="solves timed one at a time for the plain batch-1 line")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one plain device copy of this many bytes before the workload (byte-counter calibration)")
