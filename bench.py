@@ -274,7 +274,6 @@ def extra_workload(name, args):
         # amplitudes (pw (9, 9)) at 16 Bloch vectors: one step = fields + 16 amplitude sets
         from functools import partial
         from khepri_amd import beams as B
-        from oracle import beams as OB
         ns, bzs = 32, (8, 8)
         xs = np.arange(ns * bzs[0]) / ns
         X, Y = np.meshgrid(xs, xs)
@@ -291,6 +290,7 @@ def extra_workload(name, args):
 
         def cpu(nsample):
             from khepri_amd.expansion import Expansion as PE
+            from oracle import beams as OB
             gv = PE((9, 9)).g_vectors
             t0 = time.perf_counter()
             f = OB.shifted_rotated_fields(partial(OB.paraxial_gaussian_field_fn, beam_waist=2.0), X, Y, Z, 0.63, *geo)
