@@ -179,7 +179,7 @@ def _gradient_chunk(cl, names, wrt, g, N, kp, wl, te, tm, wR, wT, eta_rel, reduc
     Ps = [fwd[seq[0]][1]]
     for nm in seq[1:]:
         P, info = ops.redheffer(Ps[-1], fwd[nm][1])
-        ops.raise_on_info(info, "star product")
+        ops.raise_on_info(info, "star product", lu=True)
         Ps.append(P)
     Stot = Ps[-1]
     sR = ops.matmul(Stot[:, 0, 0], c[:, :, None])[..., 0]
@@ -194,7 +194,7 @@ def _gradient_chunk(cl, names, wrt, g, N, kp, wl, te, tm, wR, wT, eta_rel, reduc
     Sbar = {}
     for k in range(len(seq) - 1, 0, -1):
         Pbar, SkBar, info = ops.redheffer_bwd(Ps[k - 1], fwd[seq[k]][1], G)
-        ops.raise_on_info(info, "star product adjoint")
+        ops.raise_on_info(info, "star product adjoint", lu=True)
         _accumulate(Sbar, seq[k], SkBar)
         G = Pbar
     _accumulate(Sbar, seq[0], G)

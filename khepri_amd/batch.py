@@ -146,7 +146,7 @@ class CrystalBatch:
                 V = S_all[off:off + G * m].view(G, m, 2, 2, n, n)
                 sR, sT, infos = chain.rt_amplitudes([("dense", V[:, q]) for q in range(m)], c)
                 for info in infos:
-                    ops.raise_on_info(info, "star-product junction")
+                    ops.raise_on_info(info, "star-product junction", lu=True)
             r, t, _, _ = ops.poynting_vec(sR, sT, g, kpc, wlc, ei, ee)
             for q, i in enumerate(crys):
                 out_r[i] = r[q]
@@ -174,5 +174,5 @@ class CrystalBatch:
             arr = np.stack([np.asarray(layers[k].epsilon, dtype=np.complex128 if cplx else np.float64) for k in ks])
             C[torch.as_tensor(ks, device=dev)] = ops.convmat(torch.as_tensor(arr, device=dev), pw)
         IC, info = ops.inverse(C)
-        ops.raise_on_info(info, "inverse of the convolution matrices")
+        ops.raise_on_info(info, "inverse of the convolution matrices", lu=True)
         return C, IC

@@ -122,12 +122,20 @@ class Crystal:
 
     # ------------------------------------------------------------------ source
     def set_source(self, wavelength, te=1.0, tm=1.0, theta=0.0, phi=0.0, kp=None):
-        """khepri/crystal.py:345-360; every argument may be an array (batch)."""
+        """khepri/crystal.py:345-360; every argument may be an array (batch).
+
+        kp is one in-plane wave vector (kx, ky), shape (2,), or a batch of
+        them, shape (B, 2) with one (kx, ky) per row.  Any other shape raises
+        ValueError: a (2, B) array such as a flattened Brillouin-zone grid
+        must be transposed first (a (2, 2) array is read as two rows)."""
         wl = np.atleast_1d(np.asarray(wavelength, dtype=np.float64))
         self._scalar = np.ndim(wavelength) == 0 and (kp is None or np.ndim(kp) == 1) \
             and np.ndim(theta) == 0 and np.ndim(phi) == 0
         if kp is not None:
-            kpa = np.asarray(kp, dtype=np.complex128).reshape(-1, 2)
+            kpa = np.asarray(kp, dtype=np.complex128)
+            if kpa.ndim not in (1, 2) or kpa.shape[-1] != 2:
+                raise ValueError(f"kp must have shape (2,) or (B, 2), got {kpa.shape}")
+            kpa = kpa.reshape(-1, 2)
         else:
             th = np.atleast_1d(np.asarray(theta, dtype=np.float64))
             ph = np.atleast_1d(np.asarray(phi, dtype=np.float64))
@@ -214,8 +222,11 @@ class Crystal:
         if not rt_only and chunk >= B:
             Layer.solve_batch_many([self.layers[name] for name in names], kp_all, wl_all)
             self._check_layers(names)
+            infos = []
             self.stacking_matrices, self.stacking_reverse_matrices, self.Stot = stack_layers(
-                self.expansion.pw, stacked, self.stack_retain_mask, reverse=want_reverse)
+                self.expansion.pw, stacked, self.stack_retain_mask, reverse=want_reverse, infos=infos)
+            for info in infos:
+                self._check_info(info, "star product", lu=True)
             self.S = self.Stot
             return
         if want_reverse:
@@ -235,10 +246,14 @@ class Crystal:
                 c = ops.incident(N, kp, wl, te_all[sl], tm_all[sl], self.epsi)
                 sR, sT, infos = chain.rt_amplitudes([layer.chain_item() for layer in stacked], c)
                 for info in infos:
-                    self._check_info(info, "star-product junction")
+                    self._check_info(info, "star-product junction", lu=True)
                 r, t, ro, to = ops.poynting_vec(sR, sT, g, kp, wl, self.epsi, self.epse, orders=True)
             else:
-                _, _, Stot = stack_layers(self.expansion.pw, stacked, self.stack_retain_mask, reverse=False)
+                infos = []
+                _, _, Stot = stack_layers(self.expansion.pw, stacked, self.stack_retain_mask, reverse=False,
+                                          infos=infos)
+                for info in infos:
+                    self._check_info(info, "star product", lu=True)
                 r, t, ro, to = ops.poynting(Stot, g, kp, wl, te_all[sl], tm_all[sl], self.epsi, self.epse,
                                             orders=True)
             R[sl], T[sl], Ro[sl], To[sl] = r, t, ro, to
@@ -246,9 +261,9 @@ class Crystal:
                 self.layers[name].S = None
         self._RT = (R, T, Ro, To)
 
-    def _check_info(self, info, what):
+    def _check_info(self, info, what, lu=False):
         if info is not None and os.environ.get("KHX_CHECK_INFO", "1") == "1":
-            ops.raise_on_info(info, what)
+            ops.raise_on_info(info, what, lu=lu)
 
     def _check_layers(self, names):
         for name in names:

@@ -512,8 +512,7 @@ int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info) {
   KHX_REQUIRE(A.stride != 0 && B.stride != 0 || batch == 1, "zgesv: operands cannot be broadcast");
   const size_t lds = (size_t)(2 * n + m) * sizeof(cd);
   KHX_REQUIRE(lds <= 150 * 1024, "zgesv: system too large for the LDS staging buffers");
-  static size_t lds_limit = 0;
-  KHX_TRY(ensure_dynamic_lds((const void*)zgesv_kernel, lds, lds_limit));
+  KHX_TRY(ensure_dynamic_lds((const void*)zgesv_kernel, lds));
   ProfScope ps("zgesv_kernel", s);
   zgesv_kernel<<<dim3(batch), dim3(GESV_THREADS), lds, s>>>(n, m, A.p, A.ld, A.stride, B.p, B.ld,
                                                            B.stride, info);

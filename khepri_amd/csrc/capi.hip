@@ -1,7 +1,10 @@
 // extern "C" entry points of libkhx.so (declared in include/khx.h).
 #include "../../include/khx.h"
 
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 
 #include "khx_internal.h"
 
@@ -10,6 +13,24 @@ namespace khx {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
+
+int ensure_dynamic_lds(const void* fn, size_t bytes) {
+  if (bytes <= 64 * 1024) return 0;
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> set_bytes;  // (device, kernel) -> attribute value
+  int dev = 0;
+  KHX_CHECK_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& cur = set_bytes[std::make_pair(dev, fn)];
+  if (bytes <= cur) return 0;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) {
+    set_error(std::string("hipFuncSetAttribute(MaxDynamicSharedMemorySize): ") + hipGetErrorString(e));
+    return -3;
+  }
+  cur = bytes;
+  return 0;
+}
 
 // defined in the other translation units
 size_t patterned_layer_bwd_ws_bytes(int n, int batch);

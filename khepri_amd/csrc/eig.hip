@@ -1437,8 +1437,7 @@ int zeig_slots(int n) {
                    : threads == 128 ? (const void*)zeig_schur_kernel<128>
                    : threads == 64  ? (const void*)zeig_schur_kernel<64>
                                     : (const void*)zeig_schur_kernel<256>;
-  static size_t lim = 0;
-  KHX_TRY(ensure_dynamic_lds(fn, lds, lim));
+  KHX_TRY(ensure_dynamic_lds(fn, lds));
   int per_cu = 0, dev = 0, cus = 0;
   KHX_CHECK_HIP(hipGetDevice(&dev));
   KHX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1487,26 +1486,25 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
                     ((rc_env && atoi(rc_env) == 0) ? 0 : 4);
   const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
   KHX_REQUIRE(lds <= 150 * 1024, "zeig: n too large for the LDS work vectors");
-  static size_t lds_limit[5] = {0, 0, 0, 0, 0};
   {
     ProfScope ps("zeig_schur_kernel", s);
     if (threads == 1024) {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<1024>, lds, lds_limit[4]));
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<1024>, lds));
       zeig_schur_kernel<1024><<<dim3(batch), dim3(1024), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
                                                                    tphase, flags);
     } else if (threads == 512) {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<512>, lds, lds_limit[3]));
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<512>, lds));
       zeig_schur_kernel<512><<<dim3(batch), dim3(512), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
                                                                  tphase, flags);
     } else if (threads == 64) {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<64>, lds, lds_limit[0]));
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<64>, lds));
       zeig_schur_kernel<64><<<dim3(batch), dim3(64), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase, flags);
     } else if (threads == 128) {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<128>, lds, lds_limit[1]));
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<128>, lds));
       zeig_schur_kernel<128><<<dim3(batch), dim3(128), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
                                                                  tphase, flags);
     } else {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<256>, lds, lds_limit[2]));
+      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<256>, lds));
       zeig_schur_kernel<256><<<dim3(batch), dim3(256), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
                                                                  tphase, flags);
     }

@@ -91,16 +91,9 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
 int zeig_slots(int n);
 
 // Raise a kernel's dynamic-LDS limit to `bytes` when it exceeds the 64 KiB
-// default (`current` caches the value already set for that kernel).
-inline int ensure_dynamic_lds(const void* fn, size_t bytes, size_t& current) {
-  if (bytes <= 64 * 1024 || bytes <= current) return 0;
-  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) {
-    set_error(std::string("hipFuncSetAttribute(MaxDynamicSharedMemorySize): ") + hipGetErrorString(e));
-    return -3;
-  }
-  current = bytes;
-  return 0;
-}
+// default.  One cache per (device, kernel) shared by every caller; the
+// attribute is only ever raised, so a later launch that needs less never
+// lowers it under one that needs more.
+int ensure_dynamic_lds(const void* fn, size_t bytes);
 
 }  // namespace khx

@@ -685,8 +685,7 @@ static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, in
   KHX_REQUIRE(batch <= 65535, "zlu_multi: at most 65535 systems per call");
   const bool gp = lu_panel_lds<NB>(n) > 150 * 1024;
   const size_t lds = gp ? 0 : lu_panel_lds<NB>(n);
-  static size_t lds_limit = 0;
-  if (!gp) KHX_TRY(ensure_dynamic_lds((const void*)zlu_panel_kernel<NB, false>, lds, lds_limit));
+  if (!gp) KHX_TRY(ensure_dynamic_lds((const void*)zlu_panel_kernel<NB, false>, lds));
   const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
   ProfScope ps(m > 0 ? "zgesv_kernel" : "zgetrf_kernel", s);
   // right-hand sides: with 64 or more of them the panels only interchange B's
@@ -778,8 +777,7 @@ int zlu_blocked(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv,
   if (batch <= 0 || n <= 0) return 0;
   const size_t lds = lu_blocked_lds(n);
   KHX_REQUIRE(lds <= 150 * 1024, "zlu_blocked: n too large for the LDS panel");
-  static size_t lds_limit = 0;
-  KHX_TRY(ensure_dynamic_lds((const void*)zlu_blocked_kernel, lds, lds_limit));
+  KHX_TRY(ensure_dynamic_lds((const void*)zlu_blocked_kernel, lds));
   ProfScope ps(m > 0 ? "zgesv_kernel" : "zgetrf_kernel", s);
   zlu_blocked_kernel<<<dim3(batch), dim3(LB_THREADS), lds, s>>>(n, m, A.p, A.ld, A.stride, B.p, B.ld, B.stride, ipiv,
                                                                 info, keepL ? 1 : 0);

@@ -57,7 +57,7 @@ def _fourier_fields_many(self, zs, inc, kx, ky):
     Sref = self.layers["Sref"]
     Rref = ops.eigenbasis(Sref.W, Sref.V)
     sol, info = ops.solve(Rref, inc)
-    ops.raise_on_info(info, "incident-medium eigenbasis")
+    ops.raise_on_info(info, "incident-medium eigenbasis", lu=True)
     c1p = sol[:, :n, :]                                       # (1, n, 1)
     groups = {}
     for j, z in enumerate(zs):
@@ -73,7 +73,7 @@ def _fourier_fields_many(self, zs, inc, kx, ky):
         # translate_mode_amplitudes (khepri/fields.py:18-27)
         F = ops.identity_minus(ops.matmul(Sl[:, 1, 1], Sr[:, 0, 0]))
         cdp, info = ops.solve(F, ops.matmul(Sl[:, 1, 0], c1p))
-        ops.raise_on_info(info, "mode amplitudes")
+        ops.raise_on_info(info, "mode amplitudes", lu=True)
         cdm = ops.matmul(Sr[:, 0, 0], cdp)
         RI = ops.eigenbasis(layer.W, layer.V)
         if isinstance(layer, Layer):
@@ -85,7 +85,7 @@ def _fourier_fields_many(self, zs, inc, kx, ky):
         R0 = ops.eigenbasis(W0, V0)
         amp = ops.matmul(R0, torch.cat([cdp, cdm], dim=1))
         a, info = ops.solve(RI, amp)                          # (1, 2n, 1)
-        ops.raise_on_info(info, "layer eigenbasis")
+        ops.raise_on_info(info, "layer eigenbasis", lu=True)
         cols = [j for j, _ in items]
         zbar = torch.as_tensor(np.array([k0 * (layer.depth - zr) for _, zr in items], dtype=np.float64),
                                device=dev)

@@ -137,7 +137,7 @@ class Layer:
                 e = self.expansion
                 C = ops.convmat_islands(self.epsilon, self.eps_host, e.pw, e.reciprocal, e.sigma, device)
                 IC, info = ops.inverse(C[None])
-                ops.raise_on_info(info, "inverse of the convolution matrix")
+                ops.raise_on_info(info, "inverse of the convolution matrix", lu=True)
                 self._C, self._IC, self._cm_key = C, IC[0], key
             return self._C, self._IC
         eps = np.asarray(self.epsilon)
@@ -148,7 +148,7 @@ class Layer:
                 t = t.to(torch.float64)
             C = ops.convmat(t, self.expansion.pw)
             IC, info = ops.inverse(C)
-            ops.raise_on_info(info, "inverse of the convolution matrix")
+            ops.raise_on_info(info, "inverse of the convolution matrix", lu=True)
             self._C, self._IC, self._cm_key = C[0], IC[0], key
         return self._C, self._IC
 
@@ -251,18 +251,22 @@ def scattering_identity(B, n, device):
     return S
 
 
-def stack_layers(pw, layers, mask, reverse=True, ws=None):
+def stack_layers(pw, layers, mask, reverse=True, ws=None, infos=None):
     """Forward chain S_I * S_0 * S_1 * ... (copies kept where mask) and the
     reverse chain (khepri/layer.py:35-60).
 
     S_I * S = S * S_I = S exactly, so the products with the identity are not
     formed.  The reverse chain is only needed for fields (``reverse=False``
-    skips it and returns None for it).
+    skips it and returns None for it).  ``infos``, when a list, receives the
+    device status word of every star product (first zero pivot of D or F,
+    where the reference's numpy solve raises LinAlgError).
     """
     Stot = layers[0].S
     Sls = [Stot if mask[0] else None]
     for keep, layer in zip(mask[1:], layers[1:]):
         Stot, info = ops.redheffer(Stot, layer.S, ws=ws)
+        if infos is not None:
+            infos.append(info)
         Sls.append(Stot if keep else None)
     if not reverse:
         return Sls, None, Stot
@@ -275,7 +279,12 @@ def stack_layers(pw, layers, mask, reverse=True, ws=None):
                                                                          layer.S.device))
         else:
             Srs.append(None)
-        Srev = layer.S if Srev is None else ops.redheffer(layer.S, Srev, ws=ws)[0]
+        if Srev is None:
+            Srev = layer.S
+        else:
+            Srev, info = ops.redheffer(layer.S, Srev, ws=ws)
+            if infos is not None:
+                infos.append(info)
     Srs.append(Srev)
     Srs = list(reversed(Srs))
     return Sls, Srs, Stot

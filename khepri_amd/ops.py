@@ -50,11 +50,18 @@ def eig_slots(n):
     return _SLOTS[key]
 
 
-def raise_on_info(info, what):
-    """Map per-item status words to numpy-style exceptions (one host sync)."""
+def raise_on_info(info, what, lu=False):
+    """Map per-item status words to numpy-style exceptions (one host sync).
+
+    Eigen-solver words (lu=False): > 0 no convergence, < 0 an LU of the
+    S-matrix algebra met an exact zero pivot.  LU words (lu=True): nonzero =
+    first zero pivot + 1.  Both raise numpy's LinAlgError, as numpy's eig,
+    solve and inv do in the reference."""
     if info is None:
         return
     v = info.cpu().numpy()
+    if lu and (v != 0).any():
+        raise np.linalg.LinAlgError(f"{what}: singular matrix (items {np.nonzero(v)[0][:8]})")
     if (v > 0).any():
         raise np.linalg.LinAlgError(f"{what}: eigenvalues did not converge (items {np.nonzero(v > 0)[0][:8]})")
     if (v < 0).any():
@@ -184,7 +191,8 @@ def eig_adjoint_mix(G, mu, mubar, eta_rel):
     lib = _lib.load()
     _require(G, CDT, "G")
     b, n, _ = G.shape
-    check(lib.khx_eig_adjoint_mix(_p(G), _p(mu.contiguous()), _p(mubar.contiguous()), n, b, float(eta_rel),
+    mu, mubar = mu.contiguous(), mubar.contiguous()   # kept alive until the launch is enqueued
+    check(lib.khx_eig_adjoint_mix(_p(G), _p(mu), _p(mubar), n, b, float(eta_rel),
                                   _stream()), "khx_eig_adjoint_mix")
     return G
 
@@ -372,7 +380,8 @@ def redheffer(SA, SB, out=None, ws=None):
     need = lib.khx_redheffer_ws_bytes(n, B)
     if ws is None or ws.numel() < need:
         ws = _ws(need, dev)
-    check(lib.khx_redheffer(_p(SA.contiguous()), int(ab), _p(SB.contiguous()), int(bb), n, B, _p(S), _p(info),
+    SA, SB = SA.contiguous(), SB.contiguous()   # kept alive until the launch is enqueued
+    check(lib.khx_redheffer(_p(SA), int(ab), _p(SB), int(bb), n, B, _p(S), _p(info),
                             _p(ws), _stream()), "khx_redheffer")
     return S, info
 
@@ -386,11 +395,12 @@ def patterned_layer_bwd(C, IC, kx, ky, k0d, W, V, lam, Sbar, eta_rel=1e-12):
     dev = kx.device
     per_item = 1 if C.dim() == 3 else 0
     C, IC = C.contiguous(), IC.contiguous()
+    W, V, lam, Sbar = W.contiguous(), V.contiguous(), lam.contiguous(), Sbar.contiguous()
     Cbar = torch.empty((B, N, N), dtype=CDT, device=dev)
     info = torch.empty(B, dtype=torch.int32, device=dev)
     ws = _ws(lib.khx_patterned_layer_bwd_ws_bytes(n, B), dev)
-    check(lib.khx_patterned_layer_bwd(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(W.contiguous()),
-                                      _p(V.contiguous()), _p(lam.contiguous()), _p(Sbar.contiguous()),
+    check(lib.khx_patterned_layer_bwd(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(W),
+                                      _p(V), _p(lam), _p(Sbar),
                                       float(eta_rel), _p(Cbar), _p(info), _p(ws), _stream()),
           "khx_patterned_layer_bwd")
     return Cbar, info
@@ -405,7 +415,8 @@ def redheffer_bwd(SA, SB, Sbar):
     SBbar = torch.empty_like(SAbar)
     info = torch.empty(B, dtype=torch.int32, device=dev)
     ws = _ws(lib.khx_redheffer_bwd_ws_bytes(n, B), dev)
-    check(lib.khx_redheffer_bwd(_p(SA.contiguous()), _p(SB.contiguous()), n, B, _p(Sbar.contiguous()), _p(SAbar),
+    SA, SB, Sbar = SA.contiguous(), SB.contiguous(), Sbar.contiguous()
+    check(lib.khx_redheffer_bwd(_p(SA), _p(SB), n, B, _p(Sbar), _p(SAbar),
                                 _p(SBbar), _p(info), _p(ws), _stream()), "khx_redheffer_bwd")
     return SAbar, SBbar, info
 
@@ -587,10 +598,10 @@ def longitudinal(kx, ky, F):
     lib = _lib.load()
     N = kx.shape[-1]
     nz = F.shape[-1]
-    F = F.contiguous()
+    F, kx, ky = F.contiguous(), kx.contiguous(), ky.contiguous()
     uz = torch.empty((N, nz), dtype=CDT, device=F.device)
     w = torch.empty_like(uz)
-    check(lib.khx_longitudinal(_p(kx.contiguous()), _p(ky.contiguous()), _p(F), N, nz, _p(uz), _p(w), _stream()),
+    check(lib.khx_longitudinal(_p(kx), _p(ky), _p(F), N, nz, _p(uz), _p(w), _stream()),
           "khx_longitudinal")
     return uz, w
 
@@ -599,8 +610,8 @@ def idft_phase(kx, ky, k0, x, y):
     """exp(i k0 (Kx x + Ky y)) for normalised K, (npts, N)."""
     lib = _lib.load()
     N = kx.shape[-1]
-    x, y = x.contiguous(), y.contiguous()
+    x, y, kx, ky = x.contiguous(), y.contiguous(), kx.contiguous(), ky.contiguous()
     E = torch.empty((x.shape[0], N), dtype=CDT, device=x.device)
-    check(lib.khx_idft_phase(_p(kx.contiguous()), _p(ky.contiguous()), float(k0), _p(x), _p(y), N, x.shape[0], _p(E),
+    check(lib.khx_idft_phase(_p(kx), _p(ky), float(k0), _p(x), _p(y), N, x.shape[0], _p(E),
                              _stream()), "khx_idft_phase")
     return E

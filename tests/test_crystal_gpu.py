@@ -281,3 +281,33 @@ def test_island_layers_batched_and_mixed(gpu):
     Rb, Tb = CrystalBatch([make(wl) for wl in wls]).solve()
     np.testing.assert_allclose(Rb, [s[0] for s in singles], atol=1e-11)
     np.testing.assert_allclose(Tb, [s[1] for s in singles], atol=1e-11)
+
+
+def test_star_chain_reports_singular_junction(gpu):
+    """A junction whose D = I - SB11 SA22 is exactly singular (SA22 = SB11 = I)
+    is reported through the chain's status words and raises numpy's
+    LinAlgError, as the reference's np.linalg.solve does
+    (khepri/alternative.py:19-30); a regular pair reports nothing."""
+    import torch
+    from types import SimpleNamespace
+    from khepri_amd import ops
+    from khepri_amd.layer import stack_layers
+    n, B = 6, 3
+    eye = torch.eye(n, dtype=torch.complex128, device=gpu)
+    SA = torch.zeros((B, 2, 2, n, n), dtype=torch.complex128, device=gpu)
+    SB = torch.zeros_like(SA)
+    SA[:, 0, 1] = eye
+    SA[:, 1, 0] = eye
+    SA[:, 1, 1] = eye          # SA22 = I
+    SB[:, 0, 1] = eye
+    SB[:, 1, 0] = eye
+    SB[:, 0, 0] = 0.5 * eye    # SB11 = I/2: D = I/2, regular
+    SB[1, 0, 0] = eye          # item 1: SB11 = I, D = 0
+    infos = []
+    stack_layers(None, [SimpleNamespace(S=SA), SimpleNamespace(S=SB)], [True, True], reverse=False, infos=infos)
+    assert len(infos) == 1
+    v = infos[0].cpu().numpy()
+    assert v[0] == 0 and v[2] == 0 and v[1] != 0
+    with pytest.raises(np.linalg.LinAlgError):
+        ops.raise_on_info(infos[0], "star product", lu=True)
+    ops.raise_on_info(infos[0][[0, 2]], "star product", lu=True)

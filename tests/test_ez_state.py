@@ -99,3 +99,19 @@ def test_state_round_trip_unsolved(tmp_path):
     np.testing.assert_array_equal(c2._kp, cl._kp)
     np.testing.assert_array_equal(c2._tm, cl._tm)
     assert torch.equal(d["extra"], extra) and d["again"] is d["extra"]
+
+
+def test_set_source_rejects_ambiguous_kp_layouts():
+    import numpy as np
+    import pytest
+    from khepri_amd import Crystal
+    cl = Crystal((3, 3))
+    kps = np.arange(6.0).reshape(2, 3) * 0.01     # (2, B): the transposed layout
+    with pytest.raises(ValueError):
+        cl.set_source(np.full(3, 1.5), 1.0, 0.0, kp=kps)
+    with pytest.raises(ValueError):
+        cl.set_source(1.5, 1.0, 0.0, kp=np.zeros((2, 3, 2)))
+    cl.set_source(np.full(3, 1.5), 1.0, 0.0, kp=kps.T)   # (B, 2) is accepted
+    assert cl.batch == 3 and np.allclose(cl.kp[:, 0], kps[0]) and np.allclose(cl.kp[:, 1], kps[1])
+    cl.set_source(1.5, 1.0, 0.0, kp=(0.1, 0.2))           # one (kx, ky)
+    assert cl.batch == 1
