@@ -137,6 +137,19 @@ int khx_zgemm(int M, int N, int K, const double* alpha, const double* A, int lda
  * info[b] > 0: QR iteration did not converge.                             */
 size_t khx_eig_ws_bytes(int n, int batch);
 int khx_eig(int n, double* A, int batch, double* lam, double* W, int* info, void* ws, void* stream);
+/* khx_eig that allocates nothing and takes no workspace: the same kernels,
+ * with their scratch (khx_eig_ws_bytes(n, batch) - 256 bytes: the Schur
+ * vectors and triangular eigenvectors, 2 n^2 x 16 bytes per matrix) in a
+ * static device array of khx_eig_noalloc_bytes() = 64 MiB that each GPU the
+ * library runs on holds until the process ends.  No counterpart in the
+ * reference: it is for callers that cannot call hipMalloc (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * Fails (-1) when the batch does not fit the array (n = 162: up to 39
+ * matrices).  Every call on one GPU shares that GPU's array: calls on one
+ * stream are ordered, calls that overlap in time on different streams of
+ * one GPU race.  Nothing in the array needs to start at zero.              */
+int khx_eig_noalloc(int n, double* A, int batch, double* lam, double* W, int* info, void* stream);
+size_t khx_eig_noalloc_bytes(void);
 /* khx_eig that also writes, per matrix, 10 words to tphase (device,
  * batch x 10): wall-clock stamps (100 MHz) at start / after the Hessenberg
  * reduction / after the Schur iteration / after the eigenvectors, the

@@ -26,6 +26,8 @@ SIGNATURES = {
     "khx_eig_ws_bytes": (_sz, [_i, _i]),
     "khx_eig": (_i, [_i, _dp, _i, _dp, _dp, _ip, _vp, _vp]),
     "khx_eig_timed": (_i, [_i, _dp, _i, _dp, _dp, _ip, _vp, _vp, _vp]),
+    "khx_eig_noalloc": (_i, [_i, _dp, _i, _dp, _dp, _ip, _vp]),
+    "khx_eig_noalloc_bytes": (_sz, []),
     "khx_patterned_layer_ws_bytes": (_sz, [_i, _i]),
     "khx_patterned_layer": (_i, [_dp, _dp, _i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _ip, _vp, _vp]),
     "khx_uniform_layer": (_i, [_dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),

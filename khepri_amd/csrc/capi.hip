@@ -197,6 +197,16 @@ int khx_eig(int n, double* A, int batch, double* lam, double* W, int* info, void
                    batch, info, ws);
 }
 
+int khx_eig_noalloc(int n, double* A, int batch, double* lam, double* W, int* info, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && batch >= 0, "khx_eig_noalloc: bad sizes");
+  const long long nn = (long long)n * n;
+  return khx::zeig_noalloc(as_stream(stream), n, khx::Mat{as_cd(A), n, nn}, as_cd(lam), n,
+                           khx::Mat{as_cd(W), n, nn}, batch, info);
+}
+
+size_t khx_eig_noalloc_bytes(void) { return khx::zeig_noalloc_limit(); }
+
 int khx_eig_timed(int n, double* A, int batch, double* lam, double* W, int* info, void* ws,
                   unsigned long long* tphase, void* stream) {
   KHX_ENTRY_BEGIN

@@ -22,6 +22,8 @@
 #include "khx_prof.h"
 
 #include <cstdlib>
+#include <mutex>
+#include <vector>
 
 namespace khx {
 
@@ -1515,6 +1517,50 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   znormalize_cols_kernel<<<dim3(batch), dim3(256), 0, s>>>(n, W.p, W.stride);
   KHX_LAUNCH_CHECK();
   return 0;
+}
+
+// ----------------------------------------------------------------- no-alloc
+// khx_eig for callers that cannot allocate: the workspace of zeig (Zt and X,
+// 2 n^2 complex per matrix) lives in this file-scope device array, one copy
+// per GPU, held until the process ends.  Host code never passes the array's
+// name to a kernel: its device address comes from hipGetSymbolAddress, looked
+// up once per device with that device current.
+constexpr size_t EIG_NOALLOC_BYTES = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[EIG_NOALLOC_BYTES];
+
+size_t zeig_noalloc_limit() { return EIG_NOALLOC_BYTES; }
+
+static int scratch_address(void** out) {
+  static std::mutex mu;
+  static std::vector<void*> ptr;  // per device, nullptr until looked up
+  int dev = 0, ndev = 0;
+  KHX_CHECK_HIP(hipGetDevice(&dev));
+  KHX_CHECK_HIP(hipGetDeviceCount(&ndev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)ptr.size() < ndev) ptr.resize(ndev, nullptr);
+  KHX_REQUIRE(dev >= 0 && dev < ndev, "zeig_noalloc: bad current device");
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return -3;
+    }
+    ptr[dev] = p;
+  }
+  *out = ptr[dev];
+  return 0;
+}
+
+int zeig_noalloc(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info) {
+  if (batch <= 0) return 0;
+  KHX_REQUIRE(n >= 1 && n <= 8192, "khx_eig_noalloc: bad order");
+  KHX_REQUIRE(zeig_workspace_bytes(n, batch) <= EIG_NOALLOC_BYTES,
+              "khx_eig_noalloc: batch x order exceeds the static workspace (2 n^2 x 16 bytes per matrix, at most "
+              "64 MiB)");
+  void* ws = nullptr;
+  KHX_TRY(scratch_address(&ws));
+  return zeig(s, n, A, lam, lstride, W, batch, info, ws);
 }
 
 }  // namespace khx

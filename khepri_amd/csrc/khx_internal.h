@@ -89,6 +89,9 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
          unsigned long long* tphase = nullptr);
 // matrices in flight at once on the current device (a full round of workgroups)
 int zeig_slots(int n);
+// zeig with its workspace in a static device array (no allocation, no ws argument)
+int zeig_noalloc(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info);
+size_t zeig_noalloc_limit();
 
 // Raise a kernel's dynamic-LDS limit to `bytes` when it exceeds the 64 KiB
 // default.  One cache per (device, kernel) shared by every caller; the

@@ -43,3 +43,7 @@ def test_host_only_entry_points():
     assert lib.khx_gemm_tune_set(10, 10, 10, 0, 0, 1, -1) == 0
     assert lib.khx_gemm_record(1, None, 0) == 0
     assert lib.khx_gemm_record(0, None, 0) == 0
+    # the no-allocation eigensolver: a fixed 64 MiB device array, batches beyond it refused on the host
+    assert lib.khx_eig_noalloc_bytes() == 64 << 20
+    assert lib.khx_eig_noalloc(162, None, 80, None, None, None, None) == -1
+    assert b"static workspace" in lib.khx_last_error()
