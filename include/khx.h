@@ -153,7 +153,8 @@ size_t khx_eig_noalloc_bytes(void);
 /* khx_eig that also writes, per matrix, 10 words to tphase (device,
  * batch x 10): wall-clock stamps (100 MHz) at start / after the Hessenberg
  * reduction / after the Schur iteration / after the eigenvectors, the
- * number of QR sweeps, and (H in global memory) the ticks spent in the
+ * number of QR iterations (low 32 bits; the chain passes among them in the
+ * high 32 bits), and (H in global memory) the ticks spent in the
  * deflation scan and shift, the in-block chase, the rest of the blocked
  * sweeps, the rotation log and its flushes -- measurement support only.     */
 int khx_eig_timed(int n, double* A, int batch, double* lam, double* W, int* info, void* ws,
@@ -356,7 +357,8 @@ int khx_gemm_tune_set(int M, int N, int K, int opa, int opb, int batch, int cfg)
 
 /* Measurement support (no counterpart in the reference).
  * khx_prof_enable(1) clears the records and brackets every launch of the
- * instrumented kernels (zeig_schur_kernel, zgesv_kernel, zgemm_kernel) with
+ * instrumented launches (zeig: the eigensolver's kernels of one call;
+ * zgesv_kernel, zgemm_kernel) with
  * a HIP event pair on its stream; khx_prof_collect sums the durations of one
  * kernel (synchronising on its events).  khx_copy is a plain device copy of
  * `bytes` (multiple of 16) used to calibrate HBM byte counters.            */

@@ -41,14 +41,48 @@ constexpr int EIG_QB = EIG_QB_SIZE;  // QR chase block held in LDS when H is in 
 #endif
 constexpr int HB_NB = HB_NB_SIZE;  // panel width of the blocked Hessenberg reduction
 
-// LDS layout: u (n), wv (2n, unblocked Hessenberg only), vh (n), part (the
-// Hessenberg partial sums / T factor, then the QR chase block), srot (n),
-// crot (n doubles), and H itself when n <= KHX_EIG_LDS_N.
+// multishift QR (chain passes, see ms_chain_pass)
+#ifndef MS_NS_SIZE
+#define MS_NS_SIZE 8
+#endif
+#ifndef MS_W_SIZE
+#define MS_W_SIZE 40
+#endif
+constexpr int MS_NS = MS_NS_SIZE;        // bulges (shifts) per chain pass
+constexpr int MS_W = MS_W_SIZE;          // rows / columns of the LDS window
+constexpr int MS_LAG = 2 * (MS_NS - 1);  // the trailing bulge's distance behind the leading one
+constexpr int MS_R = 2 * MS_NS;          // live elements of a line around the chain
+constexpr int MS_PF = 4;                 // loads in flight per line
+constexpr int MS_CH = 32;                // steps of rotation parameters staged per chunk
+constexpr int MS_SLOTS = MS_W * MS_NS;   // rotation slots of one window (at most MS_W steps)
+constexpr int MS_MIN_WL = 3 * MS_NS;     // smaller active blocks take single-shift sweeps
+static_assert(MS_W >= 2 * MS_NS + 3 && MS_W <= 64, "the chain must fit the window, the window a wavefront");
+static_assert(MS_CH * MS_NS <= MS_SLOTS, "staging chunk");
+
+// LDS layout (complex elements): region 1 = u, vh, srot (n each) and crot
+// (n doubles), or the rotation slots of a chain window; wv (2n, unblocked
+// Hessenberg only); part (the Hessenberg partial sums / T factor, the QR
+// chase block or chain window, staged rotation parameters); then H itself
+// when n <= KHX_EIG_LDS_N, else the chain pass's row table and shifts.
+__host__ __device__ __forceinline__ int eig_r1_elems(int n, bool chain) {
+  const int base = 3 * n + (n + 1) / 2;
+  const int slots = chain && n > KHX_EIG_LDS_N ? MS_SLOTS + MS_SLOTS / 2 : 0;
+  return base > slots ? base : slots;
+}
 __host__ __device__ __forceinline__ int eig_wv_elems(int n) { return n <= KHX_EIG_LDS_N ? 2 * n : 0; }
-__host__ __device__ __forceinline__ int eig_part_elems(int n, int waves) {
+__host__ __device__ __forceinline__ int eig_part_elems(int n, int waves, bool chain) {
   if (n <= KHX_EIG_LDS_N) return waves * 2 * n;
-  const int hess = HB_NB * HB_NB + waves * HB_NB + 2 * HB_NB + 2 * 8 * HB_NB;  // T, sums, staged rows
-  return hess > EIG_QB * (EIG_QB + 1) ? hess : EIG_QB * (EIG_QB + 1);  // chase block with padded rows
+  int e = HB_NB * HB_NB + waves * HB_NB + 2 * HB_NB + 2 * 8 * HB_NB;  // T, sums, staged rows
+  if (e < EIG_QB * (EIG_QB + 1)) e = EIG_QB * (EIG_QB + 1);            // single-shift chase block
+  if (chain) {
+    if (e < MS_W * (MS_W + 1)) e = MS_W * (MS_W + 1);                  // chain window
+    if (e < MS_NS * (MS_NS + 1)) e = MS_NS * (MS_NS + 1);              // shift computation
+    if (e < MS_CH * MS_NS + (MS_CH * MS_NS + 1) / 2) e = MS_CH * MS_NS + (MS_CH * MS_NS + 1) / 2;  // staging
+  }
+  return e;
+}
+__host__ __device__ __forceinline__ int eig_tail_elems(int n, bool chain) {
+  return n <= KHX_EIG_LDS_N ? n * n : (chain ? (n + 3) / 4 + MS_NS : 0);
 }
 
 template <int EIG_THREADS>
@@ -77,7 +111,7 @@ __device__ __forceinline__ int block_max_int(int v, int* red) {
 #pragma unroll
   for (int w = 1; w < EIG_WAVES; ++w) t = max(t, red[w]);
   __syncthreads();
-  return t;
+  return __builtin_amdgcn_readfirstlane(t);  // uniform: a scalar register
 }
 
 // 1/sqrt(x) for x in the normal range: the hardware estimate refined by the
@@ -276,7 +310,7 @@ __device__ __forceinline__ void accumulate_rows(const cd* M, int n, int cc, bool
 //   MODE 1:  w = F^H M(r0:, col);  M(r0:, col) -= G w                      (F = Vt, G = V)
 //   MODE 2:  w = F^T M(r0:, col);  M(r0:, col) -= conj(G) w
 template <int EIG_THREADS, int MODE>
-__device__ void trailing_pass(cd* M, int n, int c0, int r0, const cd* F, const cd* G, cd* stage) {
+__device__ __forceinline__ void trailing_pass(cd* M, int n, int c0, int r0, const cd* F, const cd* G, cd* stage) {
 #pragma unroll 1
   for (int cb = c0; cb < n; cb += EIG_THREADS) {
     const int cc = cb + threadIdx.x;
@@ -295,7 +329,7 @@ __device__ void trailing_pass(cd* M, int n, int c0, int r0, const cd* F, const c
 }
 
 template <int EIG_THREADS>
-__device__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, cd* u, cd* vh, cd* part,
+__device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, cd* u, cd* vh, cd* part,
                                    double* red) {
   constexpr int EIG_WAVES = EIG_THREADS / 64;
   static_assert(64 % HB_NB == 0 && EIG_THREADS % HB_NB == 0, "panel width must divide the wavefront");
@@ -487,7 +521,7 @@ static_assert(3 * EIG_KLOG * (EIG_CH + 2 * (EIG_KLOG - 1)) <= 2 * EIG_QB * (EIG_
               "rotation staging does not fit the chase-block LDS region");
 
 template <int EIG_THREADS>
-__device__ void flush_log(cd* H, cd* Zt, int n, int nsw, const int* sl, const int* sh, const double* lc,
+__device__ __forceinline__ void flush_log(cd* H, cd* Zt, int n, int nsw, const int* sl, const int* sh, const double* lc,
                           const cd* ls, cd* stage) {
   constexpr int K = EIG_KLOG, W = 2 * EIG_KLOG, LAG = 2 * (EIG_KLOG - 1), CHP = EIG_CH + LAG;
   cd* st_s = stage;
@@ -601,7 +635,7 @@ __device__ void flush_log(cd* H, cd* Zt, int n, int nsw, const int* sl, const in
 // (the bulge) inside the new block.  Every element receives the same
 // rotations in the same order as an unblocked sweep.
 template <int EIG_THREADS>
-__device__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift, cd* hb, double* crot, cd* srot,
+__device__ __forceinline__ void qr_sweep_blocked(cd* H, cd* Zt, int n, int l, int ihi, cd shift, cd* hb, double* crot, cd* srot,
                                  cd* sxy, bool one_wave, unsigned long long* acc) {
   const int tid = threadIdx.x;
   int kstart = l, b0 = l;
@@ -851,7 +885,7 @@ __device__ __forceinline__ void reg_chase_block(cd* hb, int nb, int b0, int l, i
 // the rotations in sweep order: a block's rest update finishes (barrier)
 // before the next block's rest update or load touches the same lines.
 template <int EIG_THREADS>
-__device__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, double* crot, cd* srot,
+__device__ __forceinline__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, double* crot, cd* srot,
                              unsigned long long* acc) {
   const int tid = threadIdx.x;
   int kstart = l, b0 = l;
@@ -913,158 +947,471 @@ __device__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd shift, cd* hb, dou
   }
 }
 
-// Two shifts per sweep: two bulges, EIG_SEP positions apart (the leading one
-// with shift s0), chased through the same LDS blocks by one wavefront.  Each
-// step computes both rotations, then applies all row rotations (phase R) and
-// all column rotations (phase C): rotations of one kind act on disjoint
-// pairs, and a row and a column rotation commute, so each element is written
-// once per phase.  The block's rotations are applied to the rest of the
-// window bulge by bulge (leading bulge first, which the separation makes
-// equivalent to the step order) and logged per bulge for the deferred pass.
-constexpr int EIG_SEP = 3;
-constexpr int EIG_TWO_MIN_W = 12;  // smaller windows use one shift per sweep
+// ------------------------------------------------------------ multishift QR
+// A chain pass applies MS_NS shifts at once: MS_NS bulges two positions apart
+// chased together (the small-bulge multishift QR of Braman, Byers and
+// Mathias, with complex Givens bulges).  At step t bulge b is at
+// k = l + t - 2b and active while l <= k < ihi; within a step the bulges act
+// on disjoint row pairs and disjoint column pairs, and a row rotation
+// commutes with a column rotation, so the pass produces the matrix of MS_NS
+// single-shift sweeps applied one after the other (up to rounding).
+//
+// The chase runs through LDS windows of MS_W rows / columns on one
+// wavefront: per step lane b computes bulge b's rotation, then lane j applies
+// every bulge's row rotation to window column j and lane i every column
+// rotation to window row i.  Outside the window:
+//   * columns right of the window inside the active block (the next windows
+//     load them) take the window's row rotations right after its chase;
+//   * rows above the window are never read again in the pass, columns right
+//     of the active block and Zt are never read by the chase: they take all
+//     their rotations in one sweep per line at the end of the pass.
+// Per pass each line is read and written once (twice for the columns inside
+// the active block), instead of once per sweep.
+
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Wilkinson shift: the eigenvalue of [[a, b], [c, d]] closest to d.
+__device__ __forceinline__ cd wilkinson_shift(cd a, cd b, cd c, cd d) {
+  const cd bc = b * c;
+  if (bc.x == 0.0 && bc.y == 0.0) return d;
+  const cd tr = 0.5 * (a - d);
+  const cd disc = sqrtc(fmac(bc, tr, tr));
+  const cd p1 = tr + disc, p2 = tr - disc;
+  const cd den = abs2c(p1) >= abs2c(p2) ? p1 : p2;
+  return (den.x == 0.0 && den.y == 0.0) ? d : d - divc(bc, den);
+}
+
+// Shifts of a chain pass (wavefront 0): the eigenvalues of the trailing
+// MS_NS x MS_NS block of H, by single-shift QR of an LDS copy hs (lane j
+// takes column j of a row rotation, lane i row i of a column rotation), or
+// when `exceptional` the block's diagonal moved by 0.75 times its
+// subdiagonal magnitudes (zlaqr0's exceptional shifts).  Without convergence
+// the block's diagonal is used.  sh gets them ordered by decreasing distance
+// from H[ihi][ihi], so the closest is chased last.
+__device__ __forceinline__ void ms_shifts(const cd* H, int n, int ihi, bool exceptional, cd* hs, cd* sh, int lane) {
+  constexpr int m = MS_NS, ld = MS_NS + 1;
+  const int i0 = ihi - m + 1;
+  if (exceptional) {
+    if (lane < m) {
+      const int i = i0 + lane;
+      double sabs = cabs1(H[(long long)i * n + i - 1]);
+      if (i >= 2) sabs += cabs1(H[(long long)(i - 1) * n + i - 2]);
+      hs[lane * ld + lane] = H[(long long)i * n + i] + mkc(0.75 * sabs, 0.0);
+    }
+    wave_sync();
+  } else {
+    for (int e = lane; e < m * m; e += 64) {
+      const int r = e / m, c = e % m;
+      hs[r * ld + c] = (c >= r - 1) ? H[(long long)(i0 + r) * n + i0 + c] : mkc(0.0, 0.0);
+    }
+    wave_sync();
+    const double smlnum = SAFMIN * ((double)m / ULP);
+    int hi = m - 1, its = 0, tot = 0;
+    while (hi >= 1 && tot < 30 * m) {
+      int lk = 0;
+      if (lane >= 1 && lane <= hi && negligible(hs, ld, lane, smlnum)) lk = lane;
+      const int l = __builtin_amdgcn_readfirstlane(wave_max_int(lk));
+      wave_sync();
+      if (lane == l && l > 0) hs[l * ld + l - 1] = mkc(0.0, 0.0);
+      if (l == hi) {
+        --hi;
+        its = 0;
+        wave_sync();
+        continue;
+      }
+      ++its;
+      ++tot;
+      cd shift;
+      if (its % 10 == 0)
+        shift = hs[hi * ld + hi] + mkc(0.75 * fabs(hs[hi * ld + hi - 1].x), 0.0);
+      else
+        shift = wilkinson_shift(hs[(hi - 1) * ld + hi - 1], hs[(hi - 1) * ld + hi], hs[hi * ld + hi - 1],
+                                hs[hi * ld + hi]);
+      cd x = hs[l * ld + l] - shift, y = hs[(l + 1) * ld + l];
+      for (int k = l; k < hi; ++k) {
+        if (k > l) {
+          x = hs[k * ld + k - 1];
+          y = hs[(k + 1) * ld + k - 1];
+        }
+        double c;
+        cd s, r;
+        givens(x, y, c, s, r);
+        wave_sync();  // every lane has read x, y
+        const int j = lane;
+        if (k > l && j == k - 1) {
+          hs[k * ld + j] = r;
+          hs[(k + 1) * ld + j] = mkc(0.0, 0.0);
+        } else if (j >= k && j <= hi) {
+          cd a = hs[k * ld + j], b = hs[(k + 1) * ld + j];
+          rot_rows(a, b, c, s);
+          hs[k * ld + j] = a;
+          hs[(k + 1) * ld + j] = b;
+        }
+        wave_sync();
+        const int i = lane;
+        if (i >= l && i <= min(k + 2, hi)) {
+          cd a = hs[i * ld + k], b = hs[i * ld + k + 1];
+          rot_cols(a, b, c, s);
+          hs[i * ld + k] = a;
+          hs[i * ld + k + 1] = b;
+        }
+        wave_sync();
+      }
+    }
+  }
+  if (lane == 0) {  // selection sort into sh, farthest from H[ihi][ihi] first
+    const cd ref = H[(long long)ihi * n + ihi];
+    for (int i = 0; i < m; ++i) sh[i] = hs[i * ld + i];
+    for (int i = 0; i < m; ++i) {
+      int best = i;
+      double bd = abs2c(sh[i] - ref);
+      for (int j = i + 1; j < m; ++j) {
+        const double d = abs2c(sh[j] - ref);
+        if (d > bd) {
+          bd = d;
+          best = j;
+        }
+      }
+      const cd tmp = sh[i];
+      sh[i] = sh[best];
+      sh[best] = tmp;
+    }
+  }
+  wave_sync();
+}
+
+__device__ __forceinline__ bool ms_active(int l, int ihi, int t, int b, int& k) {
+  k = l + t - 2 * b;
+  return k >= l && k < ihi;
+}
+
+// Chase steps [t0, te) inside the LDS window hb (rows / columns w0 .. w0+nb-1,
+// row stride ld) on wavefront 0; the rotation of bulge b at step t goes to
+// slot (t - t0) MS_NS + b of ps / pc.
+__device__ __forceinline__ void ms_chase(cd* hb, int ld, int nb, int w0, int l, int ihi, int t0, int te, const cd* sh, cd* ps,
+                         double* pc, int lane) {
+#pragma unroll 1
+  for (int t = t0; t < te; ++t) {
+    const int sl = (t - t0) * MS_NS;
+    if (lane < MS_NS) {
+      int k;
+      if (ms_active(l, ihi, t, lane, k)) {
+        const int kk = k - w0;
+        cd x, y;
+        if (k == l) {
+          x = hb[kk * ld + kk] - sh[lane];
+          y = hb[(kk + 1) * ld + kk];
+        } else {
+          x = hb[kk * ld + kk - 1];
+          y = hb[(kk + 1) * ld + kk - 1];
+        }
+        double c;
+        cd s, r;
+        givens(x, y, c, s, r);
+        pc[sl + lane] = c;
+        ps[sl + lane] = s;
+        if (k > l) {  // the bulge this rotation removes
+          hb[kk * ld + kk - 1] = r;
+          hb[(kk + 1) * ld + kk - 1] = mkc(0.0, 0.0);
+        }
+      }
+    }
+    wave_sync();
+    // row rotations: lane = window column; every bulge's operands are loaded
+    // first (inactive ones from a valid dummy address), then rotated, then
+    // stored, so the LDS round trips of the bulges overlap
+    {
+      cd xv[MS_NS], yv[MS_NS];
+      int off[MS_NS];
+      bool on[MS_NS];
+#pragma unroll
+      for (int b = 0; b < MS_NS; ++b) {
+        int k;
+        const bool act = ms_active(l, ihi, t, b, k);
+        const int kk = k - w0;
+        on[b] = act && lane >= kk && lane < nb;
+        off[b] = on[b] ? kk * ld + lane : 0;
+        xv[b] = hb[off[b]];
+        yv[b] = hb[off[b] + ld];
+      }
+#pragma unroll
+      for (int b = 0; b < MS_NS; ++b) rot_rows(xv[b], yv[b], pc[sl + b], ps[sl + b]);
+#pragma unroll
+      for (int b = 0; b < MS_NS; ++b)
+        if (on[b]) {
+          hb[off[b]] = xv[b];
+          hb[off[b] + ld] = yv[b];
+        }
+    }
+    wave_sync();
+    // column rotations: lane = window row
+    {
+      cd xv[MS_NS], yv[MS_NS];
+      int off[MS_NS];
+      bool on[MS_NS];
+#pragma unroll
+      for (int b = 0; b < MS_NS; ++b) {
+        int k;
+        const bool act = ms_active(l, ihi, t, b, k);
+        const int kk = k - w0;
+        on[b] = act && lane <= kk + 2 && lane < nb;
+        off[b] = on[b] ? lane * ld + kk : 0;
+        xv[b] = hb[off[b]];
+        yv[b] = hb[off[b] + 1];
+      }
+#pragma unroll
+      for (int b = 0; b < MS_NS; ++b) rot_cols(xv[b], yv[b], pc[sl + b], ps[sl + b]);
+#pragma unroll
+      for (int b = 0; b < MS_NS; ++b)
+        if (on[b]) {
+          hb[off[b]] = xv[b];
+          hb[off[b] + 1] = yv[b];
+        }
+    }
+    wave_sync();
+  }
+}
+
+// One line v[p * step] (p = matrix index along the line; only p in [lo, hi]
+// is loaded or stored) taking the rotations of a stretch of chain steps:
+// ROWS = row rotations (a column of H), else column rotations (a row of H)
+// or their conjugates (a column of Zt).  Rotation (t, b) acts on the pair
+// (l + t - 2b, l + t - 2b + 1); the MS_R elements around the chain stay in
+// registers (r[i] = element l + t - MS_LAG + i at step t), one element
+// entering and one leaving per step.
+struct MsLine {
+  cd r[MS_R];
+  cd pf[MS_PF];
+};
+
+__device__ __forceinline__ cd ms_ld(const cd* v, long long step, int e, int lo, int hi) {
+  return (e >= lo && e <= hi) ? v[(long long)e * step] : mkc(0.0, 0.0);
+}
+
+__device__ __forceinline__ void ms_line_begin(MsLine& L, const cd* v, long long step, int lo, int hi, int l,
+                                              int ts) {
+  const int b0 = l + ts - MS_LAG;
+#pragma unroll
+  for (int i = 0; i < MS_R - 1; ++i) L.r[i] = ms_ld(v, step, b0 + i, lo, hi);
+  L.r[MS_R - 1] = mkc(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < MS_PF; ++i) L.pf[i] = ms_ld(v, step, l + ts + 1 + i, lo, hi);
+}
+
+// steps [t0, t1); parameters of step t at slot (t - pbase) MS_NS + b of ps / pc
+template <bool ROWS>
+__device__ __forceinline__ void ms_line_steps(MsLine& L, cd* v, long long step, int lo, int hi, int l, int ihi,
+                                              int t0, int t1, const cd* ps, const double* pc, int pbase) {
+#pragma unroll 1
+  for (int t = t0; t < t1; ++t) {
+    L.r[MS_R - 1] = L.pf[0];
+#pragma unroll
+    for (int i = 0; i + 1 < MS_PF; ++i) L.pf[i] = L.pf[i + 1];
+    L.pf[MS_PF - 1] = ms_ld(v, step, l + t + 1 + MS_PF, lo, hi);
+    const int sl = (t - pbase) * MS_NS;
+#pragma unroll
+    for (int b = 0; b < MS_NS; ++b) {
+      const int k = l + t - 2 * b;
+      if (k >= l && k < ihi) {
+        const double c = pc[sl + b];
+        const cd s = ps[sl + b];
+        if (ROWS)
+          rot_rows(L.r[MS_LAG - 2 * b], L.r[MS_LAG - 2 * b + 1], c, s);
+        else
+          rot_cols(L.r[MS_LAG - 2 * b], L.r[MS_LAG - 2 * b + 1], c, s);
+      }
+    }
+    const int eo = l + t - MS_LAG;
+    if (eo >= lo && eo <= hi) v[(long long)eo * step] = L.r[0];
+#pragma unroll
+    for (int i = 0; i + 1 < MS_R; ++i) L.r[i] = L.r[i + 1];
+  }
+}
+
+__device__ __forceinline__ void ms_line_end(const MsLine& L, cd* v, long long step, int lo, int hi, int l, int te) {
+  const int b0 = l + te - MS_LAG;
+#pragma unroll
+  for (int i = 0; i < MS_R - 1; ++i) {
+    const int e = b0 + i;
+    if (e >= lo && e <= hi) v[(long long)e * step] = L.r[i];
+  }
+}
+
+// End of a chain pass: every row of H up to ihi takes the column rotations
+// from the step it left the windows (tleave), every column of Zt the
+// conjugate column rotations and every column right of the active block the
+// row rotations of the whole pass (the log gs / gc, slot t MS_NS + b), staged
+// MS_CH steps at a time in LDS.  One loop per rotation kind (ROWS), so a
+// thread's line state is one register window.
+template <int EIG_THREADS, bool ROWS>
+__device__ __forceinline__ void ms_deferred_kind(cd* H, cd* Zt, int n, int l, int ihi, int T, const int* tleave,
+                                                 const cd* gs, const double* gc, cd* stage) {
+  cd* st_s = stage;
+  double* st_c = reinterpret_cast<double*>(stage + MS_CH * MS_NS);
+  // ROWS: columns ihi+1..n-1 of H; else rows 0..ihi of H, then the columns of Zt
+  const int nlines = ROWS ? n - 1 - ihi : ihi + 1 + n;
+#pragma unroll 1
+  for (int base = 0; base < nlines; base += EIG_THREADS) {
+    const int e = base + threadIdx.x;
+    cd* v = H;
+    long long step = ROWS ? n : 1;
+    int ts = T;
+    if (ROWS) {
+      if (e < nlines) {
+        v = H + ihi + 1 + e;
+        ts = 0;
+      }
+    } else if (e <= ihi) {
+      v = H + (long long)e * n;
+      ts = tleave[e];
+    } else if (e < nlines) {
+      v = Zt + (e - ihi - 1);
+      step = n;
+      ts = 0;
+    }
+    MsLine L;
+    bool started = false;
+#pragma unroll 1
+    for (int c0 = 0; c0 < T; c0 += MS_CH) {
+      const int c1 = min(c0 + MS_CH, T);
+      __syncthreads();  // the previous chunk is consumed
+#pragma unroll 1
+      for (int idx = threadIdx.x; idx < (c1 - c0) * MS_NS; idx += EIG_THREADS) {
+        st_s[idx] = gs[(long long)c0 * MS_NS + idx];
+        st_c[idx] = gc[(long long)c0 * MS_NS + idx];
+      }
+      __syncthreads();
+      if (ts < c1) {
+        if (!started) {
+          ms_line_begin(L, v, step, l, ihi, l, ts);
+          started = true;
+        }
+        ms_line_steps<ROWS>(L, v, step, l, ihi, l, ihi, max(c0, ts), c1, st_s, st_c, c0);
+      }
+    }
+    if (started) ms_line_end(L, v, step, l, ihi, l, T);
+  }
+}
 
 template <int EIG_THREADS>
-__device__ void qr_sweep2_blocked(cd* H, int n, int l, int ihi, cd s0, cd s1, cd* hb, double* crot0, cd* srot0,
-                                  double* crot1, cd* srot1, unsigned long long* acc) {
+__device__ __forceinline__ void ms_deferred(cd* H, cd* Zt, int n, int l, int ihi, int T, const int* tleave,
+                                            const cd* gs, const double* gc, cd* stage) {
+  ms_deferred_kind<EIG_THREADS, false>(H, Zt, n, l, ihi, T, tleave, gs, gc, stage);
+  if (ihi + 1 < n) ms_deferred_kind<EIG_THREADS, true>(H, Zt, n, l, ihi, T, tleave, gs, gc, stage);
+  __syncthreads();
+}
+
+// One chain pass over the active block [l, ihi] with the shifts sh (LDS).
+// hb: the LDS window (MS_W x (MS_W + 1)), ps / pc: the window's rotation
+// slots (LDS), tleave: n ints (LDS), gs / gc: the pass's rotation log
+// (global, (ihi - l + MS_LAG) MS_NS slots).
+template <int EIG_THREADS>
+__device__ __forceinline__ void ms_chain_pass(cd* H, cd* Zt, int n, int l, int ihi, const cd* sh, cd* hb, cd* ps, double* pc,
+                              int* tleave, cd* gs, double* gc, unsigned long long* acc) {
   const int tid = threadIdx.x;
-  const int nsteps = (ihi - l) + EIG_SEP;  // bulge j is at k = l + step - EIG_SEP j, active for l <= k < ihi
-  int st = 0;
-  while (st < nsteps) {
-    const int kt = l + st - EIG_SEP;  // trailing bulge at the first step of the block
-    const int b0 = (kt <= l) ? l : kt - 1;
-    const int b1 = min(b0 + EIG_QB - 1, ihi);
-    const int nb = b1 - b0 + 1;
-    // steps [st, se) fit the block: every active bulge keeps its inputs
-    // (column k - 1, or the window start) and its bulge (row k + 2, or the
-    // window end) inside
-    int se = st;
-    while (se < nsteps) {
+  const int T = (ihi - l) + MS_LAG;  // the trailing bulge's last rotation is at step T - 1
+  for (int i = tid; i <= ihi; i += EIG_THREADS) tleave[i] = i < l ? 0 : T;
+  const unsigned long long t_p0 = acc ? wall_clock64() : 0;
+  int t = 0, w0 = l;
+  while (true) {
+    const int w1 = min(w0 + MS_W - 1, ihi);
+    const int nb = w1 - w0 + 1, ld = nb + 1;
+    // steps [t, te) whose rotations stay inside the window: each active
+    // bulge reads column k - 1 (or starts at k = l) and writes row k + 2 (or
+    // is at its last rotation k = ihi - 1)
+    int te = t;
+    while (te < T && te - t < MS_W) {
       bool ok = true;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int k = l + se - EIG_SEP * j;
-        if (k < l || k >= ihi) continue;
-        if (!(k == l ? b0 == l : k - 1 >= b0)) ok = false;
-        if (!(k + 2 <= b1 || (k + 1 == ihi && b1 == ihi))) ok = false;
+      for (int b = 0; b < MS_NS; ++b) {
+        int k;
+        if (!ms_active(l, ihi, te, b, k)) continue;
+        if (k == l ? k < w0 : k - 1 < w0) ok = false;
+        if (!(k + 2 <= w1 || (k + 1 == ihi && ihi <= w1))) ok = false;
       }
       if (!ok) break;
-      ++se;
+      ++te;
     }
-    if (se == st) break;  // cannot happen (the first step always fits); never loop without progress
-    for (int e = tid; e < nb * nb; e += EIG_THREADS) hb[e] = H[(long long)(b0 + e / nb) * n + b0 + e % nb];
+    __syncthreads();  // tleave, the previous window's column updates
+    for (int e = tid; e < nb * nb; e += EIG_THREADS) {
+      const int r = e / nb, c = e % nb;
+      if (c >= r - 2) hb[r * ld + c] = H[(long long)(w0 + r) * n + w0 + c];
+    }
     __syncthreads();
-    unsigned long long t_c0 = acc ? wall_clock64() : 0;
-    if (tid < 64) {
-      for (int step = st; step < se; ++step) {
-        int kj[2], cs[2], cntR[2], cntC[2];
-        double cj[2];
-        cd sj[2], rj[2];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int k = l + step - EIG_SEP * j;
-          const bool act = (k >= l && k < ihi);
-          kj[j] = k;
-          cd x = mkc(1.0, 0.0), y = mkc(0.0, 0.0);
-          if (act) {
-            const int kk = k - b0;
-            if (k == l) {
-              x = hb[kk * nb + kk] - (j == 0 ? s0 : s1);
-              y = hb[(kk + 1) * nb + kk];
-            } else {
-              x = hb[kk * nb + kk - 1];
-              y = hb[(kk + 1) * nb + kk - 1];
-            }
-          }
-          givens(x, y, cj[j], sj[j], rj[j]);
-          cs[j] = (k > l) ? k - 1 : k;
-          cntR[j] = act ? b1 - cs[j] + 1 : 0;
-          cntC[j] = act ? min(k + 2, b1) - b0 + 1 : 0;
-          if (act && tid == 0) {
-            if (j == 0) {
-              crot0[k] = cj[0];
-              srot0[k] = sj[0];
-            } else {
-              crot1[k] = cj[1];
-              srot1[k] = sj[1];
-            }
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // phase R: rows (k, k+1) of every active bulge, columns cs .. b1
-        for (int e = tid; e < cntR[0] + cntR[1]; e += 64) {
-          const int j = (e < cntR[0]) ? 0 : 1;
-          const int col = cs[j] + (j == 0 ? e : e - cntR[0]);
-          const int kk = kj[j] - b0, cc = col - b0;
-          if (col == kj[j] - 1) {  // the annihilated bulge entry
-            hb[kk * nb + cc] = rj[j];
-            hb[(kk + 1) * nb + cc] = mkc(0.0, 0.0);
-          } else {
-            cd xk = hb[kk * nb + cc], yk = hb[(kk + 1) * nb + cc];
-            rot_rows(xk, yk, cj[j], sj[j]);
-            hb[kk * nb + cc] = xk;
-            hb[(kk + 1) * nb + cc] = yk;
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // phase C: columns (k, k+1) of every active bulge, rows b0 .. min(k+2, b1)
-        for (int e = tid; e < cntC[0] + cntC[1]; e += 64) {
-          const int j = (e < cntC[0]) ? 0 : 1;
-          const int row = b0 + (j == 0 ? e : e - cntC[0]);
-          const int kk = kj[j] - b0;
-          cd* pr = hb + (row - b0) * nb + kk;
-          cd xk = pr[0], yk = pr[1];
-          rot_cols(xk, yk, cj[j], sj[j]);
-          pr[0] = xk;
-          pr[1] = yk;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
+    const unsigned long long t_c0 = acc ? wall_clock64() : 0;
+    if (tid < 64 && te > t) ms_chase(hb, ld, nb, w0, l, ihi, t, te, sh, ps, pc, tid);
     __syncthreads();
     if (acc) acc[1] += wall_clock64() - t_c0;
-    for (int e = tid; e < nb * nb; e += EIG_THREADS) H[(long long)(b0 + e / nb) * n + b0 + e % nb] = hb[e];
-    // the block's rotations on the rest of the window, leading bulge first
-    const int nabove = b0 - l, nright = ihi - b1;
-    const int ka0 = max(l, l + st), kb0 = min(ihi - 1, l + se - 1);
-    const int ka1 = max(l, l + st - EIG_SEP), kb1 = min(ihi - 1, l + se - 1 - EIG_SEP);
-    for (int e = tid; e < nabove + nright; e += EIG_THREADS) {
-      if (e < nabove) {
-        cd* v = H + (long long)(l + e) * n;
-        if (ka0 <= kb0) slide<false>(v, 1, ka0, kb0 + 1, crot0, srot0);
-        if (ka1 <= kb1) slide<false>(v, 1, ka1, kb1 + 1, crot1, srot1);
-      } else {
-        cd* v = H + b1 + 1 + (e - nabove);
-        if (ka0 <= kb0) slide<true>(v, n, ka0, kb0 + 1, crot0, srot0);
-        if (ka1 <= kb1) slide<true>(v, n, ka1, kb1 + 1, crot1, srot1);
-      }
+    if (te == t) break;  // cannot happen (the chain always fits a window of MS_W >= 2 MS_NS + 3)
+    for (int e = tid; e < nb * nb; e += EIG_THREADS) {
+      const int r = e / nb, c = e % nb;
+      if (c >= r - 2) H[(long long)(w0 + r) * n + w0 + c] = hb[r * ld + c];
     }
-    __syncthreads();
-    st = se;
+    // columns w1+1..ihi (rows w0..w1) take this window's row rotations; the
+    // window's slots go to the pass log
+    for (int j = w1 + 1 + tid; j <= ihi; j += EIG_THREADS) {
+      MsLine L;
+      ms_line_begin(L, H + j, n, w0, w1, l, t);
+      ms_line_steps<true>(L, H + j, n, w0, w1, l, ihi, t, te, ps, pc, t);
+      ms_line_end(L, H + j, n, w0, w1, l, te);
+    }
+    for (int idx = tid; idx < (te - t) * MS_NS; idx += EIG_THREADS) {
+      gs[(long long)t * MS_NS + idx] = ps[idx];
+      gc[(long long)t * MS_NS + idx] = pc[idx];
+    }
+    t = te;
+    if (t >= T) break;
+    // next window: from the trailing bulge's input column, or from l while
+    // bulges are still entering; rows above it have left the windows
+    const int w0n = (t <= MS_LAG) ? l : max(l, l + t - MS_LAG - 1);
+    for (int i = w0 + tid; i < w0n; i += EIG_THREADS) tleave[i] = t;
+    w0 = w0n;
   }
+  __syncthreads();
+  const unsigned long long t_d0 = acc ? wall_clock64() : 0;
+  if (acc) acc[2] += t_d0 - t_p0;
+  ms_deferred<EIG_THREADS>(H, Zt, n, l, ihi, T, tleave, gs, gc, hb);
+  if (acc) acc[4] += wall_clock64() - t_d0;
 }
 
 // Stages 1-3.  A (n x n) becomes T (upper triangle); Zt gets the transposed
 // Schur vectors; X the eigenvectors of T (upper triangular, unit diagonal);
-// lam the eigenvalues.
-template <int EIG_THREADS>
-__global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREADS : 1) void zeig_schur_kernel(int n, cd* A, long long sA, cd* Ztg, cd* Xg,
-                                                                 long long sW, cd* lam, long long sL, int* info,
-                                                                 unsigned long long* tphase, int flags) {
+// lam the eigenvalues.  STAGES = 7: all three in one workgroup with H in LDS
+// (n <= KHX_EIG_LDS_N); otherwise one stage (1 Hessenberg, 2 Schur form,
+// 4 eigenvectors) per launch with H in global memory, so that each stage's
+// kernel gets the registers it needs (the chain passes of the Schur stage
+// would spill inside a kernel that also holds the other stages).
+template <int EIG_THREADS, int STAGES, bool CHAIN = false>
+__device__ __forceinline__ void eig_body(int n, cd* A, long long sA, cd* Ztg, cd* Xg, long long sW, cd* lam,
+                                         long long sL, int* info, unsigned long long* tphase, int flags) {
   constexpr int EIG_WAVES = EIG_THREADS / 64;
   extern __shared__ cd sm[];
-  cd* u = sm;                         // n     Householder vector / solved row
-  cd* wv = u + n;                     // 2n    u^H H (columns of H) and u^T Zt (n <= KHX_EIG_LDS_N)
-  cd* vh = wv + eig_wv_elems(n);      // n     H u
-  cd* part = vh + n;                  // waves*2n  partial sums
-  cd* srot = part + eig_part_elems(n, EIG_WAVES);  // n rotation sines (part doubles as the QR chase block)
-  double* crot = reinterpret_cast<double*>(srot + n);  // n cosines
-  cd* hl = srot + n + (n + 1) / 2;                      // n*n LDS copy of H (small n)
-  cd* srot2 = hl;                                       // second bulge's rotations (n > KHX_EIG_LDS_N)
-  double* crot2 = reinterpret_cast<double*>(srot2 + n);
+  // region 1: u, vh, srot, crot (Hessenberg, single-shift sweeps,
+  // eigenvectors) or, in a chain pass, the window's rotation slots ms_ps /
+  // ms_pc; then wv (small n), part (Hessenberg sums / T, the QR windows, the
+  // staged rotation parameters), and H itself (small n) or the chain pass's
+  // per-row step tables and shifts.
+  cd* u = sm;                                           // n     Householder vector / solved row
+  cd* vh = u + n;                                       // n     H u
+  cd* srot = vh + n;                                    // n     rotation sines
+  double* crot = reinterpret_cast<double*>(srot + n);   // n     cosines
+  cd* ms_ps = sm;                                       // MS_SLOTS sines of a chain window
+  double* ms_pc = reinterpret_cast<double*>(sm + MS_SLOTS);  // MS_SLOTS cosines
+  cd* wv = sm + eig_r1_elems(n, CHAIN);                 // 2n    u^H H and u^T Zt (n <= KHX_EIG_LDS_N)
+  cd* part = wv + eig_wv_elems(n);                      // partial sums, QR windows, staged parameters
+  cd* hl = part + eig_part_elems(n, EIG_WAVES, CHAIN);  // n*n LDS copy of H (small n)
+  int* ms_tl = reinterpret_cast<int*>(hl);              // n     step each row leaves the chain windows
+  cd* ms_sh = hl + (n + 3) / 4;                         // MS_NS shifts of a chain pass
   __shared__ double red[EIG_WAVES];
   __shared__ int redi[EIG_WAVES];
   __shared__ cd s_xy[2][2];  // double-buffered inputs of the next rotation
@@ -1075,13 +1422,14 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   cd* X = Xg + item * sW;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const double smlnum = SAFMIN * ((double)n / ULP);
-  const bool in_lds = n <= KHX_EIG_LDS_N;
+  constexpr bool in_lds = STAGES == 7;
   cd* H = in_lds ? hl : Hg;
   // optional phase timestamps (wall clock): start, Hessenberg, Schur, eigenvectors, sweeps
   unsigned long long* tp = tphase ? tphase + item * 10 : nullptr;
   unsigned long long acc_t[5] = {0, 0, 0, 0, 0};  // scan, chase, block rest, log, flush
-  if (tp && tid == 0) tp[0] = wall_clock64();
 
+  if (STAGES & 1) {
+  if (tp && tid == 0) tp[0] = wall_clock64();
   for (int i = wid; i < n; i += EIG_WAVES)
     for (int j = lane; j < n; j += 64) {
       Zt[i * n + j] = mkc(i == j ? 1.0 : 0.0, 0.0);
@@ -1166,7 +1514,9 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   }
 
   if (tp && tid == 0) tp[1] = wall_clock64();
+  }  // stage 1
   // ------------------------------------------------------------ 2. Schur form
+  if (STAGES & 2) {
   int ihi = n - 1, its = 0, itot = 0, fail = 0;
   // rotation log of the sweeps not yet applied outside their windows (X is
   // free until the eigenvector stage)
@@ -1175,7 +1525,11 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   cd* log_s = X;
   double* log_c = reinterpret_cast<double*>(X + (long long)EIG_KLOG * n);
   const int itmax = 30 * max(10, n);
-  const bool two_bulge = !in_lds && (flags & 2) != 0;
+  // chain passes (multishift) in the rest of the log area of X
+  cd* ms_gs = X + 10ll * n;
+  double* ms_gc = reinterpret_cast<double*>(ms_gs + (long long)(n + MS_LAG) * MS_NS);
+  constexpr bool multishift = !in_lds && CHAIN;
+  int npass = 0;  // chain passes among the itot iterations
   while (ihi >= 1) {
     const unsigned long long t_s0 = tp ? wall_clock64() : 0;
     int lk = 0;
@@ -1196,27 +1550,31 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       break;
     }
     ++its;
-    cd shift, shift2 = mkc(0.0, 0.0);
-    bool two = false;  // two-bulge sweep with both eigenvalues of the trailing 2x2
+    if (multishift && ihi - l + 1 >= MS_MIN_WL) {
+      unsigned long long t1 = tp ? wall_clock64() : 0;
+      if (tp) acc_t[0] += t1 - t_s0;
+      if (nlog > 0) {  // the pass updates every line directly: logged sweeps go first
+        flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s, part);
+        nlog = 0;
+      }
+      unsigned long long t2 = tp ? wall_clock64() : 0;
+      if (tp) acc_t[4] += t2 - t1;
+      if (tid < 64) ms_shifts(H, n, ihi, its % 6 == 0, part, ms_sh, lane);
+      __syncthreads();
+      if (tp) acc_t[3] += wall_clock64() - t2;
+      ms_chain_pass<EIG_THREADS>(H, Zt, n, l, ihi, ms_sh, part, ms_ps, ms_pc, ms_tl, ms_gs, ms_gc,
+                                 tp ? acc_t : nullptr);
+      ++npass;
+      continue;
+    }
+    cd shift;
     if (its % 20 == 10) {  // exceptional shifts break cycles
       shift = H[l * n + l] + mkc(0.75 * fabs(H[(l + 1) * n + l].x), 0.0);
     } else if (its % 20 == 0) {
       shift = H[ihi * n + ihi] + mkc(0.75 * fabs(H[ihi * n + ihi - 1].x), 0.0);
     } else {  // Wilkinson: eigenvalue of the trailing 2x2 closest to H[ihi][ihi]
-      const cd a = H[(ihi - 1) * n + ihi - 1], b = H[(ihi - 1) * n + ihi];
-      const cd c = H[ihi * n + ihi - 1], d = H[ihi * n + ihi];
-      const cd bc = b * c;
-      if (bc.x == 0.0 && bc.y == 0.0) {
-        shift = d;
-      } else {
-        const cd tr = 0.5 * (a - d);
-        const cd disc = sqrtc(fmac(bc, tr, tr));
-        const cd p1 = tr + disc, p2 = tr - disc;
-        const cd den = abs2c(p1) >= abs2c(p2) ? p1 : p2;
-        shift = (den.x == 0.0 && den.y == 0.0) ? d : d - divc(bc, den);
-      }
-      shift2 = (a + d) - shift;  // the other eigenvalue of the trailing 2x2
-      two = two_bulge && (ihi - l + 1 >= EIG_TWO_MIN_W);
+      shift = wilkinson_shift(H[(ihi - 1) * n + ihi - 1], H[(ihi - 1) * n + ihi], H[ihi * n + ihi - 1],
+                              H[ihi * n + ihi]);
     }
     if (!in_lds) {
       unsigned long long t1 = tp ? wall_clock64() : 0;
@@ -1228,14 +1586,7 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       }
       unsigned long long t2 = tp ? wall_clock64() : 0;
       if (tp) acc_t[4] += t2 - t1;
-      if (two && nlog > EIG_KLOG - 2) {  // room for both bulges' rotations in the log
-        flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s, part);
-        nlog = 0;
-      }
-      if (two)
-        qr_sweep2_blocked<EIG_THREADS>(H, n, l, ihi, shift, shift2, part, crot, srot, crot2, srot2,
-                                       tp ? acc_t : nullptr);
-      else if (flags & 4)
+      if (flags & 4)
         qr_sweep_reg<EIG_THREADS>(H, n, l, ihi, shift, part, crot, srot, tp ? acc_t : nullptr);
       else
         qr_sweep_blocked<EIG_THREADS>(H, Zt, n, l, ihi, shift, part, crot, srot, &s_xy[0][0], (flags & 1) != 0,
@@ -1245,23 +1596,14 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
       for (int k = l + tid; k < ihi; k += EIG_THREADS) {
         log_c[nlog * n + k] = crot[k];
         log_s[nlog * n + k] = srot[k];
-        if (two) {
-          log_c[(nlog + 1) * n + k] = crot2[k];
-          log_s[(nlog + 1) * n + k] = srot2[k];
-        }
       }
       if (tid == 0) {
         s_lo[nlog] = l;
         s_hi[nlog] = ihi;
-        if (two) {
-          s_lo[nlog + 1] = l;
-          s_hi[nlog + 1] = ihi;
-        }
       }
       __syncthreads();
       unsigned long long t4 = tp ? wall_clock64() : 0;
       if (tp) acc_t[3] += t4 - t3;
-      nlog += two ? 1 : 0;
       if (++nlog == EIG_KLOG) {
         flush_log<EIG_THREADS>(H, Zt, n, nlog, s_lo, s_hi, log_c, log_s, part);
         nlog = 0;
@@ -1342,7 +1684,7 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
   if (tid == 0 && info) info[item] = fail;
   if (tp && tid == 0) {
     tp[2] = wall_clock64();
-    tp[4] = (unsigned long long)itot;
+    tp[4] = (unsigned long long)itot | ((unsigned long long)npass << 32);
     tp[5] = acc_t[0];
     tp[6] = acc_t[1];
     tp[7] = acc_t[2] - acc_t[1];  // block load / store / in-window slides
@@ -1352,8 +1694,10 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
 
   cd* lm = lam + item * sL;
   for (int i = tid; i < n; i += EIG_THREADS) lm[i] = H[i * n + i];
+  }  // stage 2
 
   // ------------------------------------------------------------ 3. eigenvectors of T
+  if (STAGES & 4) {
   // X[i][k] = -T[i][k] (i < k), 1 (i == k), 0 (i > k)
   for (int i = wid; i < n; i += EIG_WAVES)
     for (int k = lane; k < n; k += 64) {
@@ -1385,6 +1729,43 @@ __global__ __launch_bounds__(EIG_THREADS, EIG_THREADS >= 256 ? 1024 / EIG_THREAD
     for (int i = wid; i < n; i += EIG_WAVES)
       for (int j = lane; j < n; j += 64) Hg[(long long)i * n + j] = hl[i * n + j];
   if (tp && tid == 0) tp[3] = wall_clock64();
+  }  // stage 3
+}
+
+#define KHX_EIG_BOUNDS(T) __launch_bounds__(T, T >= 256 ? 1024 / T : 1)
+#define KHX_EIG_ARGS                                                                                   \
+  int n, cd *A, long long sA, cd *Ztg, cd *Xg, long long sW, cd *lam, long long sL, int *info,        \
+      unsigned long long *tphase, int flags
+// n <= KHX_EIG_LDS_N: the whole decomposition in LDS
+template <int EIG_THREADS>
+__global__ KHX_EIG_BOUNDS(EIG_THREADS) void zeig_schur_kernel(KHX_EIG_ARGS) {
+  eig_body<EIG_THREADS, 7>(n, A, sA, Ztg, Xg, sW, lam, sL, info, tphase, flags);
+}
+// n > KHX_EIG_LDS_N: one stage per launch
+template <int EIG_THREADS>
+__global__ KHX_EIG_BOUNDS(EIG_THREADS) void zeig_hess_kernel(KHX_EIG_ARGS) {
+  eig_body<EIG_THREADS, 1>(n, A, sA, Ztg, Xg, sW, lam, sL, info, tphase, flags);
+}
+// The Schur stage: single-shift sweeps (CHAIN = false, four 256-thread
+// workgroups per CU), or chain passes of MS_NS shifts on large active blocks
+// (CHAIN = true: they need more registers than four workgroups per CU
+// leave, QRW = workgroups per CU the kernel is compiled for).
+template <int EIG_THREADS, int QRW, bool CHAIN>
+__global__ __launch_bounds__(EIG_THREADS, QRW) void zeig_qr_kernel(KHX_EIG_ARGS) {
+  eig_body<EIG_THREADS, 2, CHAIN>(n, A, sA, Ztg, Xg, sW, lam, sL, info, tphase, flags);
+}
+// the default launches (single-shift sweeps): four waves per SIMD, i.e. at
+// most 128 registers (256 threads: four workgroups per CU; 128 threads:
+// eight, when the LDS allows)
+__global__ __launch_bounds__(256, 4) __attribute__((amdgpu_num_vgpr(128))) void zeig_qr256_kernel(KHX_EIG_ARGS) {
+  eig_body<256, 2, false>(n, A, sA, Ztg, Xg, sW, lam, sL, info, tphase, flags);
+}
+__global__ __launch_bounds__(128, 4) __attribute__((amdgpu_num_vgpr(128))) void zeig_qr128_kernel(KHX_EIG_ARGS) {
+  eig_body<128, 2, false>(n, A, sA, Ztg, Xg, sW, lam, sL, info, tphase, flags);
+}
+template <int EIG_THREADS>
+__global__ KHX_EIG_BOUNDS(EIG_THREADS) void zeig_vec_kernel(KHX_EIG_ARGS) {
+  eig_body<EIG_THREADS, 4>(n, A, sA, Ztg, Xg, sW, lam, sL, info, tphase, flags);
 }
 
 // Column normalisation of W: unit 2-norm, largest |component| made real.
@@ -1416,12 +1797,70 @@ size_t zeig_workspace_bytes(int n, int batch) {
   return 2ull * (size_t)n * n * sizeof(cd) * (size_t)batch;  // Zt and X
 }
 
-static size_t eig_lds_bytes(int n, int threads) {
-  size_t b = (size_t)(3 * n + eig_wv_elems(n)) * sizeof(cd) + (size_t)eig_part_elems(n, threads / 64) * sizeof(cd) +
-             (size_t)((n + 1) / 2) * sizeof(cd);
-  if (n > KHX_EIG_LDS_N) b += (size_t)(n + (n + 1) / 2) * sizeof(cd);  // rotations of the second bulge
-  if (n <= KHX_EIG_LDS_N) b += (size_t)n * n * sizeof(cd);
-  return b;
+static size_t eig_lds_bytes(int n, int threads, bool chain = false) {
+  return (size_t)(eig_r1_elems(n, chain) + eig_wv_elems(n) + eig_part_elems(n, threads / 64, chain) +
+                  eig_tail_elems(n, chain)) *
+         sizeof(cd);
+}
+static bool eig_chain_on() {
+  const char* ms = getenv("KHX_EIG_MULTISHIFT");
+  return ms && atoi(ms) != 0;
+}
+
+// The Schur-stage kernel for T threads: single-shift sweeps by default;
+// KHX_EIG_MULTISHIFT=1 selects the chain passes (256 threads: at
+// KHX_EIG_QR_WG workgroups per CU, default 2).
+template <int T>
+static const void* eig_qr_fn() {
+  constexpr int W = T >= 256 ? 1024 / T : 1;
+  const char* ms = getenv("KHX_EIG_MULTISHIFT");
+  if (!(ms && atoi(ms) != 0))
+    return T == 256   ? (const void*)zeig_qr256_kernel
+           : T == 128 ? (const void*)zeig_qr128_kernel
+                      : (const void*)zeig_qr_kernel<T, W, false>;
+  if (T != 256) return (const void*)zeig_qr_kernel<T, (T > 256 ? 1024 / T : 1), true>;
+  const char* env = getenv("KHX_EIG_QR_WG");
+  const int w = env ? atoi(env) : 2;
+  return w == 4 ? (const void*)zeig_qr_kernel<256, 4, true>
+         : w == 3 ? (const void*)zeig_qr_kernel<256, 3, true>
+                  : (const void*)zeig_qr_kernel<256, 2, true>;
+}
+
+// Threads per matrix in the Schur stage: those of the other stages, or 128
+// with KHX_EIG_QR_THREADS=128 (single-shift sweeps only).
+static int eig_qr_threads(int T) {
+  const char* env = getenv("KHX_EIG_QR_THREADS");
+  const char* ms = getenv("KHX_EIG_MULTISHIFT");
+  if (env && atoi(env) == 128 && !(ms && atoi(ms) != 0)) return 128;
+  return T;
+}
+
+// One workgroup per matrix: n <= KHX_EIG_LDS_N in one launch, else the
+// Hessenberg, Schur and eigenvector stages in three launches.
+template <int T>
+static int launch_eig(hipStream_t s, int n, int batch, size_t lds, Mat A, cd* Zt, cd* X, long long sZ, cd* lam,
+                      long long lstride, int* info, unsigned long long* tphase, int flags) {
+  if (n <= KHX_EIG_LDS_N) {
+    KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<T>, lds));
+    zeig_schur_kernel<T><<<dim3(batch), dim3(T), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase,
+                                                           flags);
+    KHX_LAUNCH_CHECK();
+    return 0;
+  }
+  const int tq = eig_qr_threads(T);
+  const void* qr = tq == 128 ? eig_qr_fn<128>() : eig_qr_fn<T>();
+  const size_t pad = lds - eig_lds_bytes(n, T);  // KHX_EIG_LDS_PAD
+  const size_t ldsq = eig_lds_bytes(n, tq, eig_chain_on()) + pad;
+  KHX_TRY(ensure_dynamic_lds((const void*)zeig_hess_kernel<T>, lds));
+  KHX_TRY(ensure_dynamic_lds(qr, ldsq));
+  KHX_TRY(ensure_dynamic_lds((const void*)zeig_vec_kernel<T>, lds));
+  zeig_hess_kernel<T><<<dim3(batch), dim3(T), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase, flags);
+  KHX_LAUNCH_CHECK();
+  void* args[] = {&n, &A.p, &A.stride, &Zt, &X, &sZ, &lam, &lstride, &info, &tphase, &flags};
+  KHX_CHECK_HIP(hipLaunchKernel(qr, dim3(batch), dim3(tq), args, ldsq, s));
+  zeig_vec_kernel<T><<<dim3(batch), dim3(T), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase, flags);
+  KHX_LAUNCH_CHECK();
+  return 0;
 }
 
 // Matrices the eigensolver keeps in flight at once on the current device
@@ -1432,13 +1871,25 @@ int zeig_slots(int n) {
   const char* env = getenv("KHX_EIG_THREADS");
   const int threads = env ? atoi(env) : EIG_THREADS_DEFAULT;
   const char* pad_env = getenv("KHX_EIG_LDS_PAD");
-  const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
+  const size_t lds = eig_lds_bytes(n, threads, n > KHX_EIG_LDS_N && eig_chain_on()) + (pad_env ? (size_t)atol(pad_env) : 0);
   KHX_REQUIRE(lds <= 150 * 1024, "zeig_slots: n too large for the LDS work vectors");
-  const void* fn = threads == 1024 ? (const void*)zeig_schur_kernel<1024>
-                   : threads == 512 ? (const void*)zeig_schur_kernel<512>
-                   : threads == 128 ? (const void*)zeig_schur_kernel<128>
-                   : threads == 64  ? (const void*)zeig_schur_kernel<64>
-                                    : (const void*)zeig_schur_kernel<256>;
+  // the Schur stage (the longest) decides how many matrices are in flight
+  const bool small = n <= KHX_EIG_LDS_N;
+  if (!small && eig_qr_threads(threads) == 128) {
+    const void* f = eig_qr_fn<128>();
+    const size_t l = eig_lds_bytes(n, 128, false);
+    KHX_TRY(ensure_dynamic_lds(f, l));
+    int per_cu = 0, dev = 0, cus = 0;
+    KHX_CHECK_HIP(hipGetDevice(&dev));
+    KHX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    KHX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 128, l));
+    return per_cu * cus > 0 ? per_cu * cus : 1;
+  }
+  const void* fn = threads == 1024 ? (small ? (const void*)zeig_schur_kernel<1024> : eig_qr_fn<1024>())
+                   : threads == 512 ? (small ? (const void*)zeig_schur_kernel<512> : eig_qr_fn<512>())
+                   : threads == 128 ? (small ? (const void*)zeig_schur_kernel<128> : eig_qr_fn<128>())
+                   : threads == 64  ? (small ? (const void*)zeig_schur_kernel<64> : eig_qr_fn<64>())
+                                    : (small ? (const void*)zeig_schur_kernel<256> : eig_qr_fn<256>());
   KHX_TRY(ensure_dynamic_lds(fn, lds));
   int per_cu = 0, dev = 0, cus = 0;
   KHX_CHECK_HIP(hipGetDevice(&dev));
@@ -1478,39 +1929,20 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   const char* pad_env = getenv("KHX_EIG_LDS_PAD");
   // bit 0: chase each LDS block on one wavefront (default on; KHX_EIG_CHASE_WAVE=0 turns it off)
   const char* cw_env = getenv("KHX_EIG_CHASE_WAVE");
-  // bit 1: two shifts (two bulges) per sweep (KHX_EIG_TWO_SHIFT=1; off by default: on the layer
-  // eigenproblems it needs about 1.5x more shifts than single-shift sweeps, which outweighs the
-  // shared block overheads)
-  const char* ts_env = getenv("KHX_EIG_TWO_SHIFT");
   // bit 2: register chase (qr_sweep_reg; default on, KHX_EIG_REG_CHASE=0 selects the LDS chase)
   const char* rc_env = getenv("KHX_EIG_REG_CHASE");
-  const int flags = ((cw_env && atoi(cw_env) == 0) ? 0 : 1) | ((ts_env && atoi(ts_env) != 0) ? 2 : 0) |
-                    ((rc_env && atoi(rc_env) == 0) ? 0 : 4);
+  const int flags = ((cw_env && atoi(cw_env) == 0) ? 0 : 1) | ((rc_env && atoi(rc_env) == 0) ? 0 : 4);
   const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
   KHX_REQUIRE(lds <= 150 * 1024, "zeig: n too large for the LDS work vectors");
   {
-    ProfScope ps("zeig_schur_kernel", s);
-    if (threads == 1024) {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<1024>, lds));
-      zeig_schur_kernel<1024><<<dim3(batch), dim3(1024), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
-                                                                   tphase, flags);
-    } else if (threads == 512) {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<512>, lds));
-      zeig_schur_kernel<512><<<dim3(batch), dim3(512), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
-                                                                 tphase, flags);
-    } else if (threads == 64) {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<64>, lds));
-      zeig_schur_kernel<64><<<dim3(batch), dim3(64), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase, flags);
-    } else if (threads == 128) {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<128>, lds));
-      zeig_schur_kernel<128><<<dim3(batch), dim3(128), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
-                                                                 tphase, flags);
-    } else {
-      KHX_TRY(ensure_dynamic_lds((const void*)zeig_schur_kernel<256>, lds));
-      zeig_schur_kernel<256><<<dim3(batch), dim3(256), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info,
-                                                                 tphase, flags);
+    ProfScope ps("zeig", s);
+    switch (threads) {
+      case 1024: KHX_TRY(launch_eig<1024>(s, n, batch, lds, A, Zt, X, sZ, lam, lstride, info, tphase, flags)); break;
+      case 512: KHX_TRY(launch_eig<512>(s, n, batch, lds, A, Zt, X, sZ, lam, lstride, info, tphase, flags)); break;
+      case 128: KHX_TRY(launch_eig<128>(s, n, batch, lds, A, Zt, X, sZ, lam, lstride, info, tphase, flags)); break;
+      case 64: KHX_TRY(launch_eig<64>(s, n, batch, lds, A, Zt, X, sZ, lam, lstride, info, tphase, flags)); break;
+      default: KHX_TRY(launch_eig<256>(s, n, batch, lds, A, Zt, X, sZ, lam, lstride, info, tphase, flags)); break;
     }
-    KHX_LAUNCH_CHECK();
   }
   // W = Z X = Zt^T X
   KHX_TRY(zgemm_ta(s, n, n, n, mkc(1.0, 0.0), Mat{Zt, n, sZ}, Mat{X, n, sZ}, mkc(0.0, 0.0), W, batch));

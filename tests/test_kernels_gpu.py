@@ -334,7 +334,9 @@ def test_convmat_dft_variants(gpu, monkeypatch, direct, shape, pw):
 
 
 EIG_VARIANTS = [
-    {"KHX_EIG_TWO_SHIFT": "1", "KHX_EIG_REG_CHASE": "0"},
+    {"KHX_EIG_MULTISHIFT": "1"}, {"KHX_EIG_MULTISHIFT": "1", "KHX_EIG_QR_WG": "3"},
+    {"KHX_EIG_MULTISHIFT": "1", "KHX_EIG_QR_WG": "4"}, {"KHX_EIG_MULTISHIFT": "1", "KHX_EIG_THREADS": "512"},
+    {"KHX_EIG_REG_CHASE": "0"}, {"KHX_EIG_QR_THREADS": "128"},
     {"KHX_EIG_CHASE_WAVE": "0", "KHX_EIG_REG_CHASE": "0"},
     {"KHX_EIG_THREADS": "64"}, {"KHX_EIG_THREADS": "128"}, {"KHX_EIG_THREADS": "512"},
     {"KHX_EIG_THREADS": "1024"}, {"KHX_EIG_LDS_PAD": "8192"},
