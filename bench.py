@@ -37,6 +37,134 @@ def eig_flops(n):
     return 106.0 * n ** 3
 
 
+def counted_flops_per_solve(n):
+    """Real flops this build's own op sequence spends per c2 solve (one unique
+    patterned layer, the R/T-only chain), complex multiply-add = 8 flops:
+    P Q (two N x N x n GEMMs) 4 n^3; eigensolver 106 n^3 and its
+    back-transform W = Z X 8 n^3; V = Q W (two GEMMs) 4 n^3; W^-1 and V^-1 V0
+    (LU + n right-hand sides) 2 x (8/3 + 8) n^3; A^-1 [X | B] and T^-1 R2
+    (LU + 2n right-hand sides) 2 x (8/3 + 16) n^3; four n x n x n GEMMs
+    (X B G1, M B, M A, B G2) 32 n^3; the one chain junction (one GEMM, one LU,
+    vector solves) 11 n^3."""
+    return (4 + 106 + 8 + 4 + 2 * (8 / 3 + 8) + 2 * (8 / 3 + 16) + 32 + 11) * n ** 3
+
+
+def cpu_info():
+    """Host cores this process may use and the CPU model (/proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": usable, "cpu_count": os.cpu_count(), "cpu_model": model}
+
+
+def _pool_worker(args):
+    """One process of the process-pool CPU baseline: the oracle on its share
+    of the sample with one BLAS thread; returns (solves, seconds)."""
+    cfg_idx, pos = args
+    from threadpoolctl import threadpool_limits
+    from khepri_amd import workloads
+    from oracle import crystal as OC
+    cfg = workloads.disc_stack_bz()
+    cl = OC.Crystal(cfg["pw"])
+    cl.add_layer_uniform("S1", cfg["eps_gap"], cfg["depth_gap"])
+    cl.add_layer_pixmap("Scyl", cfg["pixmap"], cfg["depth_disc"])
+    cl.set_device(["Scyl", "S1", "Scyl"], False)
+    with threadpool_limits(limits=1), np.errstate(all="ignore"):
+        t0 = time.perf_counter()
+        for p in pos:
+            s = cfg_idx[p]
+            cl.set_source(1 / cfg["freqs"][s], 1.0, 0.0, kp=tuple(cfg["kp"][s]))
+            cl.solve()
+            cl.poynting_flux_end()
+        return len(pos), time.perf_counter() - t0
+
+
+def cpu_baseline_modes(cfg, idx_all, sample, workers):
+    """The two CPU layouts the reference's users run: one process whose BLAS
+    uses every core, and a pool of single-threaded processes (the pattern of
+    khepri's examples/crystal_api/bic.py, multiprocessing.Pool with
+    OPENBLAS_NUM_THREADS=1).  Same oracle, evenly spread solves of the
+    workload."""
+    import multiprocessing as mp
+    from threadpoolctl import threadpool_limits
+    from oracle import crystal as OC
+    info = cpu_info()
+    out = dict(info)
+    pos = np.linspace(0, len(idx_all) - 1, sample).astype(int)
+    cl = OC.Crystal(cfg["pw"])
+    cl.add_layer_uniform("S1", cfg["eps_gap"], cfg["depth_gap"])
+    cl.add_layer_pixmap("Scyl", cfg["pixmap"], cfg["depth_disc"])
+    cl.set_device(["Scyl", "S1", "Scyl"], False)
+    with threadpool_limits(limits=info["nproc"]), np.errstate(all="ignore"):
+        t0 = time.perf_counter()
+        for p in pos:
+            s = idx_all[p]
+            cl.set_source(1 / cfg["freqs"][s], 1.0, 0.0, kp=tuple(cfg["kp"][s]))
+            cl.solve()
+            cl.poynting_flux_end()
+        el = time.perf_counter() - t0
+    out["all_core_blas"] = {"value": round(sample / el, 3), "unit": "solves/s", "cores": info["nproc"],
+                            "kind": "port", "sample": f"{sample} solves, one process, BLAS on {info['nproc']} threads"}
+    nw = max(1, min(workers, info["nproc"]))
+    per = max(1, (sample * 4) // nw)  # every worker gets about four times the one-process sample / workers
+    wpos = np.linspace(0, len(idx_all) - 1, per * nw).astype(int)
+    chunks = [(idx_all, wpos[w::nw]) for w in range(nw)]
+    ctx = mp.get_context("spawn")  # fresh interpreters: the children never touch the GPU
+    env_old = os.environ.get("OPENBLAS_NUM_THREADS")
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    try:
+        with ctx.Pool(nw) as pool:
+            res = pool.map_async(_pool_worker, chunks).get(timeout=600)
+    finally:
+        if env_old is None:
+            os.environ.pop("OPENBLAS_NUM_THREADS", None)
+        else:
+            os.environ["OPENBLAS_NUM_THREADS"] = env_old
+    done = sum(r[0] for r in res)
+    slowest = max(r[1] for r in res)
+    out["process_pool"] = {"value": round(done / slowest, 3), "unit": "solves/s", "cores": nw, "kind": "port",
+                           "sample": f"{done} solves over {nw} single-threaded processes, rate = solves / the "
+                                     f"slowest worker's solve loop ({slowest:.1f} s; process start-up excluded)"}
+    return out
+
+
+def grad_grid_rate(steps=2):
+    """grad evals/s of the adjoint workload (pw (11,11), four 64 x 64 layers,
+    16 frequencies): one grad eval = one frequency's dT/d eps for every pixel,
+    forward + backward."""
+    import torch
+    from khepri_amd import Crystal, workloads
+    from khepri_amd.adjoint import flux_gradient
+    cfg = workloads.grad_grid()
+    cl = Crystal(cfg["pw"])
+    names = []
+    for j, pix in enumerate(cfg["pixmaps"]):
+        cl.add_layer_pixmap(f"L{j}", pix, cfg["depth"])
+        names.append(f"L{j}")
+    cl.set_device(names, False)
+    cl.set_source(1 / cfg["freqs"], 1.0, 0.0)
+    flux_gradient(cl, wT=1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        flux_gradient(cl, wT=1.0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nf = len(cfg["freqs"])
+    return {"value": round(nf * steps / el, 3), "unit": "grad evals/s",
+            "sample": f"{steps} steps of {nf} frequencies, dT/d eps over 4 x 64 x 64 pixels each, after one warm-up step"}
+
+
 def build_crystal(nsolves):
     from khepri_amd import Crystal, workloads
     cfg = workloads.disc_stack_bz()
@@ -224,6 +352,55 @@ def extra_workload(name, args):
             # forward (R, T) + backward: one dT/d eps map per frequency for all 4 x 64 x 64 pixels
             return flux_gradient(cl, wT=1.0)
         units, unit = len(cfg["freqs"]), "grad evals/s"
+    elif name == "bz_fields":
+        # the reference's Brillouin-zone field integration (examples/bzi/twisted_phc.py:126-147) at pw (9,9)
+        # on the disc stack: a tilted Gaussian beam on 8 x 8 cells of 16 x 16 points, its Bloch amplitudes at
+        # 8 x 8 Bloch vectors, one batched solve with the field chain, one batched fields_volume on a
+        # 32 x 32 x 8 grid per Bloch vector (E and H copied to the host); unit = Bloch vectors
+        from functools import partial
+        from khepri_amd import beams as B
+        cfg = workloads.disc_stack_bz()
+        ns, bzs, wl = 16, (8, 8), 1 / 0.55
+        xs = np.arange(ns * bzs[0]) / ns
+        X, Y = np.meshgrid(xs, xs)
+        Z = np.zeros_like(X)
+        geo = (4.0, 4.0, -0.5, 0.2, 0.3, 0.1)
+        kps = workloads.bz_grid(bzs).reshape(2, -1).T
+        xg, yg = np.meshgrid(np.linspace(0, 1, 32), np.linspace(0, 1, 32))
+        zs = np.linspace(-0.2, 2.4, 8)
+        f = B.shifted_rotated_fields(partial(B._paraxial_gaussian_field_fn, beam_waist=2.0), X, Y, Z, wl, *geo)
+        fr = torch.as_tensor(np.swapaxes(np.swapaxes(f[..., 0], 0, 2), 1, 3), device="cuda")
+        Xd, Yd = torch.as_tensor(X, device="cuda"), torch.as_tensor(Y, device="cuda")
+
+        def build(cls_):
+            c = cls_(cfg["pw"])
+            c.add_layer_uniform("S1", cfg["eps_gap"], cfg["depth_gap"])
+            c.add_layer_pixmap("Scyl", cfg["pixmap"], cfg["depth_disc"])
+            c.set_device(["Scyl", "S1", "Scyl"], [True] * 3)
+            return c
+        cl = build(Crystal)
+
+        def step():
+            cl.set_source(np.full(len(kps), wl), np.nan, np.nan, kp=kps)
+            cl.solve()
+            F = np.stack([B.amplitudes_from_fields(fr, cl.expansion, wl, tuple(k), Xd, Yd, bzs).reshape(-1)
+                          for k in kps])
+            E, H = cl.fields_volume(xg, yg, zs, incident_fields=F)
+            return E.sum(axis=0), H.sum(axis=0)
+        units, unit = len(kps), "Bloch vectors/s"
+
+        def cpu(nsample):
+            from oracle import beams as OB, crystal as OC
+            oc = build(OC.Crystal)
+            fo = OB.shifted_rotated_fields(partial(OB.paraxial_gaussian_field_fn, beam_waist=2.0), X, Y, Z, wl, *geo)
+            fro = np.swapaxes(np.swapaxes(fo[..., 0], 0, 2), 1, 3)
+            t0 = time.perf_counter()
+            for k in kps[:nsample]:
+                oc.set_source(wl, np.nan, np.nan, kp=tuple(k))
+                oc.solve()
+                Fo = OB.amplitudes_from_fields(fro, oc.expansion.g, cfg["pw"], tuple(k), X, Y, bzs).reshape(-1)
+                oc.fields_volume(xg, yg, zs, incident_fields=Fo)
+            return nsample / (time.perf_counter() - t0), f"{nsample} Bloch vectors of the same workload, the reference's loop"
     elif name == "islands":
         # analytical island layer (disc + rectangle, closed-form coefficients) in the c2 stack, pw (9, 9),
         # 64 frequencies x 8 x 8 Bloch vectors
@@ -330,7 +507,7 @@ def extra_workload(name, args):
     if cpu is not None and args.cpu_sample > 0:
         from threadpoolctl import threadpool_limits
         with threadpool_limits(limits=1), np.errstate(all="ignore"):
-            v, what = cpu(min(args.cpu_sample, {"islands": 30, "bands": 200, "beams": 4}[name]))
+            v, what = cpu(min(args.cpu_sample, {"islands": 30, "bands": 200, "beams": 4, "bz_fields": 2}[name]))
         out["cpu_baseline"] = {"value": round(v, 3), "unit": unit, "cores": 1, "kind": "port",
                                "sample": f"oracle, 1 BLAS thread: {what}"}
     return out
@@ -379,11 +556,16 @@ def main():
     ap.add_argument("--solves", type=int, default=65536, help="subset of the workload (default: all)")
     ap.add_argument("--cpu-sample", type=int, default=100, help="oracle solves timed for cpu_baseline (0: skip; 100 = about 10 s of CPU work)")
     ap.add_argument("--b1-sample", type=int, default=24, help="solves timed one at a time for the plain batch-1 line")
+    ap.add_argument("--grad-steps", type=int, default=2,
+                    help="steps of the adjoint workload (16 grad evals each) timed after the main line (0: skip)")
+    ap.add_argument("--cpu-workers", type=int, default=16,
+                    help="processes of the process-pool CPU baseline (capped at the usable cores; 0: skip both "
+                         "multi-core modes)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one plain device copy of this many bytes before the workload (byte-counter calibration)")
     ap.add_argument("--workload", default="disc_stack_bz",
                     help="disc_stack_bz (default, the reported line) | disc_sweep | random_devices | twisted | grad_grid | "
-                         "twisted_fields | islands | bands | beams")
+                         "twisted_fields | islands | bands | beams | bz_fields")
     ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
     ap.add_argument("--freqs", type=int, default=32, help="twisted: number of frequencies")
     ap.add_argument("--angles", type=int, default=8, help="twisted: number of twist angles")
@@ -472,12 +654,23 @@ def main():
         pb = plain_batch1(cfg, idx, args.b1_sample)
         out["plain_fp64_batch1"] = pb
         out["speedup_vs_plain_batch1"] = round(out["value"] / pb["value"], 2)
+    cf = counted_flops_per_solve(n)
+    out["counted_flops"] = {"per_solve": cf, "tflops": round(value * cf / 1e12, 3),
+                            "frac_fp64_peak": round(value * cf / 1e12 / PEAK_FP64_TFLOPS, 4),
+                            "note": "the build's own op sequence (cheaper than the reference's, whose count is "
+                                    "2.578e9 per solve)"}
+    if args.grad_steps > 0:
+        out["grad_grid"] = grad_grid_rate(args.grad_steps)
+        print("[bench] grad_grid done", file=sys.stderr, flush=True)
     if args.cpu_sample > 0:
         Rn, Tn = np.asarray(R), np.asarray(T)
         cb, dR, dT = cpu_baseline(cfg, Rn, Tn, idx, args.cpu_sample)
         out["cpu_baseline"] = cb
         out["max_abs_dR"] = dR
         out["max_abs_dT"] = dT
+        if args.cpu_workers > 0:
+            out["cpu_baseline_modes"] = cpu_baseline_modes(cfg, idx, args.cpu_sample, args.cpu_workers)
+            print("[bench] cpu baseline modes done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -271,6 +271,22 @@ int khx_longitudinal(const double* kx, const double* ky, const double* F, int N,
                      void* stream);
 int khx_idft_phase(const double* kx, const double* ky, double k0, const double* x, const double* y, int N, int npts,
                    double* E, void* stream);
+/* The same three for `batch` sources at once (the Brillouin-zone loop of
+ * khepri's examples/bzi/twisted_phc.py:126-147 runs fields_volume once per
+ * Bloch vector; here every Bloch vector / wavelength is one item):
+ *   khx_mode_propagate_batch: lam (batch, n), a (batch, 2n), zbar (nz) shared
+ *                             -> M (batch, 2n, nz)             fields.py:29-31, 53-62
+ *   khx_longitudinal_batch:   kx, ky (batch, N), F (batch, 4N, nz)
+ *                             -> uz, w (batch, N, nz)          fields.py:68-76
+ *   khx_idft_phase_batch:     kx, ky (batch, N) normalised, wl (batch) device,
+ *                             x, y (npts) shared -> E (batch, npts, N) with
+ *                             k0 = 2 pi / wl[b]                fourier.py:136-142 */
+int khx_mode_propagate_batch(const double* lam, const double* a, const double* zbar, int n, int nz, int batch,
+                             double* M, void* stream);
+int khx_longitudinal_batch(const double* kx, const double* ky, const double* F, int N, int nz, int batch, double* uz,
+                           double* w, void* stream);
+int khx_idft_phase_batch(const double* kx, const double* ky, const double* wl, const double* x, const double* y, int N,
+                         int npts, int batch, double* E, void* stream);
 
 /* ---------------------------------------------------------------- gradient
  * Reverse-mode (adjoint) building blocks for d(R, T)/d(pixel eps).  The

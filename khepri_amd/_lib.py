@@ -50,6 +50,9 @@ SIGNATURES = {
     "khx_mode_propagate": (_i, [_dp, _dp, _dp, _i, _i, _dp, _vp]),
     "khx_longitudinal": (_i, [_dp, _dp, _dp, _i, _i, _dp, _dp, _vp]),
     "khx_idft_phase": (_i, [_dp, _dp, C.c_double, _dp, _dp, _i, _i, _dp, _vp]),
+    "khx_mode_propagate_batch": (_i, [_dp, _dp, _dp, _i, _i, _i, _dp, _vp]),
+    "khx_longitudinal_batch": (_i, [_dp, _dp, _dp, _i, _i, _i, _dp, _dp, _vp]),
+    "khx_idft_phase_batch": (_i, [_dp, _dp, _dp, _dp, _dp, _i, _i, _i, _dp, _vp]),
     "khx_joint_subspace": (_i, [_dp, _i, _i, _i, _i, _dp, _vp]),
     "khx_joint_subspace_diag": (_i, [_dp, _i, _i, _i, _dp, _vp]),
     "khx_zgemm_op": (_i, [_i, _i, _i, _i, _i, _dp, _dp, _i, C.c_longlong, _dp, _i, C.c_longlong, _dp, _dp, _i,

@@ -494,6 +494,28 @@ int khx_idft_phase(const double* kx, const double* ky, double k0, const double* 
   return khx::idft_phase(as_stream(stream), as_cd(kx), as_cd(ky), k0, x, y, N, npts, as_cd(E));
 }
 
+int khx_mode_propagate_batch(const double* lam, const double* a, const double* zbar, int n, int nz, int batch,
+                             double* M, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(n > 0 && nz >= 0 && batch >= 0, "khx_mode_propagate_batch: bad sizes");
+  return khx::mode_propagate_batch(as_stream(stream), as_cd(lam), as_cd(a), zbar, n, nz, batch, as_cd(M));
+}
+
+int khx_longitudinal_batch(const double* kx, const double* ky, const double* F, int N, int nz, int batch, double* uz,
+                           double* w, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && nz >= 0 && batch >= 0, "khx_longitudinal_batch: bad sizes");
+  return khx::longitudinal_batch(as_stream(stream), as_cd(kx), as_cd(ky), as_cd(F), N, nz, batch, as_cd(uz),
+                                 as_cd(w));
+}
+
+int khx_idft_phase_batch(const double* kx, const double* ky, const double* wl, const double* x, const double* y, int N,
+                         int npts, int batch, double* E, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && npts >= 0 && batch >= 0, "khx_idft_phase_batch: bad sizes");
+  return khx::idft_phase_batch(as_stream(stream), as_cd(kx), as_cd(ky), wl, x, y, N, npts, batch, as_cd(E));
+}
+
 int khx_identity_minus(const double* A, int n, int batch, double* out, void* stream) {
   KHX_ENTRY_BEGIN
   KHX_REQUIRE(n > 0 && batch >= 0, "khx_identity_minus: bad sizes");

@@ -85,6 +85,89 @@ __global__ void idft_phase_kernel(const cd* __restrict__ kx, const cd* __restric
   }
 }
 
+// Batched forms for many sources at once (one Bloch vector / wavelength per
+// item): M[b] from lam[b], a[b] on shared zbar; uz[b], w[b] from kx[b], ky[b],
+// F[b]; E[b][p][g] = exp(i k0_b (Kx_bg x_p + Ky_bg y_p)), k0_b = 2 pi / wl[b].
+__global__ void mode_propagate_batch_kernel(const cd* __restrict__ lam, const cd* __restrict__ a,
+                                            const double* __restrict__ zbar, int n, int nz, int batch, cd* M) {
+  const long long per = (long long)2 * n * nz;
+  const long long total = per * batch;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long b = e / per, r = e % per;
+    const int i = (int)(r / nz), j = (int)(r % nz);
+    const cd l = lam[b * n + i % n];
+    const double z = zbar[j];
+    cd arg = mkc(l.x * z, l.y * z);
+    if (i >= n) arg = -arg;
+    cd L = expc(arg);
+    const double m = hypot(L.x, L.y);
+    if (m > 1e14) {
+      const double f = m / 1e14;
+      L = mkc(L.x / f, L.y / f);
+    }
+    M[e] = L * a[b * 2 * n + i];
+  }
+}
+
+__global__ void longitudinal_batch_kernel(const cd* __restrict__ kx, const cd* __restrict__ ky,
+                                          const cd* __restrict__ F, int N, int nz, int batch, cd* uz, cd* w) {
+  const long long per = (long long)N * nz;
+  const long long total = per * batch;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long b = e / per, r = e % per;
+    const int h = (int)(r / nz), j = (int)(r % nz);
+    const cd* Fb = F + b * 4 * per;
+    const cd sx = Fb[(long long)h * nz + j], sy = Fb[(long long)(N + h) * nz + j];
+    const cd ux = Fb[(long long)(2 * N + h) * nz + j], uy = Fb[(long long)(3 * N + h) * nz + j];
+    const cd kxh = kx[b * N + h], kyh = ky[b * N + h];
+    const cd t = kxh * sy - kyh * sx;
+    uz[e] = mkc(t.y, -t.x);  // -i t
+    w[e] = kxh * uy - kyh * ux;
+  }
+}
+
+__global__ void idft_phase_batch_kernel(const cd* __restrict__ kx, const cd* __restrict__ ky,
+                                        const double* __restrict__ wl, const double* __restrict__ x,
+                                        const double* __restrict__ y, int N, int npts, int batch, cd* E) {
+  const long long per = (long long)npts * N;
+  const long long total = per * batch;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long b = e / per, r = e % per;
+    const int p = (int)(r / N), g = (int)(r % N);
+    const double k0 = 2.0 * M_PI / wl[b];
+    const cd kxa = k0 * kx[b * N + g], kya = k0 * ky[b * N + g];
+    const cd ph = kxa * mkc(x[p], 0.0) + kya * mkc(y[p], 0.0);
+    E[e] = expc(mkc(-ph.y, ph.x));  // exp(i ph)
+  }
+}
+
+int mode_propagate_batch(hipStream_t s, const cd* lam, const cd* a, const double* zbar, int n, int nz, int batch,
+                         cd* M) {
+  if (nz <= 0 || batch <= 0) return 0;
+  mode_propagate_batch_kernel<<<grid_f((long long)2 * n * nz * batch), 256, 0, s>>>(lam, a, zbar, n, nz, batch, M);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+int longitudinal_batch(hipStream_t s, const cd* kx, const cd* ky, const cd* F, int N, int nz, int batch, cd* uz,
+                       cd* w) {
+  if (nz <= 0 || batch <= 0) return 0;
+  longitudinal_batch_kernel<<<grid_f((long long)N * nz * batch), 256, 0, s>>>(kx, ky, F, N, nz, batch, uz, w);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
+int idft_phase_batch(hipStream_t s, const cd* kx, const cd* ky, const double* wl, const double* x, const double* y,
+                     int N, int npts, int batch, cd* E) {
+  if (npts <= 0 || batch <= 0) return 0;
+  idft_phase_batch_kernel<<<grid_f((long long)npts * N * batch), 256, 0, s>>>(kx, ky, wl, x, y, N, npts, batch, E);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
 int eigenbasis(hipStream_t s, const cd* W, const cd* V, int n, int batch, cd* R) {
   if (batch <= 0) return 0;
   eigenbasis_kernel<<<grid_f((long long)batch * 4 * n * n), 256, 0, s>>>(W, V, n, batch, R);

@@ -83,6 +83,14 @@ int build_pq(hipStream_t s, const cd* C, const cd* IC, long long cstride, const 
 int pq_adjoint(hipStream_t s, const cd* Pbar, const cd* Qbar, const cd* kx, const cd* ky, int N, int batch,
                cd* ICbar, cd* Cbar);
 
+// field reconstruction for many sources at once (fields.hip)
+int mode_propagate_batch(hipStream_t s, const cd* lam, const cd* a, const double* zbar, int n, int nz, int batch,
+                         cd* M);
+int longitudinal_batch(hipStream_t s, const cd* kx, const cd* ky, const cd* F, int N, int nz, int batch, cd* uz,
+                       cd* w);
+int idft_phase_batch(hipStream_t s, const cd* kx, const cd* ky, const double* wl, const double* x, const double* y,
+                     int N, int npts, int batch, cd* E);
+
 // eigen-decomposition: A destroyed; lam (n per item), W (n x n, unit columns)
 size_t zeig_workspace_bytes(int n, int batch);
 int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws,

@@ -55,6 +55,19 @@ class ExtendedLayer:
         if isinstance(self.base, Layer):
             self.base.fields = self.fields
         kpb, wlb, Nb = self._base_sources(kp, wl)
+        if not isinstance(self.base, Layer):
+            # a Crystal as the base (khepri/extension.py:88-95): its whole stack
+            # solved at every kp + g, one batched solve, its total S-matrix used
+            if self.fields:
+                raise NotImplementedError("fields of an extended layer need a Layer base (a Crystal base keeps "
+                                          "no layer eigenspace)")
+            self.base.set_source(wlb.cpu().numpy(), np.nan, np.nan, kp=kpb.cpu().numpy())
+            self.base.solve(rt_only=False)
+            self.info = None
+            self.S = ops.joint_subspace(self.base.Stot.contiguous(), Nb, self.mode, 4)
+            self.W = self.V = self.L = None
+            self.IC = 1.0
+            return
         self.base.solve_batch(kpb, wlb)
         self.info = getattr(self.base, "info", None)
         self.S = ops.joint_subspace(self.base.S, Nb, self.mode, 4)
@@ -62,19 +75,34 @@ class ExtendedLayer:
             self.W = ops.joint_subspace(self.base.W, Nb, self.mode, 1)
             self.V = ops.joint_subspace(self.base.V, Nb, self.mode, 1)
             self.L = ops.joint_subspace_diag(self.base.L, Nb, self.mode)
-            # free-space modes of the base expansion at every kp + g (khepri/extension.py:114-126)
-            kx, ky = ops.kvectors(self.base.expansion.g_device(kp.device), kpb, wlb)
-            _, (W0, V0, _) = ops.analytic_layer("uniform", kx, ky, 1.0,
-                                                torch.zeros(kx.shape[0], dtype=torch.float64, device=kp.device),
-                                                modes=True)
-            self.W0 = ops.joint_subspace(W0, Nb, self.mode, 1)
-            self.V0 = ops.joint_subspace(V0, Nb, self.mode, 1)
+            self.W0, self.V0 = _extended_freespace_batch(self.base.expansion, kpb, wlb, Nb, self.mode)
         else:
             self.W = self.V = self.L = None
         self.IC = 1.0
         self.base.S = None
 
 
+def _extended_freespace_batch(e, kpb, wlb, Nb, mode):
+    """Free-space modes W0, V0 of the base expansion e at the base sources kpb
+    (B Nb, 2) / wlb (B Nb), scattered into the joint space: (B, n, n) each."""
+    kx, ky = ops.kvectors(e.g_device(kpb.device), kpb, wlb)
+    _, (W0, V0, _) = ops.analytic_layer("uniform", kx, ky, 1.0,
+                                        torch.zeros(kx.shape[0], dtype=torch.float64, device=kpb.device),
+                                        modes=True)
+    return ops.joint_subspace(W0, Nb, mode, 1), ops.joint_subspace(V0, Nb, mode, 1)
+
+
 def extended_freespace(e, gs, wl, mode, k_parallel):
-    """Host helper kept for API parity (khepri/extension.py:114-126)."""
-    raise NotImplementedError("extended free-space modes are built on the device by ExtendedLayer.solve_batch")
+    """Free-space eigenmodes of the extended basis (khepri/extension.py:114-126):
+    the base expansion e's W0, V0 at k_parallel + g for every g of gs
+    ((Nb, 2), the other lattice's harmonics), scattered into the joint space
+    with the kind-`mode` interleave.  Returns device tensors (n, n)."""
+    from .device import default_device
+    dev = default_device()
+    g = torch.as_tensor(np.ascontiguousarray(np.asarray(gs, dtype=np.float64).reshape(-1, 2)),
+                        dtype=torch.complex128, device=dev)
+    kp = torch.as_tensor(np.asarray(k_parallel, dtype=np.complex128).reshape(1, 2), device=dev)
+    kpb = (kp + g).contiguous()
+    wlb = torch.full((g.shape[0],), float(wl), dtype=torch.float64, device=dev)
+    W0, V0 = _extended_freespace_batch(e, kpb, wlb, g.shape[0], mode)
+    return W0[0], V0[0]

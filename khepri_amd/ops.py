@@ -606,6 +606,42 @@ def longitudinal(kx, ky, F):
     return uz, w
 
 
+def mode_propagate_batch(lam, a, zbar):
+    """lam (B, n), a (B, 2n), zbar (nz) -> M (B, 2n, nz)."""
+    lib = _lib.load()
+    B, n = lam.shape
+    lam, a, zbar = lam.contiguous(), a.contiguous(), zbar.contiguous()
+    M = torch.empty((B, 2 * n, zbar.shape[0]), dtype=CDT, device=lam.device)
+    check(lib.khx_mode_propagate_batch(_p(lam), _p(a), _p(zbar), n, zbar.shape[0], B, _p(M), _stream()),
+          "khx_mode_propagate_batch")
+    return M
+
+
+def longitudinal_batch(kx, ky, F):
+    """kx, ky (B, N), F (B, 4N, nz) -> uz, w (B, N, nz)."""
+    lib = _lib.load()
+    B, N = kx.shape
+    nz = F.shape[-1]
+    F, kx, ky = F.contiguous(), kx.contiguous(), ky.contiguous()
+    uz = torch.empty((B, N, nz), dtype=CDT, device=F.device)
+    w = torch.empty_like(uz)
+    check(lib.khx_longitudinal_batch(_p(kx), _p(ky), _p(F), N, nz, B, _p(uz), _p(w), _stream()),
+          "khx_longitudinal_batch")
+    return uz, w
+
+
+def idft_phase_batch(kx, ky, wl, x, y):
+    """exp(i k0_b (Kx x + Ky y)) for normalised K (B, N), k0_b = 2 pi / wl[b]:
+    (B, npts, N)."""
+    lib = _lib.load()
+    B, N = kx.shape
+    x, y, kx, ky, wl = x.contiguous(), y.contiguous(), kx.contiguous(), ky.contiguous(), wl.contiguous()
+    E = torch.empty((B, x.shape[0], N), dtype=CDT, device=x.device)
+    check(lib.khx_idft_phase_batch(_p(kx), _p(ky), _p(wl), _p(x), _p(y), N, x.shape[0], B, _p(E), _stream()),
+          "khx_idft_phase_batch")
+    return E
+
+
 def idft_phase(kx, ky, k0, x, y):
     """exp(i k0 (Kx x + Ky y)) for normalised K, (npts, N)."""
     lib = _lib.load()

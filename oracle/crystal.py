@@ -119,10 +119,15 @@ class ExtendedLayer:
         self.fields = base.fields
 
     def solve(self, kp, wl, reuse_convmat=False):
-        self.base.fields = self.fields
+        if isinstance(self.base, Layer):
+            self.base.fields = self.fields
         Ss, Ws, Vs, Ls = [], [], [], []
         for g in self.gs.T:
-            self.base.solve(np.asarray(kp) + g, wl, reuse_convmat=reuse_convmat)
+            if isinstance(self.base, Layer):
+                self.base.solve(np.asarray(kp) + g, wl, reuse_convmat=reuse_convmat)
+            else:  # a Crystal as the base (khepri/extension.py:93-95)
+                self.base.set_source(wl, np.nan, np.nan, kp=np.asarray(kp) + g)
+                self.base.solve()
             Ss.append(self.base.S.copy())
             if self.fields:
                 Ws.append(self.base.W.copy())

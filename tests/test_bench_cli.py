@@ -21,7 +21,7 @@ def test_help_lists_every_workload():
     r = _run("--help")
     assert r.returncode == 0
     for w in ("disc_stack_bz", "disc_sweep", "random_devices", "twisted", "twisted_fields", "grad_grid",
-              "islands", "bands", "beams"):
+              "islands", "bands", "beams", "bz_fields"):
         assert w in r.stdout
 
 

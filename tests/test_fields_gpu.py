@@ -68,3 +68,64 @@ def test_fields_twisted_small(gpu):
     cl.solve()
     E, H = cl.fields_volume(g["x"], g["y"], g["z"])
     assert rel_err(E, g["E"]) < 1e-8 and rel_err(H, g["H"]) < 1e-8
+
+
+def _disc_stack(C, pw, pix):
+    cl = C(pw)
+    cl.add_layer_uniform("S1", 1.0, 1.1)
+    cl.add_layer_pixmap("Scyl", pix, 0.55)
+    cl.set_device(["Scyl", "S1", "Scyl"], [True] * 3)
+    return cl
+
+
+def test_brillouin_zone_loop_batched(gpu):
+    """The reference's Brillouin-zone loop (examples/bzi/twisted_phc.py:126-147:
+    set_source(wl, nan, nan, kp) -> solve -> amplitudes_from_fields ->
+    fields_volume(incident_fields=F), one Bloch vector at a time) as ONE
+    batched solve and ONE batched fields_volume over every Bloch vector,
+    against the oracle running the reference's loop.  Relative field error
+    <= 1e-8 per Bloch vector."""
+    from functools import partial
+    from khepri_amd import Crystal, beams as B
+    from oracle import beams as OB, crystal as OC
+    pw, wl = (3, 3), 1.6
+    pix = W.disc_pixmap(64, 4.0, 0.3, 1.0)
+    ns, bzs = 8, (2, 2)
+    xs = np.arange(ns * bzs[0]) / ns
+    X, Y = np.meshgrid(xs, xs)
+    Zs = np.zeros_like(X)
+    f = OB.shifted_rotated_fields(partial(OB.paraxial_gaussian_field_fn, beam_waist=1.0), X, Y, Zs, wl, 1.0, 1.0,
+                                  -0.3, 0.1, 0.2, 0.0)
+    fr = np.swapaxes(np.swapaxes(f[..., 0], 0, 2), 1, 3)
+    kps = W.bz_grid(bzs).reshape(2, -1).T                      # (4, 2): one Bloch vector per row
+    xg, yg = np.meshgrid(np.linspace(0, 1, 6), np.linspace(0, 1, 5))
+    zs = [-0.1, 0.3, 1.0, 1.9, 2.5]
+    cl = _disc_stack(Crystal, pw, pix)
+    cl.set_source(np.full(len(kps), wl), np.nan, np.nan, kp=kps)
+    cl.solve()
+    F = np.stack([B.amplitudes_from_fields(fr, cl.expansion, wl, tuple(k), X, Y, bzs).reshape(-1) for k in kps])
+    E, H = cl.fields_volume(xg, yg, zs, incident_fields=F)
+    assert E.shape == (len(kps), len(zs), 3) + xg.shape and H.shape == E.shape
+    oc = _disc_stack(OC.Crystal, pw, pix)
+    for b, k in enumerate(kps):
+        oc.set_source(wl, np.nan, np.nan, kp=tuple(k))
+        oc.solve()
+        Fo = OB.amplitudes_from_fields(fr, oc.expansion.g, pw, tuple(k), X, Y, bzs).reshape(-1)
+        assert np.abs(Fo - F[b]).max() <= 1e-12 * np.abs(Fo).max()
+        Eo, Ho = oc.fields_volume(xg, yg, zs, incident_fields=Fo)
+        assert rel_err(E[b], Eo) < 1e-8 and rel_err(H[b], Ho) < 1e-8, b
+    # the plane-wave sources themselves (incident_fields=None), batched over wavelengths
+    freqs = np.array([0.5, 0.55, 0.6])
+    cl = _disc_stack(Crystal, pw, pix)
+    cl.set_source(1 / freqs, 1.0, 0.5, 20.0, 35.0)
+    cl.solve()
+    E, H = cl.fields_volume(xg, yg, zs)
+    E1, H1 = cl.fields_coords_xy(xg, yg, 0.3)
+    for b, fq in enumerate(freqs):
+        oc.set_source(1 / fq, 1.0, 0.5, 20.0, 35.0)
+        oc.solve()
+        Eo, Ho = oc.fields_volume(xg, yg, zs)
+        assert rel_err(E[b], Eo) < 1e-8 and rel_err(H[b], Ho) < 1e-8, b
+        assert rel_err(E1[b], Eo[1]) < 1e-8 and rel_err(H1[b], Ho[1]) < 1e-8, b
+    with pytest.raises(ValueError):
+        cl.fields_volume(xg, yg, zs, incident_fields=np.zeros((2, 4 * 9)))
