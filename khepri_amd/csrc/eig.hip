@@ -1524,7 +1524,9 @@ __device__ __forceinline__ void eig_body(int n, cd* A, long long sA, cd* Ztg, cd
   int nlog = 0;
   cd* log_s = X;
   double* log_c = reinterpret_cast<double*>(X + (long long)EIG_KLOG * n);
-  const int itmax = 30 * max(10, n);
+  // iteration limit; bits 8.. of flags lower it (KHX_EIG_ITMAX: testing the
+  // non-convergence status without a matrix that fails to converge)
+  const int itmax = (flags >> 8) > 0 ? (flags >> 8) : 30 * max(10, n);
   // chain passes (multishift) in the rest of the log area of X
   cd* ms_gs = X + 10ll * n;
   double* ms_gc = reinterpret_cast<double*>(ms_gs + (long long)(n + MS_LAG) * MS_NS);
@@ -1931,7 +1933,10 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   const char* cw_env = getenv("KHX_EIG_CHASE_WAVE");
   // bit 2: register chase (qr_sweep_reg; default on, KHX_EIG_REG_CHASE=0 selects the LDS chase)
   const char* rc_env = getenv("KHX_EIG_REG_CHASE");
-  const int flags = ((cw_env && atoi(cw_env) == 0) ? 0 : 1) | ((rc_env && atoi(rc_env) == 0) ? 0 : 4);
+  const char* cap_env = getenv("KHX_EIG_ITMAX");
+  const int cap = cap_env ? atoi(cap_env) : 0;
+  KHX_REQUIRE(cap >= 0 && cap < (1 << 22), "zeig: KHX_EIG_ITMAX out of range");
+  const int flags = ((cw_env && atoi(cw_env) == 0) ? 0 : 1) | ((rc_env && atoi(rc_env) == 0) ? 0 : 4) | (cap << 8);
   const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
   KHX_REQUIRE(lds <= 150 * 1024, "zeig: n too large for the LDS work vectors");
   {

@@ -229,13 +229,17 @@ class Layer:
             self.C = C
             self.IC = IC if modes else None
         elif self.formulation == Formulation.UNIFORM:
+            # 1 / eps as the reference's P = (1/er) [...] (khepri/alternative.py:138):
+            # ZeroDivisionError for a Python zero, inf (non-finite R, T) for a numpy one
+            inv = 1 / self.epsilon
             k0d = (k0 * self.depth).contiguous()
             self.S, (W, V, L) = ops.analytic_layer("uniform", kx, ky, self.epsilon, k0d, modes=modes)
-            self.IC = 1 / self.epsilon if modes else None
+            self.IC = inv if modes else None
         elif self.formulation in (Formulation.HALF_SPACE_INC, Formulation.HALF_SPACE_TRN):
+            inv = 1 / self.epsilon
             side = "reflexion" if self.formulation == Formulation.HALF_SPACE_INC else "transmission"
             self.S, (W, V, L) = ops.analytic_layer(side, kx, ky, self.epsilon, modes=modes)
-            self.IC = 1 / self.epsilon if modes else None
+            self.IC = inv if modes else None
         else:
             raise NotImplementedError(f"formulation {self.formulation}")
         self.W, self.V, self.L = (W, V, L) if modes else (None, None, None)

@@ -133,6 +133,44 @@ def gen_random_devices(count=10):
          pixmap_sha=np.array(shas))
 
 
+def gen_random_devices_edges():
+    """Devices of the random-device workload at its edges: pw (11,11) and
+    (13,13) (n = 242, 338) and 6-8 layers, by their index in the seeded draw."""
+    devs = W.random_device_params()
+    want = {"pw11": 3, "pw13": 1, "deep": 3}
+    picked, R, T, kinds = [], [], [], []
+    for i, dv in enumerate(devs):
+        nl = len(dv["layers"])
+        kind = None
+        if dv["pw"] == (11, 11) and want["pw11"] > 0:
+            kind = "pw11"
+        elif dv["pw"] == (13, 13) and nl >= 6 and want["pw13"] > 0:
+            kind = "pw13"
+        elif nl >= 6 and dv["pw"][0] <= 7 and want["deep"] > 0:
+            kind = "deep"
+        if kind is None:
+            continue
+        t0 = time.time()
+        cl = Crystal(dv["pw"])
+        names = []
+        for j, ly in enumerate(dv["layers"]):
+            cl.add_layer_pixmap(f"L{j}", W.render_layer(ly, drawing=Drawing), ly["depth"])
+            names.append(f"L{j}")
+        cl.set_device(names, False)
+        cl.set_source(1 / dv["freq"], 1.0, 1.0, dv["theta"], dv["phi"])
+        cl.solve()
+        r, t = cl.poynting_flux_end()
+        picked.append(i)
+        R.append(r)
+        T.append(t)
+        kinds.append(kind)
+        want[kind] -= 1
+        print(f"  device {i} {dv['pw']} {nl} layers ({kind}): {time.time() - t0:.1f}s")
+        if not any(want.values()):
+            break
+    save("random_devices_edges.npz", index=np.array(picked), R=np.array(R), T=np.array(T), kind=np.array(kinds))
+
+
 def gen_grad_grid():
     cfg = W.grad_grid()
     cl = Crystal((11, 11))
@@ -312,7 +350,7 @@ def gen_beams():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["disc_sweep", "disc_bz", "random_devices", "grad_grid",
+    which = sys.argv[1:] or ["disc_sweep", "disc_bz", "random_devices", "random_devices_edges", "grad_grid",
                              "twisted_small", "fields_patterned", "incident", "twisted", "analytic", "beams"]
     for w in which:
         t0 = time.time()

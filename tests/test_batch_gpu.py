@@ -30,6 +30,20 @@ def test_batch_golden_devices(gpu):
     assert np.abs(R - g["R"]).max() <= TOL and np.abs(T - g["T"]).max() <= TOL
 
 
+def test_batch_golden_edge_devices(gpu):
+    """Seeded devices at pw (11, 11) and (13, 13) and stacks of 6-8 layers
+    (random_devices_edges.npz, the reference's golden R/T)."""
+    from khepri_amd.batch import CrystalBatch
+    g = golden("random_devices_edges.npz")
+    devs = W.random_device_params()
+    chosen = [devs[i] for i in g["index"]]
+    assert any(dv["pw"] == (13, 13) for dv in chosen) and any(dv["pw"] == (11, 11) for dv in chosen)
+    assert sum(len(dv["layers"]) >= 6 for dv in chosen) >= 3
+    R, T = CrystalBatch([build(dv) for dv in chosen]).solve()
+    assert np.abs(R - g["R"]).max() <= TOL and np.abs(T - g["T"]).max() <= TOL, \
+        (np.abs(R - g["R"]).max(), np.abs(T - g["T"]).max())
+
+
 def test_batch_matches_single_solves(gpu):
     """A mix of harmonic counts (5..13)^2, 1-8 layers, a uniform layer and
     non-vacuum media, batched vs solved one by one."""

@@ -8,6 +8,9 @@ subsets are compared elsewhere).
   incidence, khepri/alternative.py:241).
 * c5 (8 angles x 32 frequencies, 625 extended harmonics): lossless, normal
   incidence: R + T = 1 within 1e-9.
+* c3 (4 096 random devices, pw (5..13)^2, 1-8 layers): one CrystalBatch over
+  the whole workload; the devices with golden R/T (both golden files) are
+  checked at their positions in the batch, the rest are finite.
 * c4 (pw (11, 11), 4 x 64 x 64 real pixels, 16 frequencies): R + T = 1 for
   every real permittivity, so the adjoint gradient of R + T with respect to
   every pixel vanishes: |d(R + T)/d eps| <= 1e-7 max |dT/d eps|.
@@ -38,6 +41,29 @@ def test_c2_full_energy_and_evanescent_pattern(gpu):
     assert 0.1 < evan.mean() < 0.2            # 13.9 % of the workload is evanescent
     assert np.abs(R[fin] + T[fin] - 1).max() <= 1e-9
     assert R[fin].min() >= -1e-12 and T[fin].min() >= -1e-12
+
+
+def test_c3_full_batch_golden_positions(gpu):
+    from conftest import golden
+    from khepri_amd import Crystal
+    from khepri_amd.batch import CrystalBatch
+    devs = W.random_device_params()
+    cls = []
+    for dv in devs:
+        c = Crystal(dv["pw"])
+        names = []
+        for j, ly in enumerate(dv["layers"]):
+            c.add_layer_pixmap(f"L{j}", W.render_layer(ly), ly["depth"])
+            names.append(f"L{j}")
+        c.set_device(names, False)
+        c.set_source(1 / dv["freq"], 1.0, 1.0, dv["theta"], dv["phi"])
+        cls.append(c)
+    R, T = CrystalBatch(cls).solve()
+    assert R.shape == (4096,) and np.isfinite(R).all() and np.isfinite(T).all()
+    for name in ("random_devices.npz", "random_devices_edges.npz"):
+        g = golden(name)
+        idx = g["index"]
+        assert np.abs(R[idx] - g["R"]).max() <= 1e-9 and np.abs(T[idx] - g["T"]).max() <= 1e-9, name
 
 
 def test_c5_full_energy(gpu):

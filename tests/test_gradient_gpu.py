@@ -181,14 +181,16 @@ def _grad_grid_crystal(findex):
 
 
 def test_grad_grid_vs_cpu_autograd(gpu):
-    """pw (11, 11), four 64x64 design layers: dT/d eps at two of the 16
-    frequencies against the FP64 CPU autograd reference."""
+    """pw (11, 11), four 64x64 design layers: dT/d eps at all 16 frequencies
+    (one batched adjoint solve) against the FP64 CPU autograd reference, and
+    T at the two golden frequencies against the reference's values."""
     from khepri_amd.adjoint import flux_gradient
-    findex = np.array([0, 15])
+    findex = np.arange(16)
     cfg, cl, names = _grad_grid_crystal(findex)
     R, T, grads = flux_gradient(cl)
     gold = golden("grad_grid.npz")
-    np.testing.assert_allclose(T, gold["T"], atol=1e-10)
+    np.testing.assert_allclose(T[gold["findex"]], gold["T"], atol=1e-10)
+    np.testing.assert_allclose(R[gold["findex"]], gold["R"], atol=1e-10)
     for b, fi in enumerate(findex):
         Tr, ref = AG.transmission_gradient(cfg["pixmaps"], [cfg["depth"]] * 4, cfg["pw"], 1 / cfg["freqs"][fi])
         assert abs(Tr - T[b]) < 1e-10
@@ -200,15 +202,16 @@ def test_grad_grid_vs_cpu_autograd(gpu):
 
 
 def test_grad_grid_vs_finite_differences(gpu):
-    """Central differences of the GPU forward on the seeded pixels: with the
-    step 1e-6 the difference quotient carries ~1e-14 / 2e-6 of rounding noise
-    (about 1e-4 of these gradients, the same on the numpy oracle), so it is
-    held to 2e-3; with step 1e-3 truncation and rounding are both below 1e-6."""
+    """Central differences of the GPU forward on all 64 seeded pixels: with
+    the step 1e-6 the difference quotient carries ~1e-14 / 2e-6 of rounding
+    noise (about 1e-4 of these gradients, the same on the numpy oracle), so it
+    is held to 2e-3; with step 1e-3 truncation and rounding are both below 1e-6."""
     from khepri_amd.adjoint import flux_gradient
     from khepri_amd.batch import CrystalBatch
     cfg, cl, names = _grad_grid_crystal(np.array([0]))
     _, T, grads = flux_gradient(cl)
-    pix = W.fd_pixels()[:8]
+    pix = W.fd_pixels()
+    assert len(pix) == 64
     wl = 1 / cfg["freqs"][0]
     for h, tol in ((1e-6, 2e-3), (1e-3, 1e-6)):
         cls = []

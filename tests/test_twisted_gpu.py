@@ -35,15 +35,17 @@ def test_twisted_small_rt(gpu, rt_only):
 
 
 def test_twisted_625_harmonics_rt(gpu):
-    """Base pw (5,5): 625 extended harmonics, n = 1250."""
+    """Base pw (5,5): 625 extended harmonics, n = 1250; both seeded twist
+    angles of the golden file."""
     g = golden("twisted.npz")
     cfg = W.twisted_bilayer()
-    ai = int(g["angle_index"][0])
-    cl = twisted((5, 5), np.deg2rad(cfg["angles_deg"][ai]), cfg["pixmap"])
-    cl.set_source(1 / float(g["freq"]), 1.0, 0.0)
-    cl.solve(rt_only=True)
-    r, t = cl.poynting_flux_end()
-    assert abs(r - g["R"][0]) <= TOL and abs(t - g["T"][0]) <= TOL
+    assert len(g["angle_index"]) == 2
+    for ai, r0, t0 in zip(g["angle_index"], g["R"], g["T"]):
+        cl = twisted((5, 5), np.deg2rad(cfg["angles_deg"][int(ai)]), cfg["pixmap"])
+        cl.set_source(1 / float(g["freq"]), 1.0, 0.0)
+        cl.solve(rt_only=True)
+        r, t = cl.poynting_flux_end()
+        assert abs(r - r0) <= TOL and abs(t - t0) <= TOL, (int(ai), r, r0, t, t0)
 
 
 def test_twisted_625_harmonics_fields(gpu):
