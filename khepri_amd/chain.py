@@ -87,24 +87,26 @@ def rt_amplitudes(items, c):
     infos = []
     for j in range(m - 2, -1, -1):
         Sj = seq[j]
-        D = ops.identity_minus(ops.matmul(R11[j + 1], Sj[:, 1, 1]))
+        P = ops.matmul(R11[j + 1], Sj[:, 1, 1])
+        D = ops.identity_minus(P, out=P)            # temporaries: factored in place
         if j >= 1:  # factor D_j and solve for R11_{j+1} S21_j in the same pass
-            lu, piv, info, Y = ops.lu_factor_solve(D, ops.matmul(R11[j + 1], Sj[:, 1, 0]))
+            lu, piv, info, Y = ops.lu_factor_solve(D, ops.matmul(R11[j + 1], Sj[:, 1, 0]),
+                                                   overwrite_a=True, overwrite_b=True)
             R11[j] = ops.matmul(Sj[:, 0, 1], Y, beta=1.0, C=Sj[:, 0, 0].clone())
         else:
-            lu, piv, info = ops.lu_factor(D)
+            lu, piv, info = ops.lu_factor(D, overwrite_a=True)
         infos.append(info)
         LU[j] = (lu, piv)
     # left-to-right vector pass
     S1 = seq[0]
     v = _mv(S1[:, 1, 0], c)
-    u = ops.lu_solve(LU[0][0], LU[0][1], _mv(R11[1], v))[..., 0]
+    u = ops.lu_solve(LU[0][0], LU[0][1], _mv(R11[1], v), overwrite_b=True)[..., 0]
     sR = _mv(S1[:, 0, 1], u, add=_mv(S1[:, 0, 0], c))
     y = _mv(S1[:, 1, 1], u, add=v)
     for j in range(1, m - 1):
         Sj = seq[j]
         v = _mv(Sj[:, 1, 0], y)
-        u = ops.lu_solve(LU[j][0], LU[j][1], _mv(R11[j + 1], v))[..., 0]
+        u = ops.lu_solve(LU[j][0], LU[j][1], _mv(R11[j + 1], v), overwrite_b=True)[..., 0]
         y = _mv(Sj[:, 1, 1], u, add=v)
     sT = _mv(seq[m - 1][:, 1, 0], y)
     return sR, sT, infos

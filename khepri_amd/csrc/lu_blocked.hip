@@ -12,6 +12,8 @@
 // way (NB x NB diagonal solves, then mini-GEMM updates of the rows above).
 // Every element of the trailing matrix is read and written once per panel
 // instead of once per column.
+#include <utility>
+
 #include "khx_internal.h"
 #include "khx_prof.h"
 
@@ -301,6 +303,99 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
   }
 }
 
+// After a panel is factored: its status word, then its interchanges on the
+// other columns (and B) and the triangular solve with L11 (pan: the panel's
+// first rows, unit lower, row stride pld), one thread per column.
+template <int NB>
+__device__ __forceinline__ void panel_apply(int n, int m, int j0, int nb, cd* a, int lda, cd* bm, int ldb,
+                                            const cd* pan, long long pld, const int* perm, int* info,
+                                            long long item, int first_zero, int keepL, int cend) {
+  const int tid = threadIdx.x;
+  if (tid == 0 && info) {
+    if (j0 == 0)
+      info[item] = first_zero;
+    else if (info[item] == 0)
+      info[item] = first_zero;
+  }
+  // The interchanges go in groups of LB_SG: when the rows a group touches
+  // are distinct (no pivot row is another row of the group), its swaps are
+  // independent and each column loads them all before storing, instead of
+  // one dependent load / store round per interchange.
+  __shared__ int gfree[NB / LB_SG];
+  if (tid < NB / LB_SG) {
+    bool ok = true;
+    const int g0 = tid * LB_SG;
+    for (int i = g0; i < g0 + LB_SG && i < nb; ++i) {
+      const int pi = perm[i];
+      for (int k = g0; k < g0 + LB_SG && k < nb; ++k)
+        if (k != i && (pi == k || (pi != i && perm[k] == pi))) ok = false;
+    }
+    gfree[tid] = ok ? 1 : 0;
+  }
+  __syncthreads();
+  // interchanges on the other columns and the triangular solve with L11,
+  // one thread per column
+  const int nleft = (keepL & 1) ? j0 : 0;
+  const int nright = n - j0 - nb;
+  for (int e = tid; e < nleft + nright + m; e += LB_THREADS) {
+    cd* col;
+    long long ld;
+    if (e < nleft) {
+      col = a + e;
+      ld = lda;
+    } else if (e < nleft + nright) {
+      col = a + j0 + nb + (e - nleft);
+      ld = lda;
+    } else {
+      col = bm + (e - nleft - nright);
+      ld = ldb;
+    }
+    cd* c0p = col + (long long)j0 * ld;
+    for (int g0 = 0; g0 < nb; g0 += LB_SG) {
+      if (gfree[g0 / LB_SG]) {
+        const cd z = mkc(0.0, 0.0);
+        cd xa0 = z, xa1 = z, xa2 = z, xa3 = z, xb0 = z, xb1 = z, xb2 = z, xb3 = z;
+        const int p0 = g0 < nb ? perm[g0] : g0, p1 = g0 + 1 < nb ? perm[g0 + 1] : g0 + 1;
+        const int p2 = g0 + 2 < nb ? perm[g0 + 2] : g0 + 2, p3 = g0 + 3 < nb ? perm[g0 + 3] : g0 + 3;
+        if (p0 != g0) { xa0 = c0p[(long long)g0 * ld]; xb0 = c0p[(long long)p0 * ld]; }
+        if (p1 != g0 + 1) { xa1 = c0p[(long long)(g0 + 1) * ld]; xb1 = c0p[(long long)p1 * ld]; }
+        if (p2 != g0 + 2) { xa2 = c0p[(long long)(g0 + 2) * ld]; xb2 = c0p[(long long)p2 * ld]; }
+        if (p3 != g0 + 3) { xa3 = c0p[(long long)(g0 + 3) * ld]; xb3 = c0p[(long long)p3 * ld]; }
+        if (p0 != g0) { c0p[(long long)g0 * ld] = xb0; c0p[(long long)p0 * ld] = xa0; }
+        if (p1 != g0 + 1) { c0p[(long long)(g0 + 1) * ld] = xb1; c0p[(long long)p1 * ld] = xa1; }
+        if (p2 != g0 + 2) { c0p[(long long)(g0 + 2) * ld] = xb2; c0p[(long long)p2 * ld] = xa2; }
+        if (p3 != g0 + 3) { c0p[(long long)(g0 + 3) * ld] = xb3; c0p[(long long)p3 * ld] = xa3; }
+      } else {
+        for (int jj = g0; jj < g0 + LB_SG && jj < nb; ++jj) {
+          const int p = perm[jj];
+          if (p != jj) {
+            cd* x = c0p + (long long)jj * ld;
+            cd* y = c0p + (long long)p * ld;
+            const cd t = *x;
+            *x = *y;
+            *y = t;
+          }
+        }
+      }
+    }
+    const bool trsm = (e >= nleft && e < nleft + nright) ? (j0 + nb + (e - nleft) < cend) : (e >= nleft && !(keepL & 2));
+    if (trsm) {
+      cd x[NB];
+#pragma unroll
+      for (int i = 0; i < NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ld] : mkc(0.0, 0.0);
+#pragma unroll
+      for (int i = 1; i < NB; ++i)
+        if (i < nb) {  // the last panel may be narrower: rows past it are outside the matrix
+#pragma unroll
+          for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], pan[i * pld + k], x[k]);
+        }
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        if (i < nb) col[(long long)(j0 + i) * ld] = x[i];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ multi-launch variant
 // The same right-looking LU, one panel per launch: zlu_panel_kernel factors
 // the panel A[j0:n, j0:j0+16] in LDS, applies its interchanges to the rest of
@@ -388,89 +483,131 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
       if (c < nb) a[(long long)(j0 + r) * lda + j0 + c] = pan[r * pld + c];
     }
   }
-  if (tid == 0 && info) {
-    if (j0 == 0)
-      info[item] = first_zero;
-    else if (info[item] == 0)
-      info[item] = first_zero;
-  }
-  // The interchanges go in groups of LB_SG: when the rows a group touches
-  // are distinct (no pivot row is another row of the group), its swaps are
-  // independent and each column loads them all before storing, instead of
-  // one dependent load / store round per interchange.
-  __shared__ int gfree[NB / LB_SG];
-  if (tid < NB / LB_SG) {
-    bool ok = true;
-    const int g0 = tid * LB_SG;
-    for (int i = g0; i < g0 + LB_SG && i < nb; ++i) {
-      const int pi = perm[i];
-      for (int k = g0; k < g0 + LB_SG && k < nb; ++k)
-        if (k != i && (pi == k || (pi != i && perm[k] == pi))) ok = false;
-    }
-    gfree[tid] = ok ? 1 : 0;
-  }
-  __syncthreads();
-  // interchanges on the other columns and the triangular solve with L11,
-  // one thread per column
-  const int nleft = (keepL & 1) ? j0 : 0;
-  const int nright = n - j0 - nb;
-  for (int e = tid; e < nleft + nright + m; e += LB_THREADS) {
-    cd* col;
-    long long ld;
-    if (e < nleft) {
-      col = a + e;
-      ld = lda;
-    } else if (e < nleft + nright) {
-      col = a + j0 + nb + (e - nleft);
-      ld = lda;
-    } else {
-      col = bm + (e - nleft - nright);
-      ld = ldb;
-    }
-    cd* c0p = col + (long long)j0 * ld;
-    for (int g0 = 0; g0 < nb; g0 += LB_SG) {
-      if (gfree[g0 / LB_SG]) {
-        const cd z = mkc(0.0, 0.0);
-        cd xa0 = z, xa1 = z, xa2 = z, xa3 = z, xb0 = z, xb1 = z, xb2 = z, xb3 = z;
-        const int p0 = g0 < nb ? perm[g0] : g0, p1 = g0 + 1 < nb ? perm[g0 + 1] : g0 + 1;
-        const int p2 = g0 + 2 < nb ? perm[g0 + 2] : g0 + 2, p3 = g0 + 3 < nb ? perm[g0 + 3] : g0 + 3;
-        if (p0 != g0) { xa0 = c0p[(long long)g0 * ld]; xb0 = c0p[(long long)p0 * ld]; }
-        if (p1 != g0 + 1) { xa1 = c0p[(long long)(g0 + 1) * ld]; xb1 = c0p[(long long)p1 * ld]; }
-        if (p2 != g0 + 2) { xa2 = c0p[(long long)(g0 + 2) * ld]; xb2 = c0p[(long long)p2 * ld]; }
-        if (p3 != g0 + 3) { xa3 = c0p[(long long)(g0 + 3) * ld]; xb3 = c0p[(long long)p3 * ld]; }
-        if (p0 != g0) { c0p[(long long)g0 * ld] = xb0; c0p[(long long)p0 * ld] = xa0; }
-        if (p1 != g0 + 1) { c0p[(long long)(g0 + 1) * ld] = xb1; c0p[(long long)p1 * ld] = xa1; }
-        if (p2 != g0 + 2) { c0p[(long long)(g0 + 2) * ld] = xb2; c0p[(long long)p2 * ld] = xa2; }
-        if (p3 != g0 + 3) { c0p[(long long)(g0 + 3) * ld] = xb3; c0p[(long long)p3 * ld] = xa3; }
-      } else {
-        for (int jj = g0; jj < g0 + LB_SG && jj < nb; ++jj) {
-          const int p = perm[jj];
-          if (p != jj) {
-            cd* x = c0p + (long long)jj * ld;
-            cd* y = c0p + (long long)p * ld;
-            const cd t = *x;
-            *x = *y;
-            *y = t;
-          }
-        }
+  panel_apply<NB>(n, m, j0, nb, a, lda, bm, ldb, pan, pld, perm, info, item, first_zero, keepL, cend);
+}
+
+// Pivot step JJ of the register panel (a template parameter, so that every
+// index into the row registers is a constant).
+template <int NB, int RPT, int JJ>
+__device__ __forceinline__ void panel_step(cd (&v)[RPT][NB], int nb, int pr, int j0, int n, long long item, int* ipiv,
+                                           int* perm, cd* prow, cd* jrow, double* rv, int* ri, cd* rc,
+                                           int& first_zero) {
+  if (JJ >= nb) return;  // uniform: the last panel may be narrower
+  const int tid = threadIdx.x;
+  double best = -1.0;
+  int bi = pr;
+  cd bv = mkc(0.0, 0.0);
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const int r = tid + k * LB_THREADS;
+    if (r >= JJ && r < pr) {
+      const double av = cabs1(v[k][JJ]);
+      if (av > best) {
+        best = av;
+        bi = r;
+        bv = v[k][JJ];
       }
     }
-    const bool trsm = (e >= nleft && e < nleft + nright) ? (j0 + nb + (e - nleft) < cend) : (e >= nleft && !(keepL & 2));
-    if (trsm) {
-      cd x[NB];
+  }
+  argmax_abs1(best, bi, bv, rv, ri, rc);
+  const int p = bi < pr ? bi : JJ;  // NaN column: keep the diagonal
+  // rows p (the new pivot row) and JJ trade places through LDS
 #pragma unroll
-      for (int i = 0; i < NB; ++i) x[i] = (i < nb) ? col[(long long)(j0 + i) * ld] : mkc(0.0, 0.0);
+  for (int k = 0; k < RPT; ++k) {
+    const int r = tid + k * LB_THREADS;
+    if (r == p) {
 #pragma unroll
-      for (int i = 1; i < NB; ++i)
-        if (i < nb) {  // the last panel may be narrower: rows past it are outside the matrix
+      for (int c = 0; c < NB; ++c) prow[c] = v[k][c];
+    } else if (r == JJ) {
 #pragma unroll
-          for (int k = 0; k < i; ++k) x[i] = fmsc(x[i], pan[i * pld + k], x[k]);
-        }
-#pragma unroll
-      for (int i = 0; i < NB; ++i)
-        if (i < nb) col[(long long)(j0 + i) * ld] = x[i];
+      for (int c = 0; c < NB; ++c) jrow[c] = v[k][c];
     }
   }
+  if (tid == 0) {
+    perm[JJ] = p;
+    if (ipiv) ipiv[item * n + j0 + JJ] = j0 + p;
+  }
+  __syncthreads();
+  const cd piv = prow[JJ];
+  const bool zero = (piv.x == 0.0 && piv.y == 0.0);
+  if (zero && first_zero == 0) first_zero = j0 + JJ + 1;
+  const cd rp = zero ? mkc(0.0, 0.0) : recipc(piv);
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const int r = tid + k * LB_THREADS;
+    if (r == JJ || r == p) {  // one set of loads: row JJ takes the pivot row, row p the old row JJ
+      const cd* src = (r == JJ) ? prow : jrow;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) v[k][c] = src[c];
+    }
+    if (r > JJ && r < pr) {
+      const cd l = v[k][JJ] * rp;
+      v[k][JJ] = l;
+#pragma unroll
+      for (int c = JJ + 1; c < NB; ++c) v[k][c] = fmsc(v[k][c], l, prow[c]);
+    }
+  }
+}
+
+template <int NB, int RPT, int... J>
+__device__ __forceinline__ void panel_steps(std::integer_sequence<int, J...>, cd (&v)[RPT][NB], int nb, int pr, int j0,
+                                            int n, long long item, int* ipiv, int* perm, cd* prow, cd* jrow,
+                                            double* rv, int* ri, cd* rc, int& first_zero) {
+  (panel_step<NB, RPT, J>(v, nb, pr, j0, n, item, ipiv, perm, prow, jrow, rv, ri, rc, first_zero), ...);
+}
+
+// The same panel step with the panel in registers: thread t holds the
+// panel rows t, t + 256, .. (RPT of them, pr <= 256 RPT) and every pivot
+// step runs on registers, two barriers per column (the pivot search's
+// cross-wave reduction, then the exchange of rows jj and p through LDS,
+// which also broadcasts the pivot row).  Pivot choice, arithmetic and
+// output are those of zlu_panel_kernel: the largest |Re| + |Im| below the
+// diagonal, the first such row on ties, NaN columns keep the diagonal.
+template <int NB, int RPT>
+__global__ __launch_bounds__(LB_THREADS, RPT == 1 ? 4 : 2) void zlu_panel_reg_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
+                                                                   cd* B, int ldb, long long sB, int* ipiv, int* info,
+                                                                   int keepL, int cend) {
+  __shared__ double rv[LB_THREADS / 64];
+  __shared__ int ri[LB_THREADS / 64];
+  __shared__ cd rc[LB_THREADS / 64];
+  __shared__ int perm[NB];
+  __shared__ cd prow[NB], jrow[NB];
+  // L11 for the column solves, in dynamic LDS (NB (NB + 1) elements): with a
+  // static array the compiler hoists all its loads out of the column loop
+  extern __shared__ cd l11[];
+  const long long item = blockIdx.x;
+  cd* a = A + item * sA;
+  cd* bm = m > 0 ? B + item * sB : nullptr;
+  const int tid = threadIdx.x;
+  int first_zero = 0;
+  const int nb = min(NB, n - j0);
+  const int pr = n - j0;
+  cd v[RPT][NB];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const int r = tid + k * LB_THREADS;
+    const cd* src = a + (long long)(j0 + r) * lda + j0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) v[k][c] = (r < pr && c < nb) ? src[c] : mkc(0.0, 0.0);
+  }
+  panel_steps<NB, RPT>(std::make_integer_sequence<int, NB>{}, v, nb, pr, j0, n, item, ipiv, perm, prow, jrow, rv, ri,
+                        rc, first_zero);
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const int r = tid + k * LB_THREADS;
+    if (r < pr) {
+      cd* dst = a + (long long)(j0 + r) * lda + j0;
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < nb) dst[c] = v[k][c];
+      if (r < NB) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) l11[r * (NB + 1) + c] = v[k][c];
+      }
+    }
+  }
+  __syncthreads();
+  panel_apply<NB>(n, m, j0, nb, a, lda, bm, ldb, l11, NB + 1, perm, info, item, first_zero, keepL, cend);
 }
 
 // X block = U11^-1 Y block for the rows j0 .. j0+nb of B (one thread per column)
@@ -643,9 +780,28 @@ template <int NB>
 static size_t lu_panel_lds(int n) { return (size_t)n * (NB + 1) * sizeof(cd); }
 
 // One panel step (panel in LDS, or in global memory when it does not fit).
+// KHX_LU_PANEL_REG=0 keeps every panel in LDS (zlu_panel_kernel)
+static bool panel_reg_enabled() {
+  const char* env = getenv("KHX_LU_PANEL_REG");
+  return !(env && atoi(env) == 0);
+}
+
 template <int NB>
 static int launch_panel(hipStream_t s, int n, int m, int j0, Mat A, Mat B, int batch, int* ipiv, int* info, int mode,
                         int cend, bool gp, size_t lds) {
+  if constexpr (NB == 16) {
+    const int pr = n - j0;
+    if (pr <= 2 * LB_THREADS && panel_reg_enabled()) {
+      if (pr <= LB_THREADS)
+        zlu_panel_reg_kernel<NB, 1><<<dim3(batch), dim3(LB_THREADS), NB * (NB + 1) * sizeof(cd), s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
+                                                                            B.stride, ipiv, info, mode, cend);
+      else
+        zlu_panel_reg_kernel<NB, 2><<<dim3(batch), dim3(LB_THREADS), NB * (NB + 1) * sizeof(cd), s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
+                                                                            B.stride, ipiv, info, mode, cend);
+      KHX_LAUNCH_CHECK();
+      return 0;
+    }
+  }
   if (gp) {
     zlu_panel_kernel<NB, true><<<dim3(batch), dim3(LB_THREADS), 0, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
                                                                       B.stride, ipiv, info, mode, cend);

@@ -80,7 +80,7 @@ def _fourier_fields_many(self, zs, inc, kx, ky):
     k0 = 2 * np.pi / self._wl
     Sref = self.layers["Sref"]
     Rref = ops.eigenbasis(Sref.W, Sref.V)
-    sol, info = ops.solve(Rref, inc.reshape(B, 2 * n, 1))
+    sol, info = ops.solve(Rref, inc.reshape(B, 2 * n, 1), overwrite_a=True)   # Rref: a fresh eigenbasis
     ops.raise_on_info(info, "incident-medium eigenbasis", lu=True)
     c1p = sol[:, :n, :]                                       # (B, n, 1)
     groups = {}
@@ -95,8 +95,9 @@ def _fourier_fields_many(self, zs, inc, kx, ky):
         Sl = self.stacking_matrices[li]
         Sr = self.stacking_reverse_matrices[li]
         # translate_mode_amplitudes (khepri/fields.py:18-27)
-        F = ops.identity_minus(ops.matmul(Sl[:, 1, 1], Sr[:, 0, 0]))
-        cdp, info = ops.solve(F, ops.matmul(Sl[:, 1, 0], c1p))
+        P = ops.matmul(Sl[:, 1, 1], Sr[:, 0, 0])
+        F = ops.identity_minus(P, out=P)
+        cdp, info = ops.solve(F, ops.matmul(Sl[:, 1, 0], c1p), overwrite_a=True, overwrite_b=True)
         ops.raise_on_info(info, "mode amplitudes", lu=True)
         cdm = ops.matmul(Sr[:, 0, 0], cdp)
         RI = ops.eigenbasis(layer.W, layer.V)
@@ -108,7 +109,7 @@ def _fourier_fields_many(self, zs, inc, kx, ky):
             W0, V0 = layer.W0, layer.V0
         R0 = ops.eigenbasis(W0, V0)
         amp = ops.matmul(R0, torch.cat([cdp, cdm], dim=1))
-        a, info = ops.solve(RI, amp)                          # (B, 2n, 1)
+        a, info = ops.solve(RI, amp, overwrite_b=True)        # (B, 2n, 1); RI is used again below
         ops.raise_on_info(info, "layer eigenbasis", lu=True)
         cols = [j for j, _ in items]
         if len(set(np.round(k0, 15))) == 1:                   # one wavelength: zbar shared by the batch

@@ -96,11 +96,21 @@ def convmat(eps, pw):
     return C
 
 
-def solve(A, B):
-    """X = A^-1 B for batches (A (b, n, n), B (b, n, m)); inputs untouched."""
+def _owned(t, overwrite):
+    """t itself when the caller hands it over (overwrite) and it is contiguous,
+    else a contiguous copy: the kernels work in place."""
+    if overwrite:
+        return t.contiguous()
+    return t.clone(memory_format=torch.contiguous_format)
+
+
+def solve(A, B, overwrite_a=False, overwrite_b=False):
+    """X = A^-1 B for batches (A (b, n, n), B (b, n, m)).  The inputs are left
+    untouched unless overwrite_a / overwrite_b hand them over (A then holds
+    the LU factors and B is returned as X)."""
     lib = _lib.load()
-    Af = A.clone().contiguous()
-    X = B.clone().contiguous()
+    Af = _owned(A, overwrite_a)
+    X = _owned(B, overwrite_b)
     b, n, m = X.shape
     info = torch.empty(b, dtype=torch.int32, device=A.device)
     check(lib.khx_zgesv(n, m, _p(Af), _p(X), b, _p(info), _stream()), "khx_zgesv")
@@ -110,7 +120,7 @@ def solve(A, B):
 def inverse(A):
     b, n, _ = A.shape
     I = torch.eye(n, dtype=CDT, device=A.device).expand(b, n, n).contiguous()
-    return solve(A, I)
+    return solve(A, I, overwrite_b=True)
 
 
 def _mat_view(t, name):
@@ -253,10 +263,11 @@ def poynting_vec_adjoint(sR, sT, g, kp, wl, epsi, epse, wR=None, wT=None):
     return sRbar, sTbar
 
 
-def eig(A):
-    """Eigen-decomposition of a batch of general complex matrices."""
+def eig(A, overwrite_a=False):
+    """Eigen-decomposition of a batch of general complex matrices (A is left
+    untouched unless overwrite_a hands it over as workspace)."""
     lib = _lib.load()
-    Af = A.clone().contiguous()
+    Af = _owned(A, overwrite_a)
     b, n, _ = Af.shape
     lam = torch.empty((b, n), dtype=CDT, device=A.device)
     W = torch.empty((b, n, n), dtype=CDT, device=A.device)
@@ -469,10 +480,10 @@ def poynting_vec(sR, sT, g, kp, wl, epsi, epse, orders=False):
     return R, T, Ro, To
 
 
-def lu_factor(A):
-    """(LU, ipiv, info) of a batch (A untouched)."""
+def lu_factor(A, overwrite_a=False):
+    """(LU, ipiv, info) of a batch (A untouched unless overwrite_a)."""
     lib = _lib.load()
-    LU = A.clone().contiguous()
+    LU = _owned(A, overwrite_a)
     b, n, _ = LU.shape
     piv = torch.empty((b, n), dtype=torch.int32, device=A.device)
     info = torch.empty(b, dtype=torch.int32, device=A.device)
@@ -480,11 +491,12 @@ def lu_factor(A):
     return LU, piv, info
 
 
-def lu_factor_solve(A, B):
-    """(LU, ipiv, info, X = A^-1 B): the factors kept for later solves (inputs untouched)."""
+def lu_factor_solve(A, B, overwrite_a=False, overwrite_b=False):
+    """(LU, ipiv, info, X = A^-1 B): the factors kept for later solves (inputs
+    untouched unless handed over with overwrite_a / overwrite_b)."""
     lib = _lib.load()
-    LU = A.clone().contiguous()
-    X = B.clone().contiguous()
+    LU = _owned(A, overwrite_a)
+    X = _owned(B, overwrite_b)
     b, n, m = X.shape
     piv = torch.empty((b, n), dtype=torch.int32, device=A.device)
     info = torch.empty(b, dtype=torch.int32, device=A.device)
@@ -492,9 +504,9 @@ def lu_factor_solve(A, B):
     return LU, piv, info, X
 
 
-def lu_solve(LU, piv, B):
+def lu_solve(LU, piv, B, overwrite_b=False):
     lib = _lib.load()
-    X = B.clone().contiguous()
+    X = _owned(B, overwrite_b)
     if X.dim() == 2:
         X = X.unsqueeze(-1)
     b, n, m = X.shape

@@ -370,7 +370,39 @@ def test_eig_tuning_variants(gpu, monkeypatch, env):
 
 
 LU_VARIANTS = [{"KHX_LU_NB": "32"}, {"KHX_LU_PATH": "0"}, {"KHX_LU_PATH": "1"}, {"KHX_LU_PATH": "2"},
-               {"KHX_LU_UNBLOCKED": "1"}, {"KHX_LU_RECSOLVE": "0"}, {"KHX_LU_NB": "32", "KHX_LU_RECFACTOR": "1"}]
+               {"KHX_LU_UNBLOCKED": "1"}, {"KHX_LU_RECSOLVE": "0"}, {"KHX_LU_NB": "32", "KHX_LU_RECFACTOR": "1"},
+               {"KHX_LU_PANEL_REG": "0"}]
+
+
+@pytest.mark.parametrize("n", [97, 162, 250, 338, 600])
+def test_lu_register_panel_matches_lds_panel(gpu, monkeypatch, n):
+    """The register-resident panel kernel (panel rows in registers, up to 512
+    rows) and the LDS panel kernel make the same pivot choices and the same
+    arithmetic: factors, pivots, solutions and status words agree bit for bit,
+    including an exactly singular system (zero pivot) and a NaN column."""
+    from khepri_amd import ops
+    rng = np.random.default_rng(n)
+    A = crand(rng, 4, n, n)
+    A[1, :, 5] = 0.0                       # singular: zero pivot in column 5
+    A[2, 7:, 3] = np.nan                   # a NaN column below the diagonal
+    A[3, :, :] = A[3, :, :] + 1e3 * np.eye(n)
+    out = {}
+    for reg in ("0", "1"):
+        monkeypatch.setenv("KHX_LU_PANEL_REG", reg)
+        res = []
+        for m in (5, 100):
+            B = crand(np.random.default_rng(m), 4, n, m)
+            X, info = ops.solve(dev(A, gpu), dev(B, gpu))
+            res += [X.cpu().numpy(), info.cpu().numpy()]
+        lu, piv, info = ops.lu_factor(dev(A, gpu))
+        res += [lu.cpu().numpy(), piv.cpu().numpy(), info.cpu().numpy()]
+        out[reg] = res
+    for a, b in zip(out["0"], out["1"]):
+        np.testing.assert_array_equal(a, b)
+    info = out["1"][-1]
+    assert info[0] == 0 and info[3] == 0 and info[1] > 0
+    ref = np.linalg.solve(A[[0, 3]], crand(np.random.default_rng(5), 4, n, 5)[[0, 3]])
+    np.testing.assert_allclose(out["1"][0][[0, 3]], ref, atol=1e-10 * np.abs(ref).max())
 
 
 @pytest.mark.parametrize("env", LU_VARIANTS, ids=lambda e: "-".join(f"{k[4:]}={v}" for k, v in e.items()))
