@@ -425,7 +425,7 @@ static dim3 grid_for(long long total) {
 
 // Workspace (items of n*n complex per batch element):
 //   P(1) PQ(1) | W/V when not supplied (2) | lam (n, rounded) | eig ws (2)
-//   t1(1) t2(1) Wc(1) A(1) B(1) Ac(1) R1(2) M(1) T(1) R2(2) XB(1)
+//   t1(1) t2(1) Wc(2) A(1) B(1) Ac(1) R1(2) M(1) T(1) R2(2) XB(1)
 static long long patterned_items(int n) { return 22; }
 
 size_t patterned_layer_ws_bytes(int n, int batch) {
@@ -455,8 +455,8 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   cd* V = Vout ? Vout : take(1);
   cd* eigws = take(2);
   cd* t1 = take(1);
-  cd* t2 = take(1);
-  cd* Wc = take(1);
+  cd* t2 = take(1);  // right after t1: [t1; t2] is one batch of 2 batch items
+  cd* Wc = take(2);  // copies of W and V, factored in one batch
   cd* Am = take(1);
   cd* Bm = take(1);
   cd* Ac = take(1);
@@ -497,15 +497,19 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   KHX_LAUNCH_CHECK();
 
   // ---- build_scatmat
-  // t1 = W^-1 W0 (W0 = I)
+  // t1 = W^-1 W0 (W0 = I) and t2 = V^-1 V0 as one batch of 2 batch systems
+  // (status words: info_lu1 then info_lu2, which follow it)
   KHX_TRY(zcopy(s, Mat{Wc, n, nn}, Mat{W, n, nn}, n, n, batch));
+  KHX_TRY(zcopy(s, Mat{Wc + (long long)batch * nn, n, nn}, Mat{V, n, nn}, n, n, batch));
   KHX_TRY(zset_identity(s, Mat{t1, n, nn}, n, n, one, batch));
-  KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{t1, n, nn}, batch, info_lu1));
-  // t2 = V^-1 V0
-  KHX_TRY(zcopy(s, Mat{Wc, n, nn}, Mat{V, n, nn}, n, n, batch));
   fill_v0_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(kx, ky, N, batch, t2, nn, n);
   KHX_LAUNCH_CHECK();
-  KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{t2, n, nn}, batch, info_lu2));
+  if (2 * (long long)batch <= 65535) {
+    KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{t1, n, nn}, 2 * batch, info_lu1));
+  } else {  // the multi-launch LU takes at most 65535 systems per call
+    KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{t1, n, nn}, batch, info_lu1));
+    KHX_TRY(zgesv(s, n, n, Mat{Wc + (long long)batch * nn, n, nn}, Mat{t2, n, nn}, batch, info_lu2));
+  }
   // A = t1 + t2, B = t1 - t2
   KHX_TRY(zaxpby(s, Mat{Am, n, nn}, one, Mat{t1, n, nn}, one, Mat{t2, n, nn}, n, n, batch));
   KHX_TRY(zaxpby(s, Mat{Bm, n, nn}, one, Mat{t1, n, nn}, mone, Mat{t2, n, nn}, n, n, batch));
