@@ -492,6 +492,17 @@ def extra_workload(name, args):
     else:
         raise SystemExit(f"unknown workload {name}")
     tuned = run_autotune(step, args)
+    graph = None
+    if args.graph:
+        # the step captured once as a HIP graph (khepri_amd.graph) and replayed
+        from khepri_amd.graph import GradientGraph, SolveGraph
+        if name == "disc_sweep":
+            graph = SolveGraph(cl, rt_only=True)
+        elif name == "grad_grid":
+            graph = GradientGraph(cl, wT=1.0)
+        else:
+            raise SystemExit("--graph applies to the disc_sweep and grad_grid workloads")
+        step = graph.replay
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -503,7 +514,8 @@ def extra_workload(name, args):
     out = {"metric": f"{name} throughput", "value": round(units * args.steps / el, 3), "unit": unit,
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": name, "units_per_step": units}, "gemm_autotune": tuned}
+           "config": {"workload": name, "units_per_step": units, "hip_graph": graph is not None},
+           "gemm_autotune": tuned}
     if cpu is not None and args.cpu_sample > 0:
         from threadpoolctl import threadpool_limits
         with threadpool_limits(limits=1), np.errstate(all="ignore"):
@@ -569,6 +581,8 @@ def main():
     ap.add_argument("--devices", type=int, default=4096, help="random_devices: number of devices")
     ap.add_argument("--freqs", type=int, default=32, help="twisted: number of frequencies")
     ap.add_argument("--angles", type=int, default=8, help="twisted: number of twist angles")
+    ap.add_argument("--graph", action="store_true",
+                    help="disc_sweep / grad_grid: replay the step from a captured HIP graph")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1 (default): pick GEMM tile shapes with khepri_amd.autotune on one untimed step first; 0: "
                          "built-in tile rule")

@@ -73,6 +73,13 @@ def flux_gradient(crystal, wT=1.0, wR=0.0, wrt=None, reduce=False, eta_rel=1e-12
     sources when ``reduce``.  Real pixmaps give the derivative with respect to
     the real permittivity; complex pixmaps give dL/dRe + i dL/dIm.
     """
+    R, T, grads = _flux_gradient_device(crystal, wT, wR, wrt, reduce, eta_rel, chunk)
+    return R.cpu().numpy(), T.cpu().numpy(), grads
+
+
+def _flux_gradient_device(crystal, wT=1.0, wR=0.0, wrt=None, reduce=False, eta_rel=1e-12, chunk=None):
+    """flux_gradient with R and T left on the device (no host sync: the
+    sequence can be captured in a graph; khepri_amd.graph.GradientGraph)."""
     cl = crystal
     assert cl.source_defined, "Call set_source before differentiating."
     dev = cl._dev()
@@ -89,10 +96,14 @@ def flux_gradient(crystal, wT=1.0, wR=0.0, wrt=None, reduce=False, eta_rel=1e-12
     N = cl.expansion.N
     n = 2 * N
     g = cl.expansion.g_device(dev)
-    kp_all = torch.as_tensor(cl._kp, device=dev)
-    wl_all = torch.as_tensor(cl._wl, device=dev)
-    te_all = torch.as_tensor(cl._te, device=dev)
-    tm_all = torch.as_tensor(cl._tm, device=dev)
+    static = getattr(cl, "_static_sources", None)
+    if static is not None:   # device buffers of a captured graph (khepri_amd.graph)
+        kp_all, wl_all, te_all, tm_all = static
+    else:
+        kp_all = torch.as_tensor(cl._kp, device=dev)
+        wl_all = torch.as_tensor(cl._wl, device=dev)
+        te_all = torch.as_tensor(cl._te, device=dev)
+        tm_all = torch.as_tensor(cl._tm, device=dev)
     wT_all = _weights(wT, B, dev)
     wR_all = _weights(wR, B, dev)
     if chunk is None:
@@ -107,8 +118,8 @@ def flux_gradient(crystal, wT=1.0, wR=0.0, wrt=None, reduce=False, eta_rel=1e-12
     else:
         grads = {nm: torch.empty((B,) + shapes[nm], dtype=torch.complex128 if cplx[nm] else torch.float64,
                                  device=dev) for nm in wrt}
-    R = np.empty(B)
-    T = np.empty(B)
+    R = torch.empty(B, dtype=torch.float64, device=dev)
+    T = torch.empty(B, dtype=torch.float64, device=dev)
     for c0 in range(0, B, chunk):
         sl = slice(c0, min(B, c0 + chunk))
         r, t, gch = _gradient_chunk(cl, names, wrt, g, N, kp_all[sl], wl_all[sl], te_all[sl], tm_all[sl],
@@ -234,7 +245,7 @@ def _gradient_chunk(cl, names, wrt, g, N, kp, wl, te, tm, wR, wT, eta_rel, reduc
             Cbar = ops.matmul(ones, Cbar.reshape(1, Bc, N * N)).reshape(1, N, N)
         m = ops.convmat_adjoint(Cbar, cl.expansion.pw, eps.shape, np.iscomplexobj(eps))
         out[nm] = m[0] if reduce else m
-    return r.cpu().numpy(), t.cpu().numpy(), out
+    return r, t, out
 
 
 def _accumulate(acc, name, S):

@@ -207,10 +207,14 @@ class Crystal:
 
         B = self.batch
         chunk = self._chunk(B)
-        kp_all = torch.as_tensor(self._kp, device=dev)
-        wl_all = torch.as_tensor(self._wl, device=dev)
-        te_all = torch.as_tensor(self._te, device=dev)
-        tm_all = torch.as_tensor(self._tm, device=dev)
+        static = getattr(self, "_static_sources", None)
+        if static is not None:   # device buffers of a captured graph (khepri_amd.graph)
+            kp_all, wl_all, te_all, tm_all = static
+        else:
+            kp_all = torch.as_tensor(self._kp, device=dev)
+            wl_all = torch.as_tensor(self._wl, device=dev)
+            te_all = torch.as_tensor(self._te, device=dev)
+            tm_all = torch.as_tensor(self._tm, device=dev)
         self._sources_dev = (kp_all, wl_all, te_all, tm_all)
         want_reverse = getattr(self, "_fields_requested", False)
         if rt_only is None:

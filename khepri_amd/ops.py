@@ -50,15 +50,48 @@ def eig_slots(n):
     return _SLOTS[key]
 
 
+_DEFERRED = []   # stack of lists: status words recorded (not checked) while a graph is captured
+
+
+class deferred_status:
+    """Context in which raise_on_info records the status words in `sink`
+    instead of reading them (a read is a host sync, which graph capture does
+    not allow); the caller checks them later with check_status(sink)."""
+
+    def __init__(self, sink):
+        self.sink = sink
+
+    def __enter__(self):
+        _DEFERRED.append(self.sink)
+        return self.sink
+
+    def __exit__(self, *exc):
+        _DEFERRED.pop()
+        return False
+
+
+def check_status(sink):
+    for info, what, lu in sink:
+        _raise_on_words(info, what, lu)
+
+
 def raise_on_info(info, what, lu=False):
     """Map per-item status words to numpy-style exceptions (one host sync).
 
     Eigen-solver words (lu=False): > 0 no convergence, < 0 an LU of the
     S-matrix algebra met an exact zero pivot.  LU words (lu=True): nonzero =
     first zero pivot + 1.  Both raise numpy's LinAlgError, as numpy's eig,
-    solve and inv do in the reference."""
+    solve and inv do in the reference.  Inside deferred_status the words are
+    recorded for a later check_status instead."""
     if info is None:
         return
+    if _DEFERRED:
+        _DEFERRED[-1].append((info, what, lu))
+        return
+    _raise_on_words(info, what, lu)
+
+
+def _raise_on_words(info, what, lu):
     v = info.cpu().numpy()
     if lu and (v != 0).any():
         raise np.linalg.LinAlgError(f"{what}: singular matrix (items {np.nonzero(v)[0][:8]})")

@@ -115,3 +115,22 @@ def test_set_source_rejects_ambiguous_kp_layouts():
     assert cl.batch == 3 and np.allclose(cl.kp[:, 0], kps[0]) and np.allclose(cl.kp[:, 1], kps[1])
     cl.set_source(1.5, 1.0, 0.0, kp=(0.1, 0.2))           # one (kx, ky)
     assert cl.batch == 1
+
+
+def test_deferred_status_records_then_raises():
+    """ops.deferred_status: raise_on_info records the words (no host read)
+    and check_status raises later, as the eager check would."""
+    import torch
+    from khepri_amd import ops
+    sink = []
+    good = torch.zeros(4, dtype=torch.int32)
+    bad = torch.tensor([0, 0, 3, 0], dtype=torch.int32)
+    with ops.deferred_status(sink):
+        ops.raise_on_info(good, "ok", lu=True)
+        ops.raise_on_info(bad, "lu", lu=True)
+    assert len(sink) == 2
+    with pytest.raises(np.linalg.LinAlgError):
+        ops.check_status(sink)
+    ops.check_status(sink[:1])
+    with pytest.raises(np.linalg.LinAlgError):
+        ops.raise_on_info(bad, "lu", lu=True)
