@@ -263,6 +263,147 @@ static int launch_zgemm_mfma(hipStream_t s, int M, int N, int K, cd alpha, Mat A
   return 0;
 }
 
+// ------------------------------------------------------------- FP32 MFMA GEMM (mixed precision)
+// C = alpha A B + beta C (no transposes) with the product on the FP32 matrix
+// cores (v_mfma_f32_16x16x4f32, the same tiling as zgemm_mfma_kernel): the
+// complex-double operands are rounded to complex float when they are staged
+// in LDS, partial sums are FP32 and the epilogue (alpha, beta, C) is FP64.
+// The product's relative error is about 2^-24 (K = 16..324 here), so it is
+// used only where an FP64 residual refinement follows: the triangular-solve
+// updates of the mixed-precision zgesv (KHX_LU_MIXED=1).
+typedef float khx_f4 __attribute__((ext_vector_type(4)));
+struct cfl {
+  float x, y;
+};
+// v_mfma_f32_16x16x4f32 output: lane l holds D[F32_ROW(l / 16, r)][l % 16] in register r
+#define F32_ROW(lk, r) (4 * (lk) + (r))
+
+template <int WM, int WN>
+__global__ __launch_bounds__(256) void zgemm_f32mfma_kernel(int M, int N, int K, cd alpha,
+                                                            const cd* __restrict__ A, int lda, long long sA,
+                                                            const cd* __restrict__ B, int ldb, long long sB,
+                                                            cd beta, cd* __restrict__ C, int ldc, long long sC,
+                                                            int tilesN, int tilesPerItem, int xcd_remap) {
+  constexpr int BM = 32 * WM, BN = 32 * WN, BK = 16;
+  constexpr int LA = (BM * BK) / 256, LBn = (BK * BN) / 256;
+  __shared__ cfl As[BK][BM + 1];
+  __shared__ cfl Bs[BK][BN + 1];
+  unsigned bid = blockIdx.x;
+  if (xcd_remap) {
+    const unsigned per = gridDim.x / 8;
+    bid = (bid % 8) * per + bid / 8;
+  }
+  const long long item = bid / tilesPerItem;
+  const int tile = bid % tilesPerItem;
+  const int row0 = (tile / tilesN) * BM;
+  const int col0 = (tile % tilesN) * BN;
+  const cd* a = A + item * sA;
+  const cd* b = B + item * sB;
+  cd* c = C + item * sC;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * (16 * WM), wc = (wave & 1) * (16 * WN);
+  const int li = lane & 15, lk = lane >> 4;
+  khx_f4 accR[WM][WN], accI[WM][WN];
+#pragma unroll
+  for (int t = 0; t < WM; ++t)
+#pragma unroll
+    for (int u = 0; u < WN; ++u) {
+      accR[t][u] = khx_f4{0.f, 0.f, 0.f, 0.f};
+      accI[t][u] = khx_f4{0.f, 0.f, 0.f, 0.f};
+    }
+  cfl ra[LA], rb[LBn];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      const int e = tid + 256 * q;
+      const int r = e / BK, kk = e % BK;
+      const int gr = row0 + r, gk = k0 + kk;
+      const cd v = (gr < M && gk < K) ? a[(long long)gr * lda + gk] : mkc(0.0, 0.0);
+      ra[q] = cfl{(float)v.x, (float)v.y};
+    }
+#pragma unroll
+    for (int q = 0; q < LBn; ++q) {
+      const int e = tid + 256 * q;
+      const int kk = e / BN, cc = e % BN;
+      const int gk = k0 + kk, gc = col0 + cc;
+      const cd v = (gk < K && gc < N) ? b[(long long)gk * ldb + gc] : mkc(0.0, 0.0);
+      rb[q] = cfl{(float)v.x, (float)v.y};
+    }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      const int e = tid + 256 * q;
+      As[e % BK][e / BK] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < LBn; ++q) {
+      const int e = tid + 256 * q;
+      Bs[e / BN][e % BN] = rb[q];
+    }
+    __syncthreads();
+    if (k0 + BK < K) fetch(k0 + BK);
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; ++kq) {
+      cfl av[WM], bv[WN];
+#pragma unroll
+      for (int t = 0; t < WM; ++t) av[t] = As[kq * 4 + lk][wr + t * 16 + li];
+#pragma unroll
+      for (int u = 0; u < WN; ++u) bv[u] = Bs[kq * 4 + lk][wc + u * 16 + li];
+#pragma unroll
+      for (int t = 0; t < WM; ++t) {
+        const float nai = -av[t].y;
+#pragma unroll
+        for (int u = 0; u < WN; ++u) {
+          accR[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t].x, bv[u].x, accR[t][u], 0, 0, 0);
+          accR[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai, bv[u].y, accR[t][u], 0, 0, 0);
+          accI[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t].x, bv[u].y, accI[t][u], 0, 0, 0);
+          accI[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t].y, bv[u].x, accI[t][u], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+#pragma unroll
+  for (int t = 0; t < WM; ++t)
+#pragma unroll
+    for (int u = 0; u < WN; ++u) {
+      const int gc = col0 + wc + u * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = row0 + wr + t * 16 + F32_ROW(lk, r);
+        if (gr >= M || gc >= N) continue;
+        cd v = alpha * mkc((double)accR[t][u][r], (double)accI[t][u][r]);
+        cd* dst = c + (long long)gr * ldc + gc;
+        if (use_beta) v = fmac(v, beta, *dst);
+        *dst = v;
+      }
+    }
+}
+
+int zgemm_f32(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
+  if (batch <= 0 || M <= 0 || N <= 0) return 0;
+  const bool small = M <= 96 || N <= 96;
+  const int BM = 32, BN = small ? 32 : 64;
+  const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
+  const long long blocks = (long long)tilesM * tilesN * batch;
+  KHX_REQUIRE(blocks < (1ll << 31), "zgemm_f32: grid too large");
+  ProfScope ps("zgemm_kernel", s);
+  if (small)
+    zgemm_f32mfma_kernel<1, 1><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+        M, N, K, alpha, A.p, A.ld, A.stride, B.p, B.ld, B.stride, beta, C.p, C.ld, C.stride, tilesN,
+        tilesM * tilesN, (blocks % 8 == 0) ? 1 : 0);
+  else
+    zgemm_f32mfma_kernel<1, 2><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+        M, N, K, alpha, A.p, A.ld, A.stride, B.p, B.ld, B.stride, beta, C.p, C.ld, C.stride, tilesN,
+        tilesM * tilesN, (blocks % 8 == 0) ? 1 : 0);
+  KHX_LAUNCH_CHECK();
+  return 0;
+}
+
 // ---- tile selection.  Eight kernels (all 256 threads): FMA tiles
 //   0: 64 x 64 (4 x 4 per thread), 1: 32 x 32 (2 x 2), 2: 64 x 32 (4 x 2), 3: 32 x 64 (2 x 4);
 // MFMA tiles 4: 64 x 64, 5: 32 x 32, 6: 64 x 32, 7: 32 x 64.
@@ -507,6 +648,10 @@ __global__ __launch_bounds__(GESV_THREADS) void zgesv_kernel(int n, int m, cd* A
 int zgesv(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info) {
   if (batch <= 0 || n <= 0) return 0;
   const int path = lu_path(n);
+  if (path == 2 && m >= 64 && batch <= 65535 && A.ld == n && A.stride != 0 && B.stride != 0) {
+    const char* mixed = getenv("KHX_LU_MIXED");   // mixed-precision solve (zgesv_refined), off by default
+    if (mixed && atoi(mixed) != 0) return zgesv_refined(s, n, m, A, B, batch, info);
+  }
   if (path == 2 && (batch == 1 || (A.stride != 0 && B.stride != 0))) return zlu_multi(s, n, m, A, B, batch, nullptr, info, false);
   if (path >= 1) return zlu_blocked(s, n, m, A, B, batch, nullptr, info, false);
   KHX_REQUIRE(A.stride != 0 && B.stride != 0 || batch == 1, "zgesv: operands cannot be broadcast");

@@ -18,6 +18,8 @@ inline Mat sub(Mat m, int r, int c) { return Mat{m.p + (long long)r * m.ld + c, 
 int zgemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch);
 // C = alpha A^T (A stored K x M) * B + beta C
 int zgemm_ta(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch);
+// C = alpha A B + beta C with the product on the FP32 matrix cores (operands rounded to complex float)
+int zgemm_f32(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch);
 // C = alpha op(A) op(B) + beta C with op 0 = X, 1 = X^T, 2 = X^H
 int zgemm_op(hipStream_t s, int opa, int opb, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C,
              int batch);
@@ -38,6 +40,8 @@ bool lu_blocked_fits(int n);
 // multi-launch blocked LU (panel kernel + batched zgemm trailing updates)
 bool lu_multi_fits(int n);
 int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL);
+// zgesv with FP32-product triangular solves and one FP64 residual refinement step (m >= 64)
+int zgesv_refined(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info);
 // LU path selection: 0 = unblocked, 1 = one-workgroup blocked, 2 = multi-launch blocked
 int lu_path(int n);
 // beam front end: paraxial beam fields on points, point DFT amplitudes

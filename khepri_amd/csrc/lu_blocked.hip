@@ -12,6 +12,7 @@
 // way (NB x NB diagonal solves, then mini-GEMM updates of the rows above).
 // Every element of the trailing matrix is read and written once per panel
 // instead of once per column.
+#include <mutex>
 #include <utility>
 
 #include "khx_internal.h"
@@ -675,7 +676,15 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_lower_block_kernel(int n, int 
 //   L: X1 = L11^-1 B1;  B2 -= L21 X1;  X2 = L22^-1 B2
 //   U: X2 = U22^-1 B2;  B1 -= U12 X2;  X1 = U11^-1 B1
 // Ranges of at most 2 NB rows use the NB-row block kernels directly.
-template <int NB>
+// GEMM of the triangular-solve updates: FP64 matrix cores, or FP32 ones in
+// the mixed-precision solve (zgesv_refined)
+template <bool F32>
+static int trsm_gemm(hipStream_t s, int M, int N, int K, cd alpha, Mat A, Mat B, cd beta, Mat C, int batch) {
+  if constexpr (F32) return zgemm_f32(s, M, N, K, alpha, A, B, beta, C, batch);
+  return zgemm(s, M, N, K, alpha, A, B, beta, C, batch);
+}
+
+template <int NB, bool F32 = false>
 static int trsm_lower_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int r0, int r1) {
   const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
   if (r1 - r0 <= 2 * NB) {
@@ -685,21 +694,21 @@ static int trsm_lower_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, 
           min(n, j0 + nb), m, j0, A.p, A.ld, A.stride, B.p, B.ld, B.stride);
       KHX_LAUNCH_CHECK();
       if (j0 + nb < r1)
-        KHX_TRY(zgemm(s, r1 - j0 - nb, m, nb, mone, Mat{A.p + (long long)(j0 + nb) * A.ld + j0, A.ld, A.stride},
+        KHX_TRY(trsm_gemm<F32>(s, r1 - j0 - nb, m, nb, mone, Mat{A.p + (long long)(j0 + nb) * A.ld + j0, A.ld, A.stride},
                       Mat{B.p + (long long)j0 * B.ld, B.ld, B.stride}, one,
                       Mat{B.p + (long long)(j0 + nb) * B.ld, B.ld, B.stride}, batch));
     }
     return 0;
   }
   const int mid = r0 + ((r1 - r0) / 2 + NB - 1) / NB * NB;
-  KHX_TRY(trsm_lower_rec<NB>(s, n, m, A, B, batch, r0, mid));
-  KHX_TRY(zgemm(s, r1 - mid, m, mid - r0, mone, Mat{A.p + (long long)mid * A.ld + r0, A.ld, A.stride},
+  KHX_TRY((trsm_lower_rec<NB, F32>(s, n, m, A, B, batch, r0, mid)));
+  KHX_TRY(trsm_gemm<F32>(s, r1 - mid, m, mid - r0, mone, Mat{A.p + (long long)mid * A.ld + r0, A.ld, A.stride},
                 Mat{B.p + (long long)r0 * B.ld, B.ld, B.stride}, one, Mat{B.p + (long long)mid * B.ld, B.ld, B.stride},
                 batch));
-  return trsm_lower_rec<NB>(s, n, m, A, B, batch, mid, r1);
+  return trsm_lower_rec<NB, F32>(s, n, m, A, B, batch, mid, r1);
 }
 
-template <int NB>
+template <int NB, bool F32 = false>
 static int trsm_upper_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int r0, int r1) {
   const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
   if (r1 - r0 <= 2 * NB) {
@@ -712,18 +721,18 @@ static int trsm_upper_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, 
           min(n, j0 + nb), m, j0, A.p, A.ld, A.stride, B.p, B.ld, B.stride);
       KHX_LAUNCH_CHECK();
       if (j0 > r0)
-        KHX_TRY(zgemm(s, j0 - r0, m, nb, mone, Mat{A.p + (long long)r0 * A.ld + j0, A.ld, A.stride},
+        KHX_TRY(trsm_gemm<F32>(s, j0 - r0, m, nb, mone, Mat{A.p + (long long)r0 * A.ld + j0, A.ld, A.stride},
                       Mat{B.p + (long long)j0 * B.ld, B.ld, B.stride}, one,
                       Mat{B.p + (long long)r0 * B.ld, B.ld, B.stride}, batch));
     }
     return 0;
   }
   const int mid = r0 + ((r1 - r0) / 2 + NB - 1) / NB * NB;
-  KHX_TRY(trsm_upper_rec<NB>(s, n, m, A, B, batch, mid, r1));
-  KHX_TRY(zgemm(s, mid - r0, m, r1 - mid, mone, Mat{A.p + (long long)r0 * A.ld + mid, A.ld, A.stride},
+  KHX_TRY((trsm_upper_rec<NB, F32>(s, n, m, A, B, batch, mid, r1)));
+  KHX_TRY(trsm_gemm<F32>(s, mid - r0, m, r1 - mid, mone, Mat{A.p + (long long)r0 * A.ld + mid, A.ld, A.stride},
                 Mat{B.p + (long long)mid * B.ld, B.ld, B.stride}, one, Mat{B.p + (long long)r0 * B.ld, B.ld, B.stride},
                 batch));
-  return trsm_upper_rec<NB>(s, n, m, A, B, batch, r0, mid);
+  return trsm_upper_rec<NB, F32>(s, n, m, A, B, batch, r0, mid);
 }
 
 // Row interchanges of zgetrf (ipiv, 0-based, applied in order k = 0 .. n-1)
@@ -835,7 +844,8 @@ static int lu_rec(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipi
 bool lu_multi_fits(int n) { return n >= 1; }  // larger systems keep the panel in global memory
 
 template <int NB>
-static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL) {
+static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL,
+                        bool f32 = false) {
   if (batch <= 0 || n <= 0) return 0;
   KHX_REQUIRE(A.stride != 0 && (m == 0 || B.stride != 0) || batch == 1, "zlu_multi: operands cannot be broadcast");
   KHX_REQUIRE(batch <= 65535, "zlu_multi: at most 65535 systems per call");
@@ -860,6 +870,10 @@ static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, in
     // the TRSM / GEMM updates read L in final row order: interchanges always reach the factored columns
     KHX_TRY(lu_rec<NB>(s, n, m, A, B, batch, ipiv, info, mode | 1, gp, lds, 0, n));
     if (m <= 0) return 0;
+    if (f32) {
+      KHX_TRY((trsm_lower_rec<NB, true>(s, n, m, A, B, batch, 0, n)));
+      return trsm_upper_rec<NB, true>(s, n, m, A, B, batch, 0, n);
+    }
     KHX_TRY(trsm_lower_rec<NB>(s, n, m, A, B, batch, 0, n));
     return trsm_upper_rec<NB>(s, n, m, A, B, batch, 0, n);
   }
@@ -880,6 +894,10 @@ static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, in
     }
   }
   if (m <= 0) return 0;
+  if (rec && f32) {
+    KHX_TRY((trsm_lower_rec<NB, true>(s, n, m, A, B, batch, 0, n)));
+    return trsm_upper_rec<NB, true>(s, n, m, A, B, batch, 0, n);
+  }
   if (rec) {
     KHX_TRY(trsm_lower_rec<NB>(s, n, m, A, B, batch, 0, n));
     return trsm_upper_rec<NB>(s, n, m, A, B, batch, 0, n);
@@ -907,6 +925,66 @@ int zlu_multi(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, i
   const int nb = env ? atoi(env) : 16;
   if (nb == 32) return zlu_multi_nb<32>(s, n, m, A, B, batch, ipiv, info, keepL);
   return zlu_multi_nb<16>(s, n, m, A, B, batch, ipiv, info, keepL);
+}
+
+// Mixed-precision solve for 64 or more right-hand sides (KHX_LU_MIXED=1, off
+// by default: slower here, where the FP64 matrix cores run at half the FP32
+// rate): the FP64 LU with partial pivoting of zlu_multi, triangular solves
+// whose GEMM updates run on the FP32 matrix cores (zgemm_f32), then one FP64
+// residual refinement step:
+//   X0 = U^-1 L^-1 P B (FP32 products),  R = B - A X0 (FP64),  X = X0 + U^-1 L^-1 P R.
+// The copies of A and B and the pivots live in a grow-only buffer of the
+// library, one per device, allocated on first use (not for graph capture).
+namespace {
+struct RefineBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+std::mutex g_refine_mu;
+RefineBuf g_refine[16];
+}  // namespace
+
+static int refine_buffer(size_t bytes, void** out) {
+  int dev = 0;
+  KHX_CHECK_HIP(hipGetDevice(&dev));
+  KHX_REQUIRE(dev >= 0 && dev < 16, "zgesv_refined: device index out of range");
+  std::lock_guard<std::mutex> lk(g_refine_mu);
+  RefineBuf& b = g_refine[dev];
+  if (b.cap < bytes) {
+    if (b.p) KHX_CHECK_HIP(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    KHX_CHECK_HIP(hipMalloc(&b.p, bytes));
+    b.cap = bytes;
+  }
+  *out = b.p;
+  return 0;
+}
+
+int zgesv_refined(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* info) {
+  if (batch <= 0 || n <= 0 || m <= 0) return 0;
+  KHX_REQUIRE(A.ld == n && A.stride != 0 && B.stride != 0, "zgesv_refined: dense, non-broadcast systems only");
+  KHX_REQUIRE(m >= 64, "zgesv_refined: needs 64 or more right-hand sides");
+  const long long nn = (long long)n * n, nm = (long long)n * m;
+  const size_t bytes = (size_t)batch * (nn + nm) * sizeof(cd) + (size_t)batch * n * sizeof(int) + 256;
+  void* buf = nullptr;
+  KHX_TRY(refine_buffer(bytes, &buf));
+  cd* A0 = static_cast<cd*>(buf);
+  cd* R = A0 + (long long)batch * nn;
+  int* ipiv = reinterpret_cast<int*>(R + (long long)batch * nm);
+  const Mat mA0{A0, n, nn}, mR{R, m, nm};
+  const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
+  KHX_TRY(zcopy(s, mA0, A, n, n, batch));
+  KHX_TRY(zcopy(s, mR, B, n, m, batch));
+  KHX_TRY(zlu_multi_nb<16>(s, n, m, A, B, batch, ipiv, info, true, true));  // X0 in B
+  KHX_TRY(zgemm(s, n, m, n, mone, mA0, B, one, mR, batch));                  // R = B - A X0
+  const size_t lds = (size_t)n * m * sizeof(cd);
+  const int staged = lds <= 64 * 1024 ? 1 : 0;
+  zlaswp_kernel<<<dim3(batch), dim3(256), staged ? lds : 0, s>>>(n, m, ipiv, R, m, nm, staged);
+  KHX_LAUNCH_CHECK();
+  KHX_TRY((trsm_lower_rec<16, true>(s, n, m, A, mR, batch, 0, n)));
+  KHX_TRY((trsm_upper_rec<16, true>(s, n, m, A, mR, batch, 0, n)));
+  return zaxpby(s, B, one, B, one, mR, n, m, batch);
 }
 
 // KHX_LU_PATH=0/1/2 forces a path; by default the multi-launch variant takes

@@ -51,7 +51,12 @@ def test_disc_sweep_reference_protocol(gpu):
     np.testing.assert_allclose(cl.Stot.cpu().numpy()[0], g["Stot"], atol=1e-10)
 
 
-def test_disc_bz_batched_with_nan_pattern(gpu):
+@pytest.mark.parametrize("mixed", ["0", "1"])
+def test_disc_bz_batched_with_nan_pattern(gpu, monkeypatch, mixed):
+    """The c2 golden set; mixed = "1" runs the layer solves through the
+    mixed-precision zgesv (FP32-MFMA triangular-solve products, one FP64
+    residual refinement; KHX_LU_MIXED=1), held to the same 1e-9."""
+    monkeypatch.setenv("KHX_LU_MIXED", mixed)
     g = golden("disc_bz.npz")
     cfg = W.disc_stack_bz()
     idx = g["index"]

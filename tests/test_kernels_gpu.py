@@ -374,6 +374,30 @@ LU_VARIANTS = [{"KHX_LU_NB": "32"}, {"KHX_LU_PATH": "0"}, {"KHX_LU_PATH": "1"}, 
                {"KHX_LU_PANEL_REG": "0"}]
 
 
+@pytest.mark.parametrize("n,m", [(162, 64), (162, 324), (250, 100)])
+def test_lu_mixed_refined(gpu, monkeypatch, n, m):
+    """KHX_LU_MIXED=1: FP64 LU, triangular-solve products on the FP32 matrix
+    cores (complex-float operands, error ~1e-7), one FP64 residual refinement
+    step; the solution against numpy at 1e-10 of its size, and the FP32
+    products alone (before refinement) would miss that by orders of magnitude."""
+    from khepri_amd import ops
+    monkeypatch.setenv("KHX_LU_MIXED", "1")
+    rng = np.random.default_rng(n * m)
+    A = crand(rng, 3, n, n) + np.sqrt(n) * np.eye(n)
+    A[2] = crand(rng, n, n)            # a general random matrix too
+    B = crand(rng, 3, n, m)
+    ref = np.linalg.solve(A, B)
+    X, info = ops.solve(dev(A, gpu), dev(B, gpu))
+    assert (info.cpu().numpy() == 0).all()
+    X = X.cpu().numpy()
+    for b in range(3):
+        err = np.abs(X[b] - ref[b]).max() / np.abs(ref[b]).max()
+        assert err < 1e-10, (b, err)
+    monkeypatch.setenv("KHX_LU_MIXED", "0")
+    X64, _ = ops.solve(dev(A, gpu), dev(B, gpu))
+    assert np.abs(X64.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-11
+
+
 @pytest.mark.parametrize("n", [97, 162, 250, 338, 600])
 def test_lu_register_panel_matches_lds_panel(gpu, monkeypatch, n):
     """The register-resident panel kernel (panel rows in registers, up to 512
