@@ -165,6 +165,54 @@ def grad_grid_rate(steps=2):
             "sample": f"{steps} steps of {nf} frequencies, dT/d eps over 4 x 64 x 64 pixels each, after one warm-up step"}
 
 
+def mixed_leg(nsolves, R64, T64, idx_all):
+    """The mixed-precision path (KHX_LU_MIXED=1: FP32-MFMA products in the
+    layer solves, one FP64 residual refinement step) on an evenly spread
+    subset of the c2 workload, timed over one step after a warm-up step, and
+    its max |dR|, |dT| against the FP64 line's own values for those solves."""
+    import torch
+    cl, _, idx = build_crystal(nsolves)
+    pos = np.minimum(np.searchsorted(idx_all, idx), len(idx_all) - 1)
+    same = idx_all[pos] == idx          # solves that the FP64 line also ran
+    os.environ["KHX_LU_MIXED"] = "1"
+    try:
+        cl.solve()
+        cl.poynting_flux_end()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cl.solve()
+        R, T = cl.poynting_flux_end()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+    finally:
+        del os.environ["KHX_LU_MIXED"]
+    dR = dT = 0.0
+    for r, t, p, ok in zip(R, T, pos, same):
+        if ok:
+            dR = parity_max(dR, R64[p], r)
+            dT = parity_max(dT, T64[p], t)
+    return {"value": round(len(idx) / el, 3), "unit": "solves/s", "solves": int(len(idx)),
+            "max_abs_dR_vs_fp64": dR, "max_abs_dT_vs_fp64": dT,
+            "precision": "mixed: FP64 LU, FP32-MFMA triangular-solve products, one FP64 residual refinement"}
+
+
+def graph_leg(cl, steps):
+    """The c2 step replayed from one captured HIP graph (khepri_amd.graph.SolveGraph:
+    all chunks of the solve and the R/T reduction), timed over `steps` replays."""
+    import torch
+    from khepri_amd.graph import SolveGraph
+    g = SolveGraph(cl)
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"value": round(cl.batch * steps / el, 3), "unit": "solves/s", "steps": steps,
+            "ms_per_step": round(el / steps * 1e3, 3)}
+
+
 def build_crystal(nsolves):
     from khepri_amd import Crystal, workloads
     cfg = workloads.disc_stack_bz()
@@ -570,6 +618,10 @@ def main():
     ap.add_argument("--b1-sample", type=int, default=24, help="solves timed one at a time for the plain batch-1 line")
     ap.add_argument("--grad-steps", type=int, default=2,
                     help="steps of the adjoint workload (16 grad evals each) timed after the main line (0: skip)")
+    ap.add_argument("--mixed-solves", type=int, default=8192,
+                    help="solves of the mixed-precision leg (KHX_LU_MIXED=1) after the main line (0: skip)")
+    ap.add_argument("--graph-steps", type=int, default=1,
+                    help="replays of the whole step captured as one HIP graph, after the main line (0: skip)")
     ap.add_argument("--cpu-workers", type=int, default=16,
                     help="processes of the process-pool CPU baseline (capped at the usable cores; 0: skip both "
                          "multi-core modes)")
@@ -676,6 +728,12 @@ def main():
     if args.grad_steps > 0:
         out["grad_grid"] = grad_grid_rate(args.grad_steps)
         print("[bench] grad_grid done", file=sys.stderr, flush=True)
+    if args.mixed_solves > 0:
+        out["mixed_batched"] = mixed_leg(min(args.mixed_solves, nsol), np.asarray(R), np.asarray(T), idx)
+        print("[bench] mixed leg done", file=sys.stderr, flush=True)
+    if args.graph_steps > 0:
+        out["graph_replay"] = graph_leg(cl, args.graph_steps)
+        print("[bench] graph replay done", file=sys.stderr, flush=True)
     if args.cpu_sample > 0:
         Rn, Tn = np.asarray(R), np.asarray(T)
         cb, dR, dT = cpu_baseline(cfg, Rn, Tn, idx, args.cpu_sample)
