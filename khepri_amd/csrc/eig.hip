@@ -500,6 +500,81 @@ __device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, 
   }
 }
 
+// Eigenvectors of the upper-triangular Schur factor T (global memory), one
+// thread per eigenvector: column k of X has X[k][k] = 1, zeros below, and
+// for i < k, from the bottom up,
+//   X[i][k] = (-T[i][k] - sum_{i<m<k} T[i][m] X[m][k]) / (T[i][i] - T[k][k])
+// (ztrevc's back substitution, with its small-pivot perturbation).  Rows are
+// taken in aligned blocks of EV_NB from the bottom: the block's rows live in
+// registers while they are finished, then every row above takes the block's
+// contributions in one read-modify-write.  The operations on each element
+// are those of the row-by-row sweep and in the same order (subtractions in
+// decreasing m, the division last), so the results are identical to it; the
+// traffic is one pass over the rows above per block instead of one per row,
+// and no barrier is needed (a thread only touches its own column).
+constexpr int EV_NB = 16;
+template <int EIG_THREADS>
+__device__ __forceinline__ void eigvec_columns(const cd* __restrict__ T, cd* __restrict__ X, int n, double smlnum) {
+  constexpr int EIG_WAVES = EIG_THREADS / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // rows k.. of every column: X[k][k] = 1, zeros below
+  for (int j = wid; j < n; j += EIG_WAVES)
+    for (int kk = lane; kk <= j; kk += 64) X[(long long)j * n + kk] = mkc(j == kk ? 1.0 : 0.0, 0.0);
+  for (int k0w = wid * 64; k0w < n; k0w += EIG_THREADS) {  // a wavefront takes columns k0 .. k0 + 63
+    const int k0 = __builtin_amdgcn_readfirstlane(k0w);
+    const int k = k0 + lane;
+    const bool valid = k < n;
+    const cd lk = valid ? T[(long long)k * n + k] : mkc(0.0, 0.0);
+    const double smin = fmax(ULP * cabs1(lk), smlnum);
+    const int kmax = min(n - 1, k0 + 63);
+    const int kfirst = valid && k >= 1 ? (k - 1) / EV_NB : -1;  // this column's first (lowest) block
+    for (int b = kmax >= 1 ? (kmax - 1) / EV_NB : -1; b >= 0; --b) {  // uniform over the wavefront
+      const int r0 = b * EV_NB;
+      const bool act = valid && r0 < k;
+      const bool first = b == kfirst;
+      // the block's rows; rows k.. (in the column's first block) stay 0, so
+      // that they change nothing in the updates of the rows above (x - t 0 = x)
+      cd x[EV_NB];
+#pragma unroll
+      for (int r = 0; r < EV_NB; ++r) {
+        const int j = r0 + r;
+        x[r] = mkc(0.0, 0.0);
+        if (act && j < k) {
+          const cd t = first ? T[(long long)j * n + k] : X[(long long)j * n + k];
+          x[r] = first ? -t : t;
+        }
+      }
+#pragma unroll
+      for (int r = EV_NB - 1; r >= 0; --r) {
+        const int j = r0 + r;
+        if (act && j < k) {
+          cd d = T[(long long)j * n + j] - lk;
+          if (cabs1(d) < smin) d = mkc(smin, 0.0);
+          x[r] = divc(x[r], d);
+#pragma unroll
+          for (int i = 0; i < r; ++i) x[i] = fmsc(x[i], T[(long long)(r0 + i) * n + j], x[r]);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < EV_NB; ++r)
+        if (act && r0 + r < k) X[(long long)(r0 + r) * n + k] = x[r];
+      // rows above the block (a uniform loop: row i of T's block columns is the
+      // same for every lane)
+      for (int i = 0; i < r0; ++i) {
+        const cd* ti = T + (long long)i * n + r0;
+        cd v = mkc(0.0, 0.0);
+        if (act) {
+          const cd t = first ? T[(long long)i * n + k] : X[(long long)i * n + k];
+          v = first ? -t : t;
+        }
+#pragma unroll
+        for (int r = EV_NB - 1; r >= 0; --r) v = fmsc(v, ti[r], x[r]);
+        if (act) X[(long long)i * n + k] = v;
+      }
+    }
+  }
+}
+
 // Deferred rotations of up to EIG_KLOG QR sweeps, applied in one pass per line.
 // Sweep q rotated the pairs (p, p+1), p in [lo_q, hi_q - 1]; its rotation at p
 // must follow the previous sweep's at p - 1 .. p + 1, so at step t the pass
@@ -1700,6 +1775,10 @@ __device__ __forceinline__ void eig_body(int n, cd* A, long long sA, cd* Ztg, cd
 
   // ------------------------------------------------------------ 3. eigenvectors of T
   if (STAGES & 4) {
+  if (!in_lds && !(flags & 2)) {  // one thread per eigenvector (flags bit 1: the row-by-row sweep below)
+    eigvec_columns<EIG_THREADS>(H, X, n, smlnum);
+    if (tp && tid == 0) tp[3] = wall_clock64();
+  } else {
   // X[i][k] = -T[i][k] (i < k), 1 (i == k), 0 (i > k)
   for (int i = wid; i < n; i += EIG_WAVES)
     for (int k = lane; k < n; k += 64) {
@@ -1731,6 +1810,7 @@ __device__ __forceinline__ void eig_body(int n, cd* A, long long sA, cd* Ztg, cd
     for (int i = wid; i < n; i += EIG_WAVES)
       for (int j = lane; j < n; j += 64) Hg[(long long)i * n + j] = hl[i * n + j];
   if (tp && tid == 0) tp[3] = wall_clock64();
+  }
   }  // stage 3
 }
 
@@ -1933,10 +2013,13 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   const char* cw_env = getenv("KHX_EIG_CHASE_WAVE");
   // bit 2: register chase (qr_sweep_reg; default on, KHX_EIG_REG_CHASE=0 selects the LDS chase)
   const char* rc_env = getenv("KHX_EIG_REG_CHASE");
+  // bit 1: row-by-row eigenvector sweep (KHX_EIG_VEC_ROWS=1) instead of one thread per eigenvector
+  const char* vr_env = getenv("KHX_EIG_VEC_ROWS");
   const char* cap_env = getenv("KHX_EIG_ITMAX");
   const int cap = cap_env ? atoi(cap_env) : 0;
   KHX_REQUIRE(cap >= 0 && cap < (1 << 22), "zeig: KHX_EIG_ITMAX out of range");
-  const int flags = ((cw_env && atoi(cw_env) == 0) ? 0 : 1) | ((rc_env && atoi(rc_env) == 0) ? 0 : 4) | (cap << 8);
+  const int flags = ((cw_env && atoi(cw_env) == 0) ? 0 : 1) | ((vr_env && atoi(vr_env) != 0) ? 2 : 0) |
+                    ((rc_env && atoi(rc_env) == 0) ? 0 : 4) | (cap << 8);
   const size_t lds = eig_lds_bytes(n, threads) + (pad_env ? (size_t)atol(pad_env) : 0);
   KHX_REQUIRE(lds <= 150 * 1024, "zeig: n too large for the LDS work vectors");
   {

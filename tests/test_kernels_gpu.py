@@ -374,6 +374,37 @@ LU_VARIANTS = [{"KHX_LU_NB": "32"}, {"KHX_LU_PATH": "0"}, {"KHX_LU_PATH": "1"}, 
                {"KHX_LU_PANEL_REG": "0"}]
 
 
+@pytest.mark.parametrize("pw", [(7, 7), (9, 9), (11, 11), (13, 13)])
+def test_eig_vectors_per_column_match_row_sweep(gpu, monkeypatch, pw):
+    """The eigenvector stage with one thread per eigenvector (row blocks in
+    registers) against the row-by-row sweep (KHX_EIG_VEC_ROWS=1): same
+    operations in the same order, so eigenvalues and eigenvectors agree bit
+    for bit; and the residual |M W - W diag(lam)| stays at rounding level."""
+    from khepri_amd import ops
+    pix = W.disc_pixmap()
+    C = rcwa.convolution_matrix(pix, pw)
+    g = rcwa.g_vectors(pw, np.eye(2))
+    mats = []
+    for wl, kp in [(2.0, (0.1, 0.3)), (1.7, (0.0, 0.0))]:
+        kx, ky, _ = rcwa.k_vectors(g, kp, wl)
+        P, Q = rcwa.structured_pq(kx, ky, C)
+        mats.append(P @ Q)
+    n = mats[0].shape[0]
+    mats.append(crand(np.random.default_rng(n), n, n))
+    M = np.stack(mats)
+    out = {}
+    for rows in ("1", "0"):
+        monkeypatch.setenv("KHX_EIG_VEC_ROWS", rows)
+        lam, Wm, info = ops.eig(dev(M, gpu))
+        assert (info.cpu().numpy() == 0).all()
+        out[rows] = (lam.cpu().numpy(), Wm.cpu().numpy())
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
+    np.testing.assert_array_equal(out["0"][1], out["1"][1])
+    lam, Wm = out["0"]
+    for b in range(len(M)):
+        assert np.abs(M[b] @ Wm[b] - Wm[b] * lam[b]).max() / np.abs(M[b]).max() < 1e-11
+
+
 @pytest.mark.parametrize("n,m", [(162, 64), (162, 324), (250, 100)])
 def test_lu_mixed_refined(gpu, monkeypatch, n, m):
     """KHX_LU_MIXED=1: FP64 LU, triangular-solve products on the FP32 matrix
