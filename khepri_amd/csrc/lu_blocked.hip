@@ -68,6 +68,7 @@ __device__ __forceinline__ void mini_gemm_sub(const cd* Ls, int lrow0, const cd*
   }
 }
 
+template <int TH = LB_THREADS>
 __device__ __forceinline__ void argmax_abs1(double& best, int& bi, cd& bv, double* rv, int* ri, cd* rc) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -91,7 +92,7 @@ __device__ __forceinline__ void argmax_abs1(double& best, int& bi, cd& bv, doubl
   bi = ri[0];
   bv = rc[0];
 #pragma unroll
-  for (int w = 1; w < LB_THREADS / 64; ++w)
+  for (int w = 1; w < TH / 64; ++w)
     if (rv[w] > best || (rv[w] == best && ri[w] < bi)) {
       best = rv[w];
       bi = ri[w];
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_blocked_kernel(int n, int m, c
 // After a panel is factored: its status word, then its interchanges on the
 // other columns (and B) and the triangular solve with L11 (pan: the panel's
 // first rows, unit lower, row stride pld), one thread per column.
-template <int NB>
+template <int NB, int TH = LB_THREADS>
 __device__ __forceinline__ void panel_apply(int n, int m, int j0, int nb, cd* a, int lda, cd* bm, int ldb,
                                             const cd* pan, long long pld, const int* perm, int* info,
                                             long long item, int first_zero, int keepL, int cend) {
@@ -338,7 +339,7 @@ __device__ __forceinline__ void panel_apply(int n, int m, int j0, int nb, cd* a,
   // one thread per column
   const int nleft = (keepL & 1) ? j0 : 0;
   const int nright = n - j0 - nb;
-  for (int e = tid; e < nleft + nright + m; e += LB_THREADS) {
+  for (int e = tid; e < nleft + nright + m; e += TH) {
     cd* col;
     long long ld;
     if (e < nleft) {
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(LB_THREADS) void zlu_panel_kernel(int n, int m, int
 
 // Pivot step JJ of the register panel (a template parameter, so that every
 // index into the row registers is a constant).
-template <int NB, int RPT, int JJ>
+template <int NB, int RPT, int JJ, int TH>
 __device__ __forceinline__ void panel_step(cd (&v)[RPT][NB], int nb, int pr, int j0, int n, long long item, int* ipiv,
                                            int* perm, cd* prow, cd* jrow, double* rv, int* ri, cd* rc,
                                            int& first_zero) {
@@ -500,7 +501,7 @@ __device__ __forceinline__ void panel_step(cd (&v)[RPT][NB], int nb, int pr, int
   cd bv = mkc(0.0, 0.0);
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    const int r = tid + k * LB_THREADS;
+    const int r = tid + k * TH;
     if (r >= JJ && r < pr) {
       const double av = cabs1(v[k][JJ]);
       if (av > best) {
@@ -510,12 +511,12 @@ __device__ __forceinline__ void panel_step(cd (&v)[RPT][NB], int nb, int pr, int
       }
     }
   }
-  argmax_abs1(best, bi, bv, rv, ri, rc);
+  argmax_abs1<TH>(best, bi, bv, rv, ri, rc);
   const int p = bi < pr ? bi : JJ;  // NaN column: keep the diagonal
   // rows p (the new pivot row) and JJ trade places through LDS
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    const int r = tid + k * LB_THREADS;
+    const int r = tid + k * TH;
     if (r == p) {
 #pragma unroll
       for (int c = 0; c < NB; ++c) prow[c] = v[k][c];
@@ -535,7 +536,7 @@ __device__ __forceinline__ void panel_step(cd (&v)[RPT][NB], int nb, int pr, int
   const cd rp = zero ? mkc(0.0, 0.0) : recipc(piv);
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    const int r = tid + k * LB_THREADS;
+    const int r = tid + k * TH;
     if (r == JJ || r == p) {  // one set of loads: row JJ takes the pivot row, row p the old row JJ
       const cd* src = (r == JJ) ? prow : jrow;
 #pragma unroll
@@ -550,11 +551,11 @@ __device__ __forceinline__ void panel_step(cd (&v)[RPT][NB], int nb, int pr, int
   }
 }
 
-template <int NB, int RPT, int... J>
+template <int NB, int RPT, int TH, int... J>
 __device__ __forceinline__ void panel_steps(std::integer_sequence<int, J...>, cd (&v)[RPT][NB], int nb, int pr, int j0,
                                             int n, long long item, int* ipiv, int* perm, cd* prow, cd* jrow,
                                             double* rv, int* ri, cd* rc, int& first_zero) {
-  (panel_step<NB, RPT, J>(v, nb, pr, j0, n, item, ipiv, perm, prow, jrow, rv, ri, rc, first_zero), ...);
+  (panel_step<NB, RPT, J, TH>(v, nb, pr, j0, n, item, ipiv, perm, prow, jrow, rv, ri, rc, first_zero), ...);
 }
 
 // The same panel step with the panel in registers: thread t holds the
@@ -564,13 +565,13 @@ __device__ __forceinline__ void panel_steps(std::integer_sequence<int, J...>, cd
 // which also broadcasts the pivot row).  Pivot choice, arithmetic and
 // output are those of zlu_panel_kernel: the largest |Re| + |Im| below the
 // diagonal, the first such row on ties, NaN columns keep the diagonal.
-template <int NB, int RPT>
-__global__ __launch_bounds__(LB_THREADS, RPT == 1 ? 4 : 2) void zlu_panel_reg_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
+template <int NB, int RPT, int TH = LB_THREADS>
+__global__ __launch_bounds__(TH, TH == 256 ? (RPT == 1 ? 4 : 2) : 1) void zlu_panel_reg_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
                                                                    cd* B, int ldb, long long sB, int* ipiv, int* info,
                                                                    int keepL, int cend) {
-  __shared__ double rv[LB_THREADS / 64];
-  __shared__ int ri[LB_THREADS / 64];
-  __shared__ cd rc[LB_THREADS / 64];
+  __shared__ double rv[TH / 64];
+  __shared__ int ri[TH / 64];
+  __shared__ cd rc[TH / 64];
   __shared__ int perm[NB];
   __shared__ cd prow[NB], jrow[NB];
   // L11 for the column solves, in dynamic LDS (NB (NB + 1) elements): with a
@@ -586,16 +587,16 @@ __global__ __launch_bounds__(LB_THREADS, RPT == 1 ? 4 : 2) void zlu_panel_reg_ke
   cd v[RPT][NB];
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    const int r = tid + k * LB_THREADS;
+    const int r = tid + k * TH;
     const cd* src = a + (long long)(j0 + r) * lda + j0;
 #pragma unroll
     for (int c = 0; c < NB; ++c) v[k][c] = (r < pr && c < nb) ? src[c] : mkc(0.0, 0.0);
   }
-  panel_steps<NB, RPT>(std::make_integer_sequence<int, NB>{}, v, nb, pr, j0, n, item, ipiv, perm, prow, jrow, rv, ri,
+  panel_steps<NB, RPT, TH>(std::make_integer_sequence<int, NB>{}, v, nb, pr, j0, n, item, ipiv, perm, prow, jrow, rv, ri,
                         rc, first_zero);
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    const int r = tid + k * LB_THREADS;
+    const int r = tid + k * TH;
     if (r < pr) {
       cd* dst = a + (long long)(j0 + r) * lda + j0;
 #pragma unroll
@@ -608,7 +609,7 @@ __global__ __launch_bounds__(LB_THREADS, RPT == 1 ? 4 : 2) void zlu_panel_reg_ke
     }
   }
   __syncthreads();
-  panel_apply<NB>(n, m, j0, nb, a, lda, bm, ldb, l11, NB + 1, perm, info, item, first_zero, keepL, cend);
+  panel_apply<NB, TH>(n, m, j0, nb, a, lda, bm, ldb, l11, NB + 1, perm, info, item, first_zero, keepL, cend);
 }
 
 // X block = U11^-1 Y block for the rows j0 .. j0+nb of B (one thread per column)
@@ -800,13 +801,17 @@ static int launch_panel(hipStream_t s, int n, int m, int j0, Mat A, Mat B, int b
                         int cend, bool gp, size_t lds) {
   if constexpr (NB == 16) {
     const int pr = n - j0;
-    if (pr <= 2 * LB_THREADS && panel_reg_enabled()) {
+    const size_t l11 = NB * (NB + 1) * sizeof(cd);
+    if (pr <= 3 * 512 && panel_reg_enabled()) {
       if (pr <= LB_THREADS)
-        zlu_panel_reg_kernel<NB, 1><<<dim3(batch), dim3(LB_THREADS), NB * (NB + 1) * sizeof(cd), s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
-                                                                            B.stride, ipiv, info, mode, cend);
-      else
-        zlu_panel_reg_kernel<NB, 2><<<dim3(batch), dim3(LB_THREADS), NB * (NB + 1) * sizeof(cd), s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
-                                                                            B.stride, ipiv, info, mode, cend);
+        zlu_panel_reg_kernel<NB, 1><<<dim3(batch), dim3(LB_THREADS), l11, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p,
+                                                                              B.ld, B.stride, ipiv, info, mode, cend);
+      else if (pr <= 2 * LB_THREADS)
+        zlu_panel_reg_kernel<NB, 2><<<dim3(batch), dim3(LB_THREADS), l11, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p,
+                                                                              B.ld, B.stride, ipiv, info, mode, cend);
+      else  // up to 1 536 rows: 512 threads, three rows each
+        zlu_panel_reg_kernel<NB, 3, 512><<<dim3(batch), dim3(512), l11, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p,
+                                                                            B.ld, B.stride, ipiv, info, mode, cend);
       KHX_LAUNCH_CHECK();
       return 0;
     }

@@ -429,10 +429,11 @@ def test_lu_mixed_refined(gpu, monkeypatch, n, m):
     assert np.abs(X64.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-11
 
 
-@pytest.mark.parametrize("n", [97, 162, 250, 338, 600])
+@pytest.mark.parametrize("n", [97, 162, 250, 338, 600, 1250])
 def test_lu_register_panel_matches_lds_panel(gpu, monkeypatch, n):
-    """The register-resident panel kernel (panel rows in registers, up to 512
-    rows) and the LDS panel kernel make the same pivot choices and the same
+    """The register-resident panel kernel (panel rows in registers: up to 512
+    rows on 256 threads, up to 1 536 on 512) and the LDS (or, past 150 KB,
+    global-memory) panel kernel make the same pivot choices and the same
     arithmetic: factors, pivots, solutions and status words agree bit for bit,
     including an exactly singular system (zero pivot) and a NaN column."""
     from khepri_amd import ops
