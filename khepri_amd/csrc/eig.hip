@@ -846,8 +846,7 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 // stride nb + 1) of H[b0.., b0..], recorded in crot / srot; the block rows
 // above the core take their column rotations at the end.
 __device__ __forceinline__ void reg_chase_block(cd* hb, int nb, int b0, int l, int ihi, int kstart, int kend,
-                                                cd shift, double* crot, cd* srot, int j) {
-  const int ld = nb + 1;
+                                                cd shift, double* crot, cd* srot, int j, int ld) {
   int q = kstart - b0;  // block index of the current rotation: 0 in the first block, 1 after
   cd x, y;
   if (kstart == l) {
@@ -972,7 +971,7 @@ __device__ __forceinline__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd sh
       if (EIG_HESS_LOAD == 0 || e % nb >= e / nb - 2) hb[(e / nb) * ld + e % nb] = H[(long long)(b0 + e / nb) * n + b0 + e % nb];
     __syncthreads();
     unsigned long long t_c0 = acc ? wall_clock64() : 0;
-    if (tid < 64) reg_chase_block(hb, nb, b0, l, ihi, kstart, kend, shift, crot, srot, tid);
+    if (tid < 64) reg_chase_block(hb, nb, b0, l, ihi, kstart, kend, shift, crot, srot, tid, ld);
     __syncthreads();
     if (acc) acc[1] += wall_clock64() - t_c0;
   }
@@ -1000,7 +999,7 @@ __device__ __forceinline__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd sh
         hb[(e / nbn) * ldn + e % nbn] = H[(long long)(b0n + e / nbn) * n + b0n + e % nbn];
     __syncthreads();
     unsigned long long t_c0 = acc ? wall_clock64() : 0;
-    if (tid < 64) reg_chase_block(hb, nbn, b0n, l, ihi, kstartn, kendn, shift, crot, srot, tid);
+    if (tid < 64) reg_chase_block(hb, nbn, b0n, l, ihi, kstartn, kendn, shift, crot, srot, tid, ldn);
     if (EIG_THREADS == 64 || tid >= 64) {
       // rest of this block: rows l..b0-1 (column rotations) and columns b1n+1..ihi (row rotations);
       // a one-wavefront workgroup does it after the chase
@@ -1688,6 +1687,11 @@ __device__ __forceinline__ void eig_body(int n, cd* A, long long sA, cd* Ztg, cd
       if (tp) acc_t[4] += wall_clock64() - t4;
       continue;
     }
+    if (flags & 4) {
+      // register chase over the whole window (at most 64 columns: one block)
+      if (tid < 64) reg_chase_block(H + l * n + l, ihi - l + 1, l, l, ihi, l, ihi - 1, shift, crot, srot, tid, n);
+      __syncthreads();
+    } else {
     cd x = H[l * n + l] - shift;
     cd y = H[(l + 1) * n + l];
     // bulge chase restricted to the window [l, ihi]
@@ -1743,6 +1747,7 @@ __device__ __forceinline__ void eig_body(int n, cd* A, long long sA, cd* Ztg, cd
         }
       }
       __syncthreads();
+    }
     }
     // deferred: rows above the window, columns right of it, and Z
     const int nabove = l, nright = n - 1 - ihi;

@@ -374,6 +374,41 @@ LU_VARIANTS = [{"KHX_LU_NB": "32"}, {"KHX_LU_PATH": "0"}, {"KHX_LU_PATH": "1"}, 
                {"KHX_LU_PANEL_REG": "0"}]
 
 
+@pytest.mark.parametrize("pw", [(3, 3), (5, 5), (1, 31)])
+def test_eig_small_register_chase_matches_lds_chase(gpu, monkeypatch, pw):
+    """n <= 64 (H in LDS): the register chase over the whole window against
+    the barrier-per-rotation LDS chase (KHX_EIG_REG_CHASE=0), on layer
+    matrices (n = 18, 50, 62) and random ones: the same rotations, so the
+    eigenvalues agree to rounding; residuals at rounding level both ways."""
+    from khepri_amd import ops
+    pix = W.disc_pixmap()
+    C = rcwa.convolution_matrix(pix, pw)
+    g = rcwa.g_vectors(pw, np.eye(2))
+    mats = []
+    for wl, kp in [(2.0, (0.1, 0.3)), (1.7, (0.0, 0.0)), (0.9, (0.4, -0.2))]:
+        kx, ky, _ = rcwa.k_vectors(g, kp, wl)
+        P, Q = rcwa.structured_pq(kx, ky, C)
+        mats.append(P @ Q)
+    n = mats[0].shape[0]
+    rng = np.random.default_rng(n)
+    mats += [crand(rng, n, n), crand(rng, n, n)]
+    M = np.stack(mats)
+    out = {}
+    for rc in ("0", "1"):
+        monkeypatch.setenv("KHX_EIG_REG_CHASE", rc)
+        lam, Wm, info = ops.eig(dev(M, gpu))
+        assert (info.cpu().numpy() == 0).all()
+        out[rc] = (lam.cpu().numpy(), Wm.cpu().numpy())
+        for b in range(len(M)):
+            assert np.abs(M[b] @ out[rc][1][b] - out[rc][1][b] * out[rc][0][b]).max() / np.abs(M[b]).max() < 1e-12
+    for b in range(len(M)):
+        ref = np.linalg.eigvals(M[b])
+        for rc in ("0", "1"):
+            d = np.abs(out[rc][0][b][:, None] - ref[None, :]).min(axis=1).max()
+            assert d < 1e-10 * np.abs(ref).max()
+    np.testing.assert_allclose(out["1"][0], out["0"][0], atol=1e-11 * np.abs(out["0"][0]).max())
+
+
 @pytest.mark.parametrize("pw", [(7, 7), (9, 9), (11, 11), (13, 13)])
 def test_eig_vectors_per_column_match_row_sweep(gpu, monkeypatch, pw):
     """The eigenvector stage with one thread per eigenvector (row blocks in
