@@ -867,10 +867,11 @@ static int zlu_multi_nb(hipStream_t s, int n, int m, Mat A, Mat B, int batch, in
   const char* rs_env = getenv("KHX_LU_RECSOLVE");
   const bool rec = m >= 64 && !(rs_env && atoi(rs_env) == 0);
   const int mode = (keepL || rec ? 1 : 0) | (rec ? 2 : 0);
-  // the factorization itself recursively (deep-K GEMM updates) for n >= 200
+  // the factorization itself recursively (deep-K GEMM updates) for n >= 400
   // when B is absent or solved afterwards (KHX_LU_RECFACTOR=0/1 forces it)
   const char* rf_env = getenv("KHX_LU_RECFACTOR");
-  const bool recf = rf_env ? atoi(rf_env) != 0 : n >= 200;  // measured: panel-by-panel is faster below
+  // measured: with the register panels, panel-by-panel is faster below (n = 242, 64 systems: +6 %)
+  const bool recf = rf_env ? atoi(rf_env) != 0 : n >= 400;
   if ((m <= 0 || rec) && recf) {
     // the TRSM / GEMM updates read L in final row order: interchanges always reach the factored columns
     KHX_TRY(lu_rec<NB>(s, n, m, A, B, batch, ipiv, info, mode | 1, gp, lds, 0, n));
