@@ -9,6 +9,7 @@ only orchestrates.
 from __future__ import annotations
 
 import cmath
+import contextlib
 import os
 from copy import copy
 from types import SimpleNamespace
@@ -241,28 +242,37 @@ class Crystal:
         T = torch.empty_like(R)
         Ro = torch.empty((B, N), dtype=torch.float64, device=dev)
         To = torch.empty_like(Ro)
-        for c0 in range(0, B, chunk):
-            sl = slice(c0, min(B, c0 + chunk))
-            kp, wl = kp_all[sl], wl_all[sl]
-            Layer.solve_batch_many([self.layers[name] for name in names], kp, wl, rt_only=rt_only)
-            self._check_layers(names)
-            if rt_only:
-                c = ops.incident(N, kp, wl, te_all[sl], tm_all[sl], self.epsi)
-                sR, sT, infos = chain.rt_amplitudes([layer.chain_item() for layer in stacked], c)
-                for info in infos:
-                    self._check_info(info, "star-product junction", lu=True)
-                r, t, ro, to = ops.poynting_vec(sR, sT, g, kp, wl, self.epsi, self.epse, orders=True)
-            else:
-                infos = []
-                _, _, Stot = stack_layers(self.expansion.pw, stacked, self.stack_retain_mask, reverse=False,
-                                          infos=infos)
-                for info in infos:
-                    self._check_info(info, "star product", lu=True)
-                r, t, ro, to = ops.poynting(Stot, g, kp, wl, te_all[sl], tm_all[sl], self.epsi, self.epse,
-                                            orders=True)
-            R[sl], T[sl], Ro[sl], To[sl] = r, t, ro, to
-            for name in names:
-                self.layers[name].S = None
+        # the status words of every chunk are checked once, after the last chunk
+        # is enqueued (a check is a host synchronisation, which would leave the
+        # GPU idle while the host enqueues the next chunk); inside an outer
+        # deferral (graph capture) they go to its list instead
+        own = not ops._DEFERRED
+        sink = []
+        with ops.deferred_status(sink) if own else contextlib.nullcontext():
+            for c0 in range(0, B, chunk):
+                sl = slice(c0, min(B, c0 + chunk))
+                kp, wl = kp_all[sl], wl_all[sl]
+                Layer.solve_batch_many([self.layers[name] for name in names], kp, wl, rt_only=rt_only)
+                self._check_layers(names)
+                if rt_only:
+                    c = ops.incident(N, kp, wl, te_all[sl], tm_all[sl], self.epsi)
+                    sR, sT, infos = chain.rt_amplitudes([layer.chain_item() for layer in stacked], c)
+                    for info in infos:
+                        self._check_info(info, "star-product junction", lu=True)
+                    r, t, ro, to = ops.poynting_vec(sR, sT, g, kp, wl, self.epsi, self.epse, orders=True)
+                else:
+                    infos = []
+                    _, _, Stot = stack_layers(self.expansion.pw, stacked, self.stack_retain_mask, reverse=False,
+                                              infos=infos)
+                    for info in infos:
+                        self._check_info(info, "star product", lu=True)
+                    r, t, ro, to = ops.poynting(Stot, g, kp, wl, te_all[sl], tm_all[sl], self.epsi, self.epse,
+                                                orders=True)
+                R[sl], T[sl], Ro[sl], To[sl] = r, t, ro, to
+                for name in names:
+                    self.layers[name].S = None
+        if own:
+            ops.check_status(sink)
         self._RT = (R, T, Ro, To)
 
     def _check_info(self, info, what, lu=False):
