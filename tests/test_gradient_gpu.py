@@ -155,7 +155,7 @@ def test_flux_gradient_small_vs_autograd(gpu, pw, theta, phi):
         layers = [("pixmap", ts[0], depths[0]), ("uniform", 2.25, 0.2), ("pixmap", ts[1], depths[1])]
         Rr, Tr = AG.fluxes(layers, pw, wl, theta=theta, phi=phi)
         Tr.backward()
-        assert abs(T[b] - float(Tr)) < 1e-11 and abs(R[b] - float(Rr)) < 1e-11
+        assert abs(T[b] - float(Tr.detach())) < 1e-11 and abs(R[b] - float(Rr.detach())) < 1e-11
         for j, nm in enumerate(names):
             ref = ts[j].grad.numpy()
             np.testing.assert_allclose(grads[nm][b].cpu().numpy(), ref, atol=1e-9 * np.abs(ref).max())
