@@ -242,6 +242,7 @@ class Crystal:
         T = torch.empty_like(R)
         Ro = torch.empty((B, N), dtype=torch.float64, device=dev)
         To = torch.empty_like(Ro)
+        out = (R, T, Ro, To)
         # the status words of every chunk are checked once, after the last chunk
         # is enqueued (a check is a host synchronisation, which would leave the
         # GPU idle while the host enqueues the next chunk); inside an outer
@@ -250,30 +251,35 @@ class Crystal:
         sink = []
         with ops.deferred_status(sink) if own else contextlib.nullcontext():
             for c0 in range(0, B, chunk):
-                sl = slice(c0, min(B, c0 + chunk))
-                kp, wl = kp_all[sl], wl_all[sl]
-                Layer.solve_batch_many([self.layers[name] for name in names], kp, wl, rt_only=rt_only)
-                self._check_layers(names)
-                if rt_only:
-                    c = ops.incident(N, kp, wl, te_all[sl], tm_all[sl], self.epsi)
-                    sR, sT, infos = chain.rt_amplitudes([layer.chain_item() for layer in stacked], c)
-                    for info in infos:
-                        self._check_info(info, "star-product junction", lu=True)
-                    r, t, ro, to = ops.poynting_vec(sR, sT, g, kp, wl, self.epsi, self.epse, orders=True)
-                else:
-                    infos = []
-                    _, _, Stot = stack_layers(self.expansion.pw, stacked, self.stack_retain_mask, reverse=False,
-                                              infos=infos)
-                    for info in infos:
-                        self._check_info(info, "star product", lu=True)
-                    r, t, ro, to = ops.poynting(Stot, g, kp, wl, te_all[sl], tm_all[sl], self.epsi, self.epse,
-                                                orders=True)
-                R[sl], T[sl], Ro[sl], To[sl] = r, t, ro, to
-                for name in names:
-                    self.layers[name].S = None
+                self._solve_chunk(slice(c0, min(B, c0 + chunk)), names, stacked, rt_only, g, N, out)
         if own:
             ops.check_status(sink)
         self._RT = (R, T, Ro, To)
+
+    def _solve_chunk(self, sl, names, stacked, rt_only, g, N, out):
+        """Sources sl of the batch: layer solves, chain, R/T into out[...][sl]."""
+        kp_all, wl_all, te_all, tm_all = self._sources_dev
+        kp, wl = kp_all[sl], wl_all[sl]
+        Layer.solve_batch_many([self.layers[name] for name in names], kp, wl, rt_only=rt_only)
+        self._check_layers(names)
+        if rt_only:
+            c = ops.incident(N, kp, wl, te_all[sl], tm_all[sl], self.epsi)
+            sR, sT, infos = chain.rt_amplitudes([layer.chain_item() for layer in stacked], c)
+            for info in infos:
+                self._check_info(info, "star-product junction", lu=True)
+            r, t, ro, to = ops.poynting_vec(sR, sT, g, kp, wl, self.epsi, self.epse, orders=True)
+        else:
+            infos = []
+            _, _, Stot = stack_layers(self.expansion.pw, stacked, self.stack_retain_mask, reverse=False,
+                                      infos=infos)
+            for info in infos:
+                self._check_info(info, "star product", lu=True)
+            r, t, ro, to = ops.poynting(Stot, g, kp, wl, te_all[sl], tm_all[sl], self.epsi, self.epse,
+                                        orders=True)
+        R, T, Ro, To = out
+        R[sl], T[sl], Ro[sl], To[sl] = r, t, ro, to
+        for name in names:
+            self.layers[name].S = None
 
     def _check_info(self, info, what, lu=False):
         if info is not None and os.environ.get("KHX_CHECK_INFO", "1") == "1":
