@@ -9,7 +9,7 @@ linspace(0.45, 0.65) x a 32x32 cell-centred Bloch grid = 65 536 solves, each
 the timed region; a step is Crystal.solve() + poynting_flux_end() over all
 solves (R/T copied back to the host).
 
-Prints one JSON line (the driver's contract) with a roofline entry for the
+Prints one JSON line (the benchmark contract) with a roofline entry for the
 dominant kernel (HIP events on its launch stream) and a CPU baseline (the
 oracle on a bounded sample of the same workload, 1 BLAS thread).
 """
@@ -38,7 +38,7 @@ def eig_flops(n):
 
 
 def counted_flops_per_solve(n):
-    """Real flops this build's own op sequence spends per c2 solve (one unique
+    """Real flops the library's own op sequence spends per c2 solve (one unique
     patterned layer, the R/T-only chain), complex multiply-add = 8 flops:
     P Q (two N x N x n GEMMs) 4 n^3; eigensolver 106 n^3 and its
     back-transform W = Z X 8 n^3; V = Q W (two GEMMs) 4 n^3; W^-1 and V^-1 V0
@@ -298,7 +298,7 @@ def plain_batch1(cfg, idx_all, sample):
 
 
 def extra_workload(name, args):
-    """Secondary measurements (not the driver's line): returns a JSON dict."""
+    """Secondary measurements (not the main line): returns a JSON dict."""
     import torch
     from khepri_amd import Crystal, Expansion, Layer, workloads
     from khepri_amd.batch import CrystalBatch
@@ -642,7 +642,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}): every solve is independent, "
-              "so N GPUs would each run the same per-GPU workload; multi-GPU runs are not part of this build.",
+              "so N GPUs would each run the same per-GPU workload; multi-GPU runs are not provided.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -723,7 +723,7 @@ def main():
     cf = counted_flops_per_solve(n)
     out["counted_flops"] = {"per_solve": cf, "tflops": round(value * cf / 1e12, 3),
                             "frac_fp64_peak": round(value * cf / 1e12 / PEAK_FP64_TFLOPS, 4),
-                            "note": "the build's own op sequence (cheaper than the reference's, whose count is "
+                            "note": "the library's own op sequence (cheaper than the reference's, whose count is "
                                     "2.578e9 per solve)"}
     if args.grad_steps > 0:
         out["grad_grid"] = grad_grid_rate(args.grad_steps)

@@ -5,8 +5,8 @@ solver's algorithm (Kaeryv/Khepri, ``khepri/`` package).  Every function
 cites the reference file:line it follows.
 
 Who may use it: ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` -- and only as the checker or the timed
-CPU baseline.  The product package ``khepri_amd`` never imports it; the GPU
+``cpu_baseline`` leg of ``bench.py`` -- and only as the reference results that
+the GPU path is compared with, or as the timed CPU baseline.  The product package ``khepri_amd`` never imports it; the GPU
 path has no CPU fallback.
 
 Pinning status of each function (what checks it, besides retyping):
