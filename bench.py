@@ -38,7 +38,7 @@ def eig_flops(n):
 
 
 def counted_flops_per_solve(n):
-    """Real flops the library's own op sequence spends per c2 solve (one unique
+    """Real flops the library's own op sequence spends per disc_stack_bz solve (one unique
     patterned layer, the R/T-only chain), complex multiply-add = 8 flops:
     P Q (two N x N x n GEMMs) 4 n^3; eigensolver 106 n^3 and its
     back-transform W = Z X 8 n^3; V = Q W (two GEMMs) 4 n^3; W^-1 and V^-1 V0
@@ -168,7 +168,7 @@ def grad_grid_rate(steps=2):
 def mixed_leg(nsolves, R64, T64, idx_all):
     """The mixed-precision path (KHX_LU_MIXED=1: FP32-MFMA products in the
     layer solves, one FP64 residual refinement step) on an evenly spread
-    subset of the c2 workload, timed over one step after a warm-up step, and
+    subset of the disc_stack_bz workload (65 536 solves), timed over one step after a warm-up step, and
     its max |dR|, |dT| against the FP64 line's own values for those solves."""
     import torch
     cl, _, idx = build_crystal(nsolves)
@@ -197,7 +197,7 @@ def mixed_leg(nsolves, R64, T64, idx_all):
 
 
 def graph_leg(cl, steps):
-    """The c2 step replayed from one captured HIP graph (khepri_amd.graph.SolveGraph:
+    """The disc_stack_bz step replayed from one captured HIP graph (khepri_amd.graph.SolveGraph:
     all chunks of the solve and the R/T reduction), timed over `steps` replays."""
     import torch
     from khepri_amd.graph import SolveGraph
@@ -450,7 +450,7 @@ def extra_workload(name, args):
                 oc.fields_volume(xg, yg, zs, incident_fields=Fo)
             return nsample / (time.perf_counter() - t0), f"{nsample} Bloch vectors of the same workload, the reference's loop"
     elif name == "islands":
-        # analytical island layer (disc + rectangle, closed-form coefficients) in the c2 stack, pw (9, 9),
+        # analytical island layer (disc + rectangle, closed-form coefficients) in the disc stack, pw (9, 9),
         # 64 frequencies x 8 x 8 Bloch vectors
         isl = [{"type": "disc", "params": [0.5, 0.5, 0.3], "epsilon": 1.0},
                {"type": "rectangle", "params": [0.05, 0.05, 0.25, 0.9], "epsilon": 4.0}]
@@ -482,7 +482,7 @@ def extra_workload(name, args):
                 oc.poynting_flux_end()
             return nsample / (time.perf_counter() - t0), f"{nsample} solves of the same workload"
     elif name == "bands":
-        # Bloch eigenpairs of 2 048 period S-matrices (the c2 disc period, pw (5, 5): 4ng = 100),
+        # Bloch eigenpairs of 2 048 period S-matrices (the disc-stack period, pw (5, 5): 4ng = 100),
         # 32 frequencies x 8 x 8 Bloch vectors, solved in one batch
         from khepri_amd import eigentricks as E
         cfg = workloads.disc_stack_bz()

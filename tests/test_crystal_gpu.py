@@ -53,7 +53,7 @@ def test_disc_sweep_reference_protocol(gpu):
 
 @pytest.mark.parametrize("mixed", ["0", "1"])
 def test_disc_bz_batched_with_nan_pattern(gpu, monkeypatch, mixed):
-    """The c2 golden set; mixed = "1" runs the layer solves through the
+    """The disc_stack_bz golden set; mixed = "1" runs the layer solves through the
     mixed-precision zgesv (FP32-MFMA triangular-solve products, one FP64
     residual refinement; KHX_LU_MIXED=1), held to the same 1e-9."""
     monkeypatch.setenv("KHX_LU_MIXED", mixed)

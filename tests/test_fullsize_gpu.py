@@ -1,17 +1,18 @@
-"""Full-size runs of the c2, c4 and c5 workloads, checked through properties
+"""Full-size runs of the disc_stack_bz, random-device, grad_grid and twisted
+workloads, checked through properties
 that hold at any size (the oracle is too slow at these sizes; the golden
 subsets are compared elsewhere).
 
-* c2 (65 536 solves, pw (9, 9)): the disc stack is lossless (eps 12 and 1),
+* disc_stack_bz (65 536 solves, pw (9, 9)): the disc stack is lossless (eps 12 and 1),
   so R + T = 1 within 1e-9 wherever the incident wave propagates, R and T lie
   in [0, 1], and R, T are non-finite exactly where |k_par| > k0 (evanescent
   incidence, khepri/alternative.py:241).
-* c5 (8 angles x 32 frequencies, 625 extended harmonics): lossless, normal
+* twisted bilayer (8 angles x 32 frequencies, 625 extended harmonics): lossless, normal
   incidence: R + T = 1 within 1e-9.
-* c3 (4 096 random devices, pw (5..13)^2, 1-8 layers): one CrystalBatch over
+* random devices (4 096 devices, pw (5..13)^2, 1-8 layers): one CrystalBatch over
   the whole workload; the devices with golden R/T (both golden files) are
   checked at their positions in the batch, the rest are finite.
-* c4 (pw (11, 11), 4 x 64 x 64 real pixels, 16 frequencies): R + T = 1 for
+* grad_grid (pw (11, 11), 4 x 64 x 64 real pixels, 16 frequencies): R + T = 1 for
   every real permittivity, so the adjoint gradient of R + T with respect to
   every pixel vanishes: |d(R + T)/d eps| <= 1e-7 max |dT/d eps|.
 """
@@ -23,7 +24,7 @@ from khepri_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def test_c2_full_energy_and_evanescent_pattern(gpu):
+def test_disc_stack_bz_full_energy_and_evanescent_pattern(gpu):
     from khepri_amd import Crystal
     cfg = W.disc_stack_bz()
     cl = Crystal(cfg["pw"])
@@ -43,7 +44,7 @@ def test_c2_full_energy_and_evanescent_pattern(gpu):
     assert R[fin].min() >= -1e-12 and T[fin].min() >= -1e-12
 
 
-def test_c3_full_batch_golden_positions(gpu):
+def test_random_devices_full_batch_golden_positions(gpu):
     from conftest import golden
     from khepri_amd import Crystal
     from khepri_amd.batch import CrystalBatch
@@ -66,7 +67,7 @@ def test_c3_full_batch_golden_positions(gpu):
         assert np.abs(R[idx] - g["R"]).max() <= 1e-9 and np.abs(T[idx] - g["T"]).max() <= 1e-9, name
 
 
-def test_c5_full_energy(gpu):
+def test_twisted_full_energy(gpu):
     from khepri_amd import Crystal, Expansion, Layer
     cfg = W.twisted_bilayer()
     for ang in cfg["angles_deg"]:
@@ -85,7 +86,7 @@ def test_c5_full_energy(gpu):
         assert np.abs(R + T - 1).max() <= 1e-9, ang
 
 
-def test_c4_full_gradient_of_total_flux_vanishes(gpu):
+def test_grad_grid_full_gradient_of_total_flux_vanishes(gpu):
     from khepri_amd import Crystal
     from khepri_amd.adjoint import flux_gradient
     cfg = W.grad_grid()
