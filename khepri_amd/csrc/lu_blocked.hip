@@ -566,7 +566,7 @@ __device__ __forceinline__ void panel_steps(std::integer_sequence<int, J...>, cd
 // output are those of zlu_panel_kernel: the largest |Re| + |Im| below the
 // diagonal, the first such row on ties, NaN columns keep the diagonal.
 template <int NB, int RPT, int TH = LB_THREADS>
-__global__ __launch_bounds__(TH, TH == 256 ? (RPT == 1 ? 4 : 2) : 1) void zlu_panel_reg_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
+__global__ __launch_bounds__(TH, TH == 256 ? ((RPT == 1 && NB == 16) ? 4 : 2) : 1) void zlu_panel_reg_kernel(int n, int m, int j0, cd* A, int lda, long long sA,
                                                                    cd* B, int ldb, long long sB, int* ipiv, int* info,
                                                                    int keepL, int cend) {
   __shared__ double rv[TH / 64];
@@ -799,6 +799,15 @@ static bool panel_reg_enabled() {
 template <int NB>
 static int launch_panel(hipStream_t s, int n, int m, int j0, Mat A, Mat B, int batch, int* ipiv, int* info, int mode,
                         int cend, bool gp, size_t lds) {
+  if constexpr (NB == 32) {  // one row per thread, up to 256 rows
+    const int pr = n - j0;
+    if (pr <= LB_THREADS && panel_reg_enabled()) {
+      zlu_panel_reg_kernel<NB, 1><<<dim3(batch), dim3(LB_THREADS), NB * (NB + 1) * sizeof(cd), s>>>(
+          n, m, j0, A.p, A.ld, A.stride, B.p, B.ld, B.stride, ipiv, info, mode, cend);
+      KHX_LAUNCH_CHECK();
+      return 0;
+    }
+  }
   if constexpr (NB == 16) {
     const int pr = n - j0;
     const size_t l11 = NB * (NB + 1) * sizeof(cd);
