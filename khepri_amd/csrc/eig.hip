@@ -1021,6 +1021,279 @@ __device__ __forceinline__ void qr_sweep_reg(cd* H, int n, int l, int ihi, cd sh
   }
 }
 
+// ------------------------------------------------------------ paired sweeps
+// Two matrices per workgroup, one chase wavefront for both: every lane of the
+// register chase evaluates the whole rotation and 2 x 2 core, so a wavefront
+// whose lanes 0-31 chase one matrix's block and lanes 32-63 the other's
+// issues one instruction stream for two chases.  Blocks are EIG_PQB wide (a
+// half-wavefront's lanes); the two matrices keep their own windows, shifts,
+// deflations and rotation logs.
+constexpr int EIG_PQB = 31;  // chase block of the paired sweeps (four 256-thread workgroups per CU at n = 162)
+__host__ __device__ __forceinline__ int eig_pair_elems(int n) {  // LDS per matrix: srot, crot, block
+  return n + (n + 1) / 2 + EIG_PQB * (EIG_PQB + 1);
+}
+static_assert(3 * EIG_KLOG * (EIG_CH + 2 * (EIG_KLOG - 1)) <= 2 * EIG_PQB * (EIG_PQB + 1),
+              "rotation staging does not fit a paired chase block");
+
+// reg_chase_block for the half-wavefront of lane `lane` (on: this half has a
+// block to chase).  The steps run on a counter t shared by the whole
+// wavefront; a half with fewer rotations idles for the rest.  The entry
+// (k+1, k+2) comes from the half's lane q+2 by readlane at an index uniform
+// per half.  Per element, the operations and their order are those of
+// reg_chase_block.
+__device__ __forceinline__ void reg_chase_pair(bool on, cd* hb, int nb, int b0, int l, int kstart, int kend, cd shift,
+                                               double* crot, cd* srot, int lane, int ld) {
+  const int j = lane & 31, half = lane & 32;
+  const int len = on ? kend - kstart + 1 : 0;
+  const int q0 = on ? kstart - b0 : 0;
+  const int tmax = max(__builtin_amdgcn_readlane(len, 0), __builtin_amdgcn_readlane(len, 32));
+  const int qa = __builtin_amdgcn_readlane(q0, 0), qb = __builtin_amdgcn_readlane(q0, 32);
+  int q = q0;
+  cd x = mkc(0.0, 0.0), y = x, a00 = x, a01 = x, a10 = x, a11 = x, v = x, yj = x, wq = x, dq = x;
+  if (len > 0) {
+    if (kstart == l) {
+      x = hb[0] - shift;
+      y = hb[ld];
+    } else {
+      x = hb[q * ld + q - 1];
+      y = hb[(q + 1) * ld + q - 1];
+    }
+    a00 = hb[q * ld + q];
+    a01 = hb[q * ld + q + 1];
+    a10 = hb[(q + 1) * ld + q];
+    a11 = hb[(q + 1) * ld + q + 1];
+    const bool mine0 = (j >= q + 2 && j < nb);
+    if (mine0) {
+      v = hb[q * ld + j];
+      yj = hb[(q + 1) * ld + j];
+    }
+    if (q + 2 < nb) {
+      wq = hb[(q + 2) * ld + q + 1];
+      dq = hb[(q + 2) * ld + q + 2];
+    }
+  }
+  for (int t = 0; t < tmax; ++t, ++q) {
+    // lanes of entry (k+1, k+2) for the next step, uniform per half (clamped
+    // into the half: a half past its last step never uses its value)
+    const int sa = min(qa + t + 2, 31), sb = 32 + min(qb + t + 2, 31);
+    if (t < len) {
+      const int k = kstart + t;
+      double c;
+      cd s, r;
+      givens(x, y, c, s, r);
+      const bool mine = (j >= q + 2 && j < nb);
+      const bool more = (t + 1 < len);
+      cd yj_n = mkc(0.0, 0.0), wq_n = mkc(0.0, 0.0), dq_n = mkc(0.0, 0.0);
+      if (more) {
+        if (j >= q + 3 && j < nb) yj_n = hb[(q + 2) * ld + j];
+        if (q + 3 < nb) {
+          wq_n = hb[(q + 3) * ld + q + 2];
+          dq_n = hb[(q + 3) * ld + q + 3];
+        }
+      }
+      if (mine) {
+        hb[q * ld + j] = fmac(c * v, s, yj);
+        v = fmsc(c * yj, conjc(s), v);
+      }
+      rot_rows(a00, a10, c, s);
+      rot_rows(a01, a11, c, s);
+      rot_cols(a00, a01, c, s);
+      rot_cols(a10, a11, c, s);
+      const cd bulge = conjc(s) * wq;
+      const cd w2 = c * wq;
+      if (j == 0) {
+        crot[k] = c;
+        srot[k] = s;
+        if (k > l) {
+          hb[q * ld + q - 1] = r;
+          hb[(q + 1) * ld + q - 1] = mkc(0.0, 0.0);
+        }
+        hb[q * ld + q] = a00;
+        hb[q * ld + q + 1] = a01;
+      }
+      if (!more) {
+        if (j == 0) {
+          hb[(q + 1) * ld + q] = a10;
+          hb[(q + 1) * ld + q + 1] = a11;
+          if (q + 2 < nb) {
+            hb[(q + 2) * ld + q] = bulge;
+            hb[(q + 2) * ld + q + 1] = w2;
+          }
+        }
+        if (mine) hb[(q + 1) * ld + j] = v;
+      } else {
+        x = a10;
+        y = bulge;
+        a00 = a11;
+        const double ax = readlane_d(v.x, sa), ay = readlane_d(v.y, sa);
+        const double bx = readlane_d(v.x, sb), by = readlane_d(v.y, sb);
+        a01 = half ? mkc(bx, by) : mkc(ax, ay);
+        a10 = w2;
+        a11 = dq;
+        yj = yj_n;
+        wq = wq_n;
+        dq = dq_n;
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // column rotations of the block rows above the core (as in reg_chase_block)
+  const int k0 = max(kstart, b0 + j + 1);
+  if (len > 0 && j < nb && k0 <= kend) {
+    cd* row = hb + j * ld - b0;
+    cd xv = row[k0];
+    int k = k0;
+    for (; k + 4 <= kend + 1; k += 4) {
+      cd yv[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) yv[m] = row[k + 1 + m];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        rot_cols(xv, yv[m], crot[k + m], srot[k + m]);
+        row[k + m] = xv;
+        xv = yv[m];
+      }
+    }
+    for (; k <= kend; ++k) {
+      cd yv = row[k + 1];
+      rot_cols(xv, yv, crot[k], srot[k]);
+      row[k] = xv;
+      xv = yv;
+    }
+    row[kend + 1] = xv;
+  }
+}
+
+// One matrix of a paired sweep.
+struct PairSweep {
+  cd* H;
+  cd* hb;
+  double* crot;
+  cd* srot;
+  cd shift;
+  int l, ihi;
+};
+
+// qr_sweep_reg for two matrices (on[m]: matrix m takes a sweep), blocks of
+// EIG_PQB rows.  Per matrix the phases and their order are those of
+// qr_sweep_reg: store the block; rows above the last block, or the next
+// block's new columns; load the next block; chase it while the other
+// wavefronts apply the finished block's rotations to the rest of the window.
+// The line work of both matrices shares one index space.
+template <int EIG_THREADS>
+__device__ __forceinline__ void qr_sweep_pair(const PairSweep (&S)[2], const bool (&on)[2], int n,
+                                              unsigned long long* acc) {
+  constexpr int RW = EIG_THREADS - 64;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int h = lane >> 5;  // the chase wavefront's half
+  int kstart[2], kend[2], b0[2], b1[2];
+  bool run[2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    run[m] = on[m];
+    kstart[m] = S[m].l;
+    b0[m] = S[m].l;
+    b1[m] = min(b0[m] + EIG_PQB - 1, S[m].ihi);
+    kend[m] = (b1[m] == S[m].ihi) ? S[m].ihi - 1 : b1[m] - 2;
+    if (run[m]) {
+      const int nb = b1[m] - b0[m] + 1, ld = nb + 1;
+      for (int e = tid; e < nb * nb; e += EIG_THREADS)
+        if (e % nb >= e / nb - 2) S[m].hb[(e / nb) * ld + e % nb] = S[m].H[(long long)(b0[m] + e / nb) * n + b0[m] + e % nb];
+    }
+  }
+  __syncthreads();
+  unsigned long long t_c0 = acc ? wall_clock64() : 0;
+  if (tid < 64) {
+    const PairSweep& P = h ? S[1] : S[0];
+    const int pb0 = h ? b0[1] : b0[0], pnb = (h ? b1[1] : b1[0]) - pb0 + 1;
+    reg_chase_pair(h ? run[1] : run[0], P.hb, pnb, pb0, P.l, h ? kstart[1] : kstart[0], h ? kend[1] : kend[0],
+                   P.shift, P.crot, P.srot, lane, pnb + 1);
+  }
+  __syncthreads();
+  if (acc) acc[1] += wall_clock64() - t_c0;
+  while (run[0] || run[1]) {
+    bool last[2];
+    int b0n[2], b1n[2], ksn[2], ken[2], cnt[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      last[m] = kend[m] == S[m].ihi - 1;
+      b0n[m] = kend[m];
+      b1n[m] = min(b0n[m] + EIG_PQB - 1, S[m].ihi);
+      ksn[m] = kend[m] + 1;
+      ken[m] = (b1n[m] == S[m].ihi) ? S[m].ihi - 1 : b1n[m] - 2;
+      cnt[m] = !run[m] ? 0 : last[m] ? b0[m] - S[m].l : b1n[m] - b1[m];
+      if (run[m]) {
+        const int nb = b1[m] - b0[m] + 1, ld = nb + 1;
+        for (int e = tid; e < nb * nb; e += EIG_THREADS)
+          if (e % nb >= e / nb - 2) S[m].H[(long long)(b0[m] + e / nb) * n + b0[m] + e % nb] = S[m].hb[(e / nb) * ld + e % nb];
+      }
+    }
+    // the last block's rows above (column rotations) or the next block's new
+    // columns b1+1..b1n (row rotations)
+    for (int e = tid; e < cnt[0] + cnt[1]; e += EIG_THREADS) {
+      const int m = e >= cnt[0];
+      const int i = m ? e - cnt[0] : e;
+      const PairSweep& P = m ? S[1] : S[0];
+      const int ks = m ? kstart[1] : kstart[0], ke = m ? kend[1] : kend[0];
+      if (m ? last[1] : last[0])
+        slide<false, EIG_SLIDE_PF>(P.H + (long long)(P.l + i) * n, 1, ks, ke + 1, P.crot, P.srot);
+      else
+        slide<true, EIG_SLIDE_PF>(P.H + (m ? b1[1] : b1[0]) + 1 + i, n, ks, ke + 1, P.crot, P.srot);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      if (run[m] && last[m]) run[m] = false;
+      if (run[m]) {
+        const int nb = b1n[m] - b0n[m] + 1, ld = nb + 1;
+        for (int e = tid; e < nb * nb; e += EIG_THREADS)
+          if (e % nb >= e / nb - 2) S[m].hb[(e / nb) * ld + e % nb] = S[m].H[(long long)(b0n[m] + e / nb) * n + b0n[m] + e % nb];
+      }
+    }
+    if (!(run[0] || run[1])) break;
+    __syncthreads();
+    t_c0 = acc ? wall_clock64() : 0;
+    if (tid < 64) {
+      const PairSweep& P = h ? S[1] : S[0];
+      const int pb0 = h ? b0n[1] : b0n[0], pnb = (h ? b1n[1] : b1n[0]) - pb0 + 1;
+      reg_chase_pair(h ? run[1] : run[0], P.hb, pnb, pb0, P.l, h ? ksn[1] : ksn[0], h ? ken[1] : ken[0], P.shift,
+                     P.crot, P.srot, lane, pnb + 1);
+    } else {
+      // rest of the finished blocks: rows l..b0-1 (column rotations) and
+      // columns b1n+1..ihi (row rotations)
+      int na[2], tot[2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        na[m] = b0[m] - S[m].l;
+        tot[m] = run[m] ? na[m] + S[m].ihi - b1n[m] : 0;
+      }
+      for (int e = tid - 64; e < tot[0] + tot[1]; e += RW) {
+        const int m = e >= tot[0];
+        const int i = m ? e - tot[0] : e;
+        const PairSweep& P = m ? S[1] : S[0];
+        const int ks = m ? kstart[1] : kstart[0], ke = m ? kend[1] : kend[0];
+        const int nam = m ? na[1] : na[0];
+        if (i < nam)
+          slide<false, EIG_SLIDE_PF>(P.H + (long long)(P.l + i) * n, 1, ks, ke + 1, P.crot, P.srot);
+        else
+          slide<true, EIG_SLIDE_PF>(P.H + (m ? b1n[1] : b1n[0]) + 1 + (i - nam), n, ks, ke + 1, P.crot, P.srot);
+      }
+    }
+    __syncthreads();
+    if (acc) acc[1] += wall_clock64() - t_c0;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      if (run[m]) {
+        kstart[m] = ksn[m];
+        kend[m] = ken[m];
+        b0[m] = b0n[m];
+        b1[m] = b1n[m];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------ multishift QR
 // A chain pass applies MS_NS shifts at once: MS_NS bulges two positions apart
 // chased together (the small-bulge multishift QR of Braman, Byers and
@@ -1819,6 +2092,169 @@ __device__ __forceinline__ void eig_body(int n, cd* A, long long sA, cd* Ztg, cd
   }  // stage 3
 }
 
+// The Schur stage (single-shift sweeps, deferred rotation logs) of
+// matrices 2 blockIdx.x and 2 blockIdx.x + 1 in one workgroup: both run the
+// steps of eig_body's stage 2 -- deflation scan, shift, sweep, log, deferred
+// pass -- with their sweeps taken together by qr_sweep_pair.  A matrix that
+// has converged (or an absent second matrix of an odd batch) leaves the
+// other to sweep alone.  Results are those of eig_body, bit for bit.
+template <int EIG_THREADS>
+__device__ __forceinline__ void eig_qr_pair(int n, cd* A, long long sA, cd* Ztg, cd* Xg, long long sW, cd* lam,
+                                            long long sL, int* info, int batch, unsigned long long* tphase,
+                                            int flags) {
+  constexpr int EIG_WAVES = EIG_THREADS / 64, HT = EIG_THREADS / 2;
+  extern __shared__ cd sm[];
+  __shared__ int s_lo[2][EIG_KLOG], s_hi[2][EIG_KLOG];
+  __shared__ int redi[EIG_WAVES];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const double smlnum = SAFMIN * ((double)n / ULP);
+  const int itmax = (flags >> 8) > 0 ? (flags >> 8) : 30 * max(10, n);
+  const int per = eig_pair_elems(n);
+  PairSweep S[2];
+  cd* Zt[2];
+  cd* log_s[2];
+  double* log_c[2];
+  long long item[2];
+  int ihi[2], its[2], itot[2], fail[2], nlog[2];
+  bool live[2];
+  unsigned long long acc_t[5] = {0, 0, 0, 0, 0};  // scan, chase, block rest, log, flush (the pair's wall time)
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    item[m] = 2ll * blockIdx.x + m;
+    live[m] = item[m] < batch;
+    const long long it = live[m] ? item[m] : 2ll * blockIdx.x;
+    S[m].H = A + it * sA;
+    Zt[m] = Ztg + it * sW;
+    log_s[m] = Xg + it * sW;
+    log_c[m] = reinterpret_cast<double*>(log_s[m] + (long long)EIG_KLOG * n);
+    S[m].srot = sm + m * per;
+    S[m].crot = reinterpret_cast<double*>(S[m].srot + n);
+    S[m].hb = S[m].srot + n + (n + 1) / 2;
+    S[m].shift = mkc(0.0, 0.0);
+    S[m].l = 0;
+    S[m].ihi = n - 1;
+    ihi[m] = n - 1;
+    its[m] = itot[m] = fail[m] = nlog[m] = 0;
+  }
+  while (live[0] || live[1]) {
+    bool srch[2] = {live[0], live[1]}, sw[2] = {false, false};
+    int l[2] = {0, 0};
+    const unsigned long long t_s0 = tphase ? wall_clock64() : 0;
+    while (srch[0] || srch[1]) {
+      // deflation scans: the first half of the workgroup scans matrix 0, the second matrix 1
+      const int m = tid >= HT;
+      const int hi = m ? ihi[1] : ihi[0];
+      cd* Hm = m ? S[1].H : S[0].H;
+      int lk = 0;
+      if (m ? srch[1] : srch[0]) {
+        for (int k = 1 + tid - m * HT; k <= hi; k += HT) {
+          if (negligible(Hm, n, k, smlnum)) {
+            Hm[k * n + k - 1] = mkc(0.0, 0.0);
+            lk = max(lk, k);
+          }
+        }
+      }
+      lk = wave_max_int(lk);
+      if (lane == 0) redi[wid] = lk;
+      __syncthreads();
+      int lm[2] = {redi[0], redi[EIG_WAVES / 2]};
+#pragma unroll
+      for (int w = 1; w < EIG_WAVES / 2; ++w) {
+        lm[0] = max(lm[0], redi[w]);
+        lm[1] = max(lm[1], redi[EIG_WAVES / 2 + w]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (!srch[q]) continue;
+        l[q] = __builtin_amdgcn_readfirstlane(lm[q]);
+        if (l[q] == ihi[q]) {
+          --ihi[q];
+          its[q] = 0;
+          if (ihi[q] < 1) srch[q] = live[q] = false;
+          continue;
+        }
+        if (++itot[q] > itmax) {
+          fail[q] = ihi[q] + 1;
+          srch[q] = live[q] = false;
+          continue;
+        }
+        ++its[q];
+        const cd* H = S[q].H;
+        const int lq = l[q], hq = ihi[q];
+        if (its[q] % 20 == 10) {
+          S[q].shift = H[lq * n + lq] + mkc(0.75 * fabs(H[(lq + 1) * n + lq].x), 0.0);
+        } else if (its[q] % 20 == 0) {
+          S[q].shift = H[hq * n + hq] + mkc(0.75 * fabs(H[hq * n + hq - 1].x), 0.0);
+        } else {
+          S[q].shift = wilkinson_shift(H[(hq - 1) * n + hq - 1], H[(hq - 1) * n + hq], H[hq * n + hq - 1],
+                                       H[hq * n + hq]);
+        }
+        S[q].l = lq;
+        S[q].ihi = hq;
+        sw[q] = true;
+        srch[q] = false;
+      }
+    }
+    unsigned long long t1 = tphase ? wall_clock64() : 0;
+    if (tphase) acc_t[0] += t1 - t_s0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (!sw[q] && item[q] < batch && !live[q] && nlog[q] >= 0) {  // converged or failed just now
+        if (nlog[q] > 0) flush_log<EIG_THREADS>(S[q].H, Zt[q], n, nlog[q], s_lo[q], s_hi[q], log_c[q], log_s[q], S[q].hb);
+        nlog[q] = -1;
+        if (tid == 0 && info) info[item[q]] = fail[q];
+        if (tphase && tid == 0) {
+          unsigned long long* tp = tphase + item[q] * 10;
+          tp[2] = wall_clock64();
+          tp[4] = (unsigned long long)itot[q];
+          tp[5] = acc_t[0];
+          tp[6] = acc_t[1];
+          tp[7] = acc_t[2] - acc_t[1];
+          tp[8] = acc_t[3];
+          tp[9] = acc_t[4];
+        }
+        cd* lmq = lam + item[q] * sL;
+        for (int i = tid; i < n; i += EIG_THREADS) lmq[i] = S[q].H[i * n + i];
+      }
+      // rows l.. of H must be current before the window reaches them
+      if (sw[q] && nlog[q] > 0 && l[q] < s_lo[q][nlog[q] - 1]) {
+        flush_log<EIG_THREADS>(S[q].H, Zt[q], n, nlog[q], s_lo[q], s_hi[q], log_c[q], log_s[q], S[q].hb);
+        nlog[q] = 0;
+      }
+    }
+    if (!(sw[0] || sw[1])) continue;
+    unsigned long long t2 = tphase ? wall_clock64() : 0;
+    if (tphase) acc_t[4] += t2 - t1;
+    qr_sweep_pair<EIG_THREADS>(S, sw, n, tphase ? acc_t : nullptr);
+    unsigned long long t3 = tphase ? wall_clock64() : 0;
+    if (tphase) acc_t[2] += t3 - t2;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (!sw[q]) continue;
+      for (int k = l[q] + tid; k < ihi[q]; k += EIG_THREADS) {
+        log_c[q][nlog[q] * n + k] = S[q].crot[k];
+        log_s[q][nlog[q] * n + k] = S[q].srot[k];
+      }
+      if (tid == 0) {
+        s_lo[q][nlog[q]] = l[q];
+        s_hi[q][nlog[q]] = ihi[q];
+      }
+    }
+    __syncthreads();
+    unsigned long long t4 = tphase ? wall_clock64() : 0;
+    if (tphase) acc_t[3] += t4 - t3;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (sw[q] && ++nlog[q] == EIG_KLOG) {
+        flush_log<EIG_THREADS>(S[q].H, Zt[q], n, nlog[q], s_lo[q], s_hi[q], log_c[q], log_s[q], S[q].hb);
+        nlog[q] = 0;
+      }
+    }
+    if (tphase) acc_t[4] += wall_clock64() - t4;
+  }
+}
+
 #define KHX_EIG_BOUNDS(T) __launch_bounds__(T, T >= 256 ? 1024 / T : 1)
 #define KHX_EIG_ARGS                                                                                   \
   int n, cd *A, long long sA, cd *Ztg, cd *Xg, long long sW, cd *lam, long long sL, int *info,        \
@@ -1849,6 +2285,13 @@ __global__ __launch_bounds__(256, 4) __attribute__((amdgpu_num_vgpr(128))) void 
 }
 __global__ __launch_bounds__(128, 4) __attribute__((amdgpu_num_vgpr(128))) void zeig_qr128_kernel(KHX_EIG_ARGS) {
   eig_body<128, 2, false>(n, A, sA, Ztg, Xg, sW, lam, sL, info, tphase, flags);
+}
+// two matrices per workgroup (eig_qr_pair): four waves per SIMD, i.e. at
+// most 128 registers
+template <int EIG_THREADS>
+__global__ __launch_bounds__(EIG_THREADS, 4) __attribute__((amdgpu_num_vgpr(128))) void zeig_qr_pair_kernel(
+    KHX_EIG_ARGS, int batch) {
+  eig_qr_pair<EIG_THREADS>(n, A, sA, Ztg, Xg, sW, lam, sL, info, batch, tphase, flags);
 }
 template <int EIG_THREADS>
 __global__ KHX_EIG_BOUNDS(EIG_THREADS) void zeig_vec_kernel(KHX_EIG_ARGS) {
@@ -1913,6 +2356,23 @@ static const void* eig_qr_fn() {
                   : (const void*)zeig_qr_kernel<256, 2, true>;
 }
 
+// Paired Schur stage (zeig_qr_pair_kernel) for the default single-shift
+// sweeps of a 256-thread launch, while two workgroups fit a CU's LDS:
+// KHX_EIG_PAIR=1 (256 threads per pair) or 2 (512).  Returns the pair
+// kernel's thread count, 0 for the one-matrix kernel.
+static size_t eig_pair_lds_bytes(int n) { return 2 * (size_t)eig_pair_elems(n) * sizeof(cd); }
+static int eig_pair_threads(int n, int T, int tq, int flags, size_t pad) {
+  const char* env = getenv("KHX_EIG_PAIR");
+  const int v = env ? atoi(env) : 0;
+  if (v != 1 && v != 2) return 0;
+  const bool ok = n > KHX_EIG_LDS_N && eig_pair_lds_bytes(n) <= 80 * 1024 && T == 256 && tq == 256 &&
+                  !eig_chain_on() && (flags & 4) && pad == 0;
+  return ok ? (v == 2 ? 512 : 256) : 0;
+}
+static const void* eig_pair_fn(int threads) {
+  return threads == 512 ? (const void*)zeig_qr_pair_kernel<512> : (const void*)zeig_qr_pair_kernel<256>;
+}
+
 // Threads per matrix in the Schur stage: those of the other stages, or 128
 // with KHX_EIG_QR_THREADS=128 (single-shift sweeps only).
 static int eig_qr_threads(int T) {
@@ -1941,10 +2401,18 @@ static int launch_eig(hipStream_t s, int n, int batch, size_t lds, Mat A, cd* Zt
   KHX_TRY(ensure_dynamic_lds((const void*)zeig_hess_kernel<T>, lds));
   KHX_TRY(ensure_dynamic_lds(qr, ldsq));
   KHX_TRY(ensure_dynamic_lds((const void*)zeig_vec_kernel<T>, lds));
+  const int pt = eig_pair_threads(n, T, tq, flags, pad);
+  if (pt) KHX_TRY(ensure_dynamic_lds(eig_pair_fn(pt), eig_pair_lds_bytes(n)));
   zeig_hess_kernel<T><<<dim3(batch), dim3(T), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase, flags);
   KHX_LAUNCH_CHECK();
-  void* args[] = {&n, &A.p, &A.stride, &Zt, &X, &sZ, &lam, &lstride, &info, &tphase, &flags};
-  KHX_CHECK_HIP(hipLaunchKernel(qr, dim3(batch), dim3(tq), args, ldsq, s));
+  if (pt) {
+    size_t ldsp = eig_pair_lds_bytes(n);
+    void* args[] = {&n, &A.p, &A.stride, &Zt, &X, &sZ, &lam, &lstride, &info, &tphase, &flags, &batch};
+    KHX_CHECK_HIP(hipLaunchKernel(eig_pair_fn(pt), dim3((batch + 1) / 2), dim3(pt), args, ldsp, s));
+  } else {
+    void* args[] = {&n, &A.p, &A.stride, &Zt, &X, &sZ, &lam, &lstride, &info, &tphase, &flags};
+    KHX_CHECK_HIP(hipLaunchKernel(qr, dim3(batch), dim3(tq), args, ldsq, s));
+  }
   zeig_vec_kernel<T><<<dim3(batch), dim3(T), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase, flags);
   KHX_LAUNCH_CHECK();
   return 0;
@@ -1971,6 +2439,22 @@ int zeig_slots(int n) {
     KHX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     KHX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 128, l));
     return per_cu * cus > 0 ? per_cu * cus : 1;
+  }
+  {
+    const char* rc_env = getenv("KHX_EIG_REG_CHASE");
+    const int flags = (rc_env && atoi(rc_env) == 0) ? 0 : 4;
+    const size_t pad = pad_env ? (size_t)atol(pad_env) : 0;
+    const int pt = small ? 0 : eig_pair_threads(n, threads, eig_qr_threads(threads), flags, pad);
+    if (pt) {
+      const void* f = eig_pair_fn(pt);
+      const size_t l = eig_pair_lds_bytes(n);
+      KHX_TRY(ensure_dynamic_lds(f, l));
+      int per_cu = 0, dev = 0, cus = 0;
+      KHX_CHECK_HIP(hipGetDevice(&dev));
+      KHX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      KHX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, pt, l));
+      return 2 * per_cu * cus > 0 ? 2 * per_cu * cus : 1;
+    }
   }
   const void* fn = threads == 1024 ? (small ? (const void*)zeig_schur_kernel<1024> : eig_qr_fn<1024>())
                    : threads == 512 ? (small ? (const void*)zeig_schur_kernel<512> : eig_qr_fn<512>())

@@ -440,6 +440,52 @@ def test_eig_vectors_per_column_match_row_sweep(gpu, monkeypatch, pw):
         assert np.abs(M[b] @ Wm[b] - Wm[b] * lam[b]).max() / np.abs(M[b]).max() < 1e-11
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("pw", [(7, 7), (9, 9)])
+def test_eig_paired_schur_matches_single(gpu, monkeypatch, pw, mode):
+    """The Schur stage with two matrices per workgroup (one chase wavefront
+    for both, blocks of 31; KHX_EIG_PAIR=1: 256 threads, 2: 512) against one
+    matrix per workgroup (KHX_EIG_PAIR=0, blocks of 40) on an odd batch larger than the CU count
+    (the paired launch's shape; the last workgroup holds one matrix): layer
+    matrices and random ones in a shuffled order, so partners deflate at
+    different sweeps.  Same rotations in the same order per matrix, so
+    eigenvalues, eigenvectors and status words agree bit for bit; so do the
+    status words of a run stopped by a low iteration cap."""
+    from khepri_amd import ops
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    pix = W.disc_pixmap()
+    C = rcwa.convolution_matrix(pix, pw)
+    g = rcwa.g_vectors(pw, np.eye(2))
+    rng = np.random.default_rng(sum(pw))
+    mats = []
+    for i in range(12):
+        kx, ky, _ = rcwa.k_vectors(g, tuple(rng.uniform(-0.4, 0.4, 2)), rng.uniform(0.9, 2.2))
+        P, Q = rcwa.structured_pq(kx, ky, C)
+        mats.append(P @ Q)
+    n = mats[0].shape[0]
+    mats += [crand(rng, n, n) for _ in range(4)]
+    batch = cus + 37
+    M = np.stack(mats)[rng.integers(0, len(mats), batch)]
+    out = {}
+    for pair in ("0", mode):
+        monkeypatch.setenv("KHX_EIG_PAIR", pair)
+        lam, Wm, info = ops.eig(dev(M, gpu))
+        out[pair] = (lam.cpu().numpy(), Wm.cpu().numpy(), info.cpu().numpy())
+    assert (out[mode][2] == 0).all()
+    for a, b in zip(out["0"], out[mode]):
+        np.testing.assert_array_equal(a, b)
+    lam, Wm, _ = out[mode]
+    for b in list(range(6)) + [batch - 1]:
+        assert np.abs(M[b] @ Wm[b] - Wm[b] * lam[b]).max() / np.abs(M[b]).max() < 1e-11
+    monkeypatch.setenv("KHX_EIG_ITMAX", str(n // 2))
+    infos = {}
+    for pair in ("0", mode):
+        monkeypatch.setenv("KHX_EIG_PAIR", pair)
+        infos[pair] = ops.eig(dev(M, gpu))[2].cpu().numpy()
+    assert (infos[mode] > 0).all()
+    np.testing.assert_array_equal(infos["0"], infos[mode])
+
+
 @pytest.mark.parametrize("n,m", [(162, 64), (162, 324), (250, 100)])
 def test_lu_mixed_refined(gpu, monkeypatch, n, m):
     """KHX_LU_MIXED=1: FP64 LU, triangular-solve products on the FP32 matrix
