@@ -43,14 +43,14 @@ def rotation_matrix(polar_angle, azimuthal_angle, polarization_angle):
 
 
 def gen_bzi_grid(shape, a=1, reciproc=None):
-    """Cell-centred Bloch vectors, (2, *shape).  khepri/beams.py:193-207."""
-    if reciproc is None:
-        b1, b2 = [2 * np.pi / a, 0], [0, 2 * np.pi / a]
-    else:
-        b1, b2 = reciproc
-    si, sj = 1 / shape[0], 1 / shape[1]
-    i, j = np.meshgrid(np.arange(-0.5 + si / 2, 0.5, si), np.arange(-0.5 + sj / 2, 0.5, sj), indexing="ij")
-    return np.stack([b1[0] * i + b2[0] * j, b1[1] * i + b2[1] * j])
+    """Bloch vectors at the centres of a shape[0] x shape[1] grid of cells
+    over the Brillouin zone, shape (2, *shape) (khepri/beams.py:193-207):
+    fractional coordinates -1/2 + (i + 1/2)/N along each reciprocal vector
+    (b1, b2; by default those of a square lattice of pitch a)."""
+    basis = reciproc if reciproc is not None else ((2 * np.pi / a, 0), (0, 2 * np.pi / a))
+    fractions = [np.arange(-0.5 + 0.5 / N, 0.5, 1 / N) for N in shape]
+    f1, f2 = np.meshgrid(*fractions, indexing="ij")
+    return np.stack([basis[0][c] * f1 + basis[1][c] * f2 for c in range(2)])
 
 
 def _device():

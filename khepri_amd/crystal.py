@@ -11,7 +11,6 @@ from __future__ import annotations
 import cmath
 import contextlib
 import os
-from copy import copy
 from types import SimpleNamespace
 
 import numpy as np
@@ -29,51 +28,64 @@ def _bytes_per_item(n, unique, length):
     return 16 * n * n * (4 * unique + 22 + 8 + 4 * (length + 2) + 4)
 
 
+# Named lattices: rows are the lattice vectors in units of the pitch
+# (khepri/crystal.py:60-72 accepts the same two names).
+_NAMED_LATTICES = {
+    "square": ((1.0, 0.0), (0.0, 1.0)),
+    "hexagonal": ((np.sqrt(3) / 2, 0.5), (np.sqrt(3) / 2, -0.5)),
+}
+
+# The half spaces framing every stack: layer name, side, medium attribute.
+_HALF_SPACES = (("Sref", "reflexion", "epsi"), ("Strans", "transmission", "epse"))
+
+
+def _lattice_vectors(lattice, pitch):
+    """Explicit 2 x 2 lattice vectors, or a named lattice scaled by pitch.
+    An unknown name raises NotImplementedError, as the reference does."""
+    if not isinstance(lattice, str):
+        return lattice
+    rows = _NAMED_LATTICES.get(lattice)
+    if rows is None:
+        raise NotImplementedError(f"lattice {lattice!r} is not available; use one of "
+                                  f"{sorted(_NAMED_LATTICES)} or explicit lattice vectors")
+    return pitch * np.asarray(rows)
+
+
 class Crystal:
     def __init__(self, pw, lattice="square", lattice_pitch=1, void=False, epsi=1, epse=1, device=None) -> None:
-        self.pw = pw
-        self.a = lattice_pitch
-        self.void = void
-        self.epsi = epsi
-        self.epse = epse
-        self.source = None
-        self.device = torch.device(device) if device is not None else None
-        if isinstance(lattice, str):
-            if lattice == "square":
-                self.lattice = self.a * np.asarray([[1, 0], [0, 1]])
-            elif lattice == "hexagonal":
-                self.lattice = self.a * np.asarray([[np.sqrt(3) / 2, 0.5], [np.sqrt(3) / 2, -0.5]])
-            else:
-                raise NotImplementedError(f"This {lattice} magic-string is not emplemented.")
-        else:
-            self.lattice = lattice
+        self.pw, self.a, self.void = pw, lattice_pitch, void
+        self.epsi, self.epse = epsi, epse
+        self.device = None if device is None else torch.device(device)
+        self.lattice = _lattice_vectors(lattice, lattice_pitch)
         self.expansion = Expansion(pw, self.lattice)
-        self.layers = dict()
-        self.stacking_matrices = list()
-        self.stacking_reverse_matrices = None
+        self.layers = {}
+        self.source = None
+        # solve results: full S-matrix chain (S, stacking matrices) or R/T only
+        self.S, self._RT = None, None
+        self.stacking_matrices, self.stacking_reverse_matrices = [], None
         self.stack_positions = []
-        self.S = None
-        self._RT = None
 
     @classmethod
     def from_expansion(cls, expansion, **kwargs):
-        obj = cls(expansion.pw, **kwargs)
-        obj.expansion = expansion
-        return obj
+        """A crystal over a given (e.g. twisted) expansion; kwargs go to __init__."""
+        crystal = cls(expansion.pw, **kwargs)
+        crystal.expansion = expansion
+        return crystal
 
     # ------------------------------------------------------------------ layers
     def add_layer_uniform(self, name, epsilon, depth):
         self.layers[name] = Layer.uniform(self.expansion, epsilon, depth)
 
-    def add_pixmap_or_uniform(self, name, epsilon, depth):
-        sample = epsilon.flatten()[0]
-        if np.all(epsilon == sample):
-            self.add_layer_uniform(name, sample, depth)
-        else:
-            self.add_layer_pixmap(name, epsilon, depth)
-
     def add_layer_pixmap(self, name, epsilon, depth):
         self.layers[name] = Layer.pixmap(self.expansion, epsilon, depth)
+
+    def add_pixmap_or_uniform(self, name, epsilon, depth):
+        """A constant map becomes a uniform layer of that value."""
+        first = epsilon.flat[0]
+        if (epsilon == first).all():
+            self.add_layer_uniform(name, first, depth)
+        else:
+            self.add_layer_pixmap(name, epsilon, depth)
 
     def add_layer_analytical(self, name, epsilon, epsilon_host, depth):
         self.layers[name] = Layer.analytical(self.expansion, epsilon, epsilon_host, depth)
@@ -82,28 +94,28 @@ class Crystal:
         self.layers[name] = ExtendedLayer(self.expansion, layer) if extended else layer
 
     def set_device(self, layers_stack, fields_mask=False):
-        """Prepends / appends the half spaces (khepri/crystal.py:131-164)."""
-        self.device_stack = copy(layers_stack)
-        self.global_stacking = []
-        if not self.void:
-            self.global_stacking.append("Sref")
-        self.global_stacking.extend(layers_stack)
-        if not self.void:
-            self.global_stacking.append("Strans")
-        required = set(self.global_stacking)
-        if "Sref" in required and "Sref" not in self.layers:
-            self.layers["Sref"] = Layer.half_infinite(self.expansion, "reflexion", self.epsi)
-            self.layers["Sref"].fields = True
-        if "Strans" in required and "Strans" not in self.layers:
-            self.layers["Strans"] = Layer.half_infinite(self.expansion, "transmission", self.epse)
-            self.layers["Strans"].fields = True
-        fields_mask = [fields_mask] * len(layers_stack) if isinstance(fields_mask, bool) else fields_mask
-        self.stack_retain_mask = [True] + list(fields_mask) + [True]
+        """The user's stack framed by the incidence and emergence half spaces
+        (unless void), which are created on first use; fields_mask (one flag,
+        or one per user layer) marks the layers whose fields are kept
+        (khepri/crystal.py:131-164)."""
+        self.device_stack = list(layers_stack)
+        framed = not self.void
+        self.global_stacking = (["Sref"] if framed else []) + self.device_stack + (["Strans"] if framed else [])
+        for name, side, medium in _HALF_SPACES:
+            if name in self.global_stacking and name not in self.layers:
+                half = Layer.half_infinite(self.expansion, side, getattr(self, medium))
+                half.fields = True
+                self.layers[name] = half
+        if isinstance(fields_mask, bool):
+            fields_mask = [fields_mask] * len(self.device_stack)
+        self.stack_retain_mask = [True, *fields_mask, True]
         self._fields_requested = any(fields_mask)
-        for name, enabled in zip(self.global_stacking, self.stack_retain_mask):
-            self.layers[name].fields |= enabled
-            if hasattr(self.layers[name], "base"):
-                self.layers[name].base.fields |= enabled
+        for name, keep in zip(self.global_stacking, self.stack_retain_mask):
+            layer = self.layers[name]
+            layer.fields |= keep
+            base = getattr(layer, "base", None)
+            if base is not None:
+                base.fields |= keep
 
     @property
     def depth(self):

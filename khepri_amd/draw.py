@@ -13,36 +13,39 @@ import numpy as np
 
 
 class Drawing:
+    """A unit cell painted shape by shape; ``canvas()`` is the permittivity
+    map, ``islands()`` the same shapes as closed-form island descriptions
+    (corner coordinates shifted to the cell's [0, 1] frame)."""
+
     def __init__(self, shape, epsilon_background, lattice=None) -> None:
-        if lattice is None:
-            lattice = np.array([[1, 0], [0, 1]])
-        self.lattice = np.asarray(lattice)
+        self.lattice = np.eye(2, dtype=int) if lattice is None else np.asarray(lattice)
         self.background = epsilon_background
-        self._canvas = np.ones(shape) * epsilon_background
-        xbar = np.linspace(-0.5, 0.5, shape[0])
-        ybar = np.linspace(-0.5, 0.5, shape[1])
-        nX, nY = np.meshgrid(xbar, ybar, indexing="ij")
-        self.X = self.lattice[0, 0] * nX + self.lattice[1, 0] * nY
-        self.Y = self.lattice[0, 1] * nX + self.lattice[1, 1] * nY
+        self._canvas = epsilon_background * np.ones(shape)
+        # pixel centres in lattice coordinates (u along the first vector, v
+        # along the second), then in Cartesian coordinates
+        u, v = np.meshgrid(np.linspace(-0.5, 0.5, shape[0]), np.linspace(-0.5, 0.5, shape[1]), indexing="ij")
+        (a1x, a1y), (a2x, a2y) = self.lattice[0], self.lattice[1]
+        self.X = a1x * u + a2x * v
+        self.Y = a1y * u + a2y * v
         self.geometric_description = []
 
+    def _paint(self, inside, epsilon, kind, params):
+        self._canvas[inside] = epsilon
+        self.geometric_description.append({"type": kind, "params": params, "epsilon": epsilon})
+
     def disc(self, xy, radius, epsilon):
-        x, y = xy
-        mask = np.sqrt((self.X - x) ** 2 + (self.Y - y) ** 2) < radius
-        self._canvas[mask] = epsilon
-        self.geometric_description.append(
-            {"type": "disc", "params": [0.5 + x, 0.5 + y, radius], "epsilon": epsilon})
+        """Pixels strictly closer than radius to the centre xy."""
+        cx, cy = xy
+        inside = np.sqrt((self.X - cx) ** 2 + (self.Y - cy) ** 2) < radius
+        self._paint(inside, epsilon, "disc", [0.5 + cx, 0.5 + cy, radius])
 
     def rectangle(self, xy, wh, epsilon):
-        x, y = xy
+        """Pixels inside the closed w x h rectangle centred on xy."""
         w, h = wh
-        x = x - w / 2
-        y = y - h / 2
-        mask = (self.X >= x) & (self.X <= x + w) & (self.Y >= y) & (self.Y <= y + h)
-        self._canvas[mask] = epsilon
-        self.geometric_description.append(
-            {"type": "rectangle", "params": [0.5 + x, 0.5 + y, 0.5 + x + w, 0.5 + y + h],
-             "epsilon": epsilon})
+        left, bottom = xy[0] - w / 2, xy[1] - h / 2
+        right, top = left + w, bottom + h
+        inside = (self.X >= left) & (self.X <= right) & (self.Y >= bottom) & (self.Y <= top)
+        self._paint(inside, epsilon, "rectangle", [0.5 + left, 0.5 + bottom, 0.5 + right, 0.5 + top])
 
     def canvas(self):
         return self._canvas.copy()

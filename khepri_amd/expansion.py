@@ -15,40 +15,57 @@ TWOPI = 2 * np.pi
 
 
 def generate_expansion_indices(pw):
-    """(2, N) integer pairs (m, n), m fastest (khepri/expansion.py:4-16)."""
-    assert pw[0] % 2 == 1
-    assert pw[1] % 2 == 1
-    M, N = (pw[0] - 1) // 2, (pw[1] - 1) // 2
-    mm, nn = np.meshgrid(np.arange(-M, M + 1), np.arange(-N, N + 1))
-    return np.vstack([mm.ravel(), nn.ravel()])
+    """Harmonic orders as a (2, P Q) integer array for pw = (P, Q), both odd:
+    row 0 holds m in -(P-1)/2 .. (P-1)/2, row 1 holds n likewise, m varying
+    fastest -- the ordering of khepri/expansion.py:4-16, which every matrix
+    of the basis follows.  An even count fails the same assertion."""
+    P, Q = pw
+    assert P % 2 == 1 and Q % 2 == 1, f"plane-wave counts must be odd, got {pw}"
+    m = np.arange(P) - P // 2
+    n = np.arange(Q) - Q // 2
+    return np.stack([np.tile(m, Q), np.repeat(n, P)])
 
 
-def reciproc(a1, a2):
-    """khepri/tools.py:64-69."""
-    coef = TWOPI / (a1[0] * a2[1] - a1[1] * a2[0])
-    return (a2[1] * coef, -a2[0] * coef), (-a1[1] * coef, a1[0] * coef)
+def _cross(a1, a2):
+    """z component of a1 x a2 (the signed cell area)."""
+    return a1[0] * a2[1] - a1[1] * a2[0]
 
 
 def unitcellarea(a1, a2):
-    return abs(a1[0] * a2[1] - a1[1] * a2[0])
+    """Area of the cell spanned by a1, a2."""
+    return abs(_cross(a1, a2))
+
+
+def reciproc(a1, a2):
+    """Reciprocal basis (b1, b2) with a_i . b_j = 2 pi delta_ij, as tuples
+    (khepri/tools.py:64-69): b1 is a2 turned by -90 degrees, b2 is a1 turned
+    by +90 degrees, both scaled by 2 pi / (a1 x a2)."""
+    scale = TWOPI / _cross(a1, a2)
+    b1 = (a2[1] * scale, -a2[0] * scale)
+    b2 = (-a1[1] * scale, a1[0] * scale)
+    return b1, b2
 
 
 def rotation_matrix(theta):
-    return np.array([[np.cos(theta), -np.sin(theta)], [np.sin(theta), np.cos(theta)]])
+    """Counter-clockwise rotation by theta radians."""
+    c, s = np.cos(theta), np.sin(theta)
+    return np.array([[c, -s], [s, c]])
 
 
 def compute_kplanar(eps_inc, wavelength, theta_deg=0.0, phi_deg=0.0):
-    """In-plane wave vector from angles in degrees (khepri/tools.py:17-21)."""
-    phi = np.deg2rad(phi_deg)
-    theta = np.deg2rad(theta_deg)
-    kp = np.array([np.cos(phi), np.sin(phi)], dtype=complex)
-    return kp * cmath.sqrt(eps_inc) * TWOPI / wavelength * np.sin(theta)
+    """In-plane wave vector of a plane wave in the incidence medium, polar
+    angle theta and azimuth phi in degrees (khepri/tools.py:17-21):
+    n k0 sin(theta) (cos phi, sin phi), complex."""
+    k_inc = cmath.sqrt(eps_inc) * TWOPI / wavelength
+    theta, phi = np.deg2rad(theta_deg), np.deg2rad(phi_deg)
+    return (k_inc * np.sin(theta)) * np.array([np.cos(phi), np.sin(phi)], dtype=complex)
 
 
 def kz_from_kplanar(kx, ky, k0, epsilon):
-    """khepri/expansion.py:19-26."""
-    arg = k0 ** 2 * np.conj(epsilon) - kx ** 2 - ky ** 2
-    return np.conj(np.sqrt(np.asarray(arg).astype(complex)))
+    """Normal wave-vector component with the reference's branch
+    (khepri/expansion.py:19-26): conj(sqrt(k0^2 conj(eps) - kx^2 - ky^2))."""
+    kz2 = np.asarray(k0 ** 2 * np.conj(epsilon) - kx ** 2 - ky ** 2, dtype=complex)
+    return np.conj(np.sqrt(kz2))
 
 
 class Expansion:

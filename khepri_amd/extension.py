@@ -18,21 +18,20 @@ class ExtendedLayer:
     """khepri/extension.py:66-112."""
 
     def __init__(self, expansion, base) -> None:
-        if expansion.expansion_lhs is base.expansion:
-            self.mode = 1
-            self.gs = expansion.expansion_rhs.g_vectors
-        elif expansion.expansion_rhs is base.expansion:
-            self.mode = 0
-            self.gs = expansion.expansion_lhs.g_vectors
-        else:
-            raise NotImplementedError("Base layer expansion should be in the extented expansion.")
-        self.expansion = expansion
-        self.base = base
-        self.depth = self.base.depth
-        self.fields = self.base.fields
+        # the base lives on one side of the product expansion; its solves are
+        # shifted by the harmonics of the other side (mode: which side is the
+        # base; the left side wins when both sides are the base's expansion)
+        sides = {id(expansion.expansion_rhs): (0, expansion.expansion_lhs),
+                 id(expansion.expansion_lhs): (1, expansion.expansion_rhs)}
+        if id(base.expansion) not in sides:
+            raise NotImplementedError("the base layer's expansion must be one of the two factors "
+                                      "of the extended expansion")
+        self.mode, other = sides[id(base.expansion)]
+        self.gs = other.g_vectors
+        self.expansion, self.base = expansion, base
+        self.depth, self.fields = base.depth, base.fields
         self.S = self.W = self.V = self.L = None
-        self.IC = 1.0
-        self.info = None
+        self.IC, self.info = 1.0, None
 
     def _base_sources(self, kp, wl):
         """(B*Nb, 2) kp + g and (B*Nb,) wavelengths; base solve m of item b is b*Nb + m."""

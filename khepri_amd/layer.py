@@ -71,57 +71,49 @@ class Layer:
         self.info = None
 
     # ------------------------------------------------------------- builders
+    # The reference's classmethods (khepri/layer.py:78-129): each records the
+    # layer's formulation, its permittivity description and its depth.
+    @classmethod
+    def _build(cls, expansion, formulation, epsilon, depth, **extra):
+        layer = cls()
+        layer.expansion, layer.formulation = expansion, formulation
+        layer.epsilon, layer.depth = epsilon, depth
+        for key, value in extra.items():
+            setattr(layer, key, value)
+        return layer
+
     @classmethod
     def pixmap_or_uniform(cls, expansion, pixmap, depth):
-        eps0 = pixmap.flatten()[0]
-        if np.all(pixmap == eps0):
-            return Layer.uniform(expansion, eps0, depth)
-        return Layer.pixmap(expansion, pixmap, depth)
+        """A uniform layer when every pixel holds the same value, else a pixmap layer."""
+        first = pixmap.flat[0]
+        if (pixmap == first).all():
+            return cls.uniform(expansion, first, depth)
+        return cls.pixmap(expansion, pixmap, depth)
 
     @classmethod
     def pixmap(cls, expansion, pixmap, depth):
-        layer = cls()
-        layer.expansion = expansion
-        layer.formulation = Formulation.FFT
-        layer.epsilon = pixmap
-        layer.depth = depth
-        return layer
+        """Convolution matrix from the 2-D DFT of the pixel map."""
+        return cls._build(expansion, Formulation.FFT, pixmap, depth)
 
     @classmethod
     def uniform(cls, expansion, epsilon, depth):
-        layer = cls()
-        layer.expansion = expansion
-        layer.formulation = Formulation.UNIFORM
-        layer.epsilon = epsilon
-        layer.depth = depth
-        return layer
+        """Homogeneous layer: modes in closed form, no eigenproblem."""
+        return cls._build(expansion, Formulation.UNIFORM, epsilon, depth)
 
     @classmethod
     def half_infinite(cls, expansion, type, epsilon):
-        layer = cls()
-        if type == "reflexion":
-            layer.formulation = Formulation.HALF_SPACE_INC
-        elif type == "transmission":
-            layer.formulation = Formulation.HALF_SPACE_TRN
-        else:
+        """Incidence ("reflexion") or emergence ("transmission") half space."""
+        sides = {"reflexion": Formulation.HALF_SPACE_INC, "transmission": Formulation.HALF_SPACE_TRN}
+        if type not in sides:
             raise ValueError(f"half space type must be 'reflexion' or 'transmission', got {type!r}")
-        layer.expansion = expansion
-        layer.depth = 0
-        layer.epsilon = epsilon
-        return layer
+        return cls._build(expansion, sides[type], epsilon, 0)
 
     @classmethod
     def analytical(cls, expansion, islands_description, eps_host, depth):
         """Island layer with closed-form Fourier coefficients
         (khepri/layer.py:112-119): islands_description is a list of
         {"type": "disc" | "rectangle", "params": [...], "epsilon": eps}."""
-        layer = cls()
-        layer.expansion = expansion
-        layer.formulation = Formulation.ANALYTICAL
-        layer.epsilon = islands_description
-        layer.eps_host = eps_host
-        layer.depth = depth
-        return layer
+        return cls._build(expansion, Formulation.ANALYTICAL, islands_description, depth, eps_host=eps_host)
 
     @property
     def patterned(self):
