@@ -40,6 +40,9 @@ constexpr int EIG_QB = EIG_QB_SIZE;  // QR chase block held in LDS when H is in 
 #define HB_NB_SIZE 8
 #endif
 constexpr int HB_NB = HB_NB_SIZE;  // panel width of the blocked Hessenberg reduction
+#ifndef EIG_MV_ROWS
+#define EIG_MV_ROWS 2  // rows per wavefront step in the Hessenberg panel's matrix-vector product (1, 3, 4, 8: slower)
+#endif
 
 // multishift QR (chain passes, see ms_chain_pass)
 #ifndef MS_NS_SIZE
@@ -81,6 +84,12 @@ __host__ __device__ __forceinline__ int eig_part_elems(int n, int waves, bool ch
   }
   return e;
 }
+// The Hessenberg stage's LDS copy of the current panel (rows x HB_NB, flags
+// bit 3) sits in part after the stage's own factors and staged rows.
+__host__ __device__ __forceinline__ int hess_pan_offset(int waves) {
+  return HB_NB * HB_NB + waves * HB_NB + 2 * HB_NB + 2 * 8 * HB_NB;
+}
+__host__ __device__ __forceinline__ int hess_part_elems(int n, int waves) { return hess_pan_offset(waves) + n * HB_NB; }
 __host__ __device__ __forceinline__ int eig_tail_elems(int n, bool chain) {
   return n <= KHX_EIG_LDS_N ? n * n : (chain ? (n + 3) / 4 + MS_NS : 0);
 }
@@ -328,7 +337,11 @@ __device__ __forceinline__ void trailing_pass(cd* M, int n, int c0, int r0, cons
   __syncthreads();
 }
 
-template <int EIG_THREADS>
+// LDSP: the panel's columns (rows k0+1..) live in LDS while the panel is
+// factored (steps 1-3 read and write them there; the matrix-vector products
+// read the panel-start matrix from H, whose panel columns are untouched
+// until the panel is written back).  Same operations in the same order.
+template <int EIG_THREADS, bool LDSP>
 __device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, cd* Y, cd* Vt, cd* u, cd* vh, cd* part,
                                    double* red) {
   constexpr int EIG_WAVES = EIG_THREADS / 64;
@@ -339,6 +352,9 @@ __device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, 
   cd* wv = wsum + EIG_WAVES * HB_NB;  // HB_NB reduced vector
   cd* wt = wv + HB_NB;                // HB_NB transformed vector
   cd* stage = wt + HB_NB;             // 2 x HB_RC x HB_NB staged factor rows
+  cd* pan = part + hess_pan_offset(EIG_WAVES);  // LDSP: pan[i * HB_NB + j] = H[i][k0 + j], rows k0+1..
+  // element (i, c) of the panel column c = k0 + j
+  auto pc = [&](int i, int c, int j) -> cd& { return LDSP ? pan[i * HB_NB + j] : H[(long long)i * n + c]; };
   // wv[q] = sum_{i >= r0} conj(V(i, q)) x_i for every q (columns of V past the
   // current reflector are zero): thread -> (q = tid % HB_NB, every 16th row).
   auto vh_dot = [&](const cd* x, long long xstride, int r0) {
@@ -369,6 +385,9 @@ __device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, 
       V[e] = mkc(0.0, 0.0);
       Y[e] = mkc(0.0, 0.0);
     }
+    if (LDSP)
+      for (int e = (k0 + 1) * HB_NB + tid; e < n * HB_NB; e += EIG_THREADS)
+        if (e % HB_NB < nbp) pan[e] = H[(long long)(e / HB_NB) * n + k0 + e % HB_NB];
     __syncthreads();
     for (int j = 0; j < nbp; ++j) {
       const int c = k0 + j;
@@ -378,13 +397,16 @@ __device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, 
         for (int i = k0 + 1 + tid; i < n; i += EIG_THREADS) {
           const cd* yr = Y + (long long)i * HB_NB;
           const cd* vc = V + (long long)c * HB_NB;
-          cd a = H[(long long)i * n + c];
+          cd a = pc(i, c, j);
           for (int q = 0; q < j; ++q) a = fmsc(a, yr[q], conjc(vc[q]));
-          H[(long long)i * n + c] = a;
+          pc(i, c, j) = a;
         }
         __syncthreads();
         // (2) left update of rows k0+1.. of column c: x -= V T^H V^H x
-        vh_dot(H + c, n, k0 + 1);
+        if (LDSP)
+          vh_dot(pan + j, HB_NB, k0 + 1);
+        else
+          vh_dot(H + c, n, k0 + 1);
         if (tid < j) {
           cd t = mkc(0.0, 0.0);
           for (int p = 0; p <= tid; ++p) t = fmac_cj(t, Ts[p * HB_NB + tid], wv[p]);
@@ -393,17 +415,17 @@ __device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, 
         __syncthreads();
         for (int i = k0 + 1 + tid; i < n; i += EIG_THREADS) {
           const cd* vr = V + (long long)i * HB_NB;
-          cd a = H[(long long)i * n + c];
+          cd a = pc(i, c, j);
           for (int q = 0; q < j; ++q) a = fmsc(a, vr[q], wt[q]);
-          H[(long long)i * n + c] = a;
+          pc(i, c, j) = a;
         }
         __syncthreads();
       }
       // (3) reflector from H[c+1.., c]
       const int L = n - c - 1;
-      const cd alpha = H[(long long)(c + 1) * n + c];  // read before the barrier in block_sum
+      const cd alpha = pc(c + 1, c, j);  // read before the barrier in block_sum
       double part2 = 0.0;
-      for (int i = c + 2 + tid; i < n; i += EIG_THREADS) part2 += abs2c(H[(long long)i * n + c]);
+      for (int i = c + 2 + tid; i < n; i += EIG_THREADS) part2 += abs2c(pc(i, c, j));
       const double rest2 = block_sum<EIG_THREADS>(part2, red);
       const double aa = absc(alpha);
       const double xn = sqrt(aa * aa + rest2);
@@ -411,22 +433,24 @@ __device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, 
       const bool skip = (rest2 == 0.0);  // already reduced: P = I (u = 0)
       const double sc = skip ? 0.0 : 1.0 / sqrt(xn * (xn + aa));
       for (int i = c + 1 + tid; i < n; i += EIG_THREADS) {
-        cd x = H[(long long)i * n + c];
+        cd x = pc(i, c, j);
         if (i == c + 1) x = x + xn * ph;
         const cd v = sc * x;
         u[i - c - 1] = v;
         V[(long long)i * HB_NB + j] = v;
-        if (!skip) H[(long long)i * n + c] = (i == c + 1) ? -(xn * ph) : mkc(0.0, 0.0);
+        if (!skip) pc(i, c, j) = (i == c + 1) ? -(xn * ph) : mkc(0.0, 0.0);
       }
       __syncthreads();
-      // (4) vh = H(:, c+1:) u  (panel-start columns): each wavefront takes four
-      // rows at a time (four independent loads in flight per lane)
-      for (int i0 = k0 + 1 + wid * 4; i0 < n; i0 += EIG_WAVES * 4) {
-        double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
+      // (4) vh = H(:, c+1:) u  (panel-start columns): each wavefront takes
+      // EIG_MV_ROWS rows at a time (as many independent loads in flight per lane)
+      for (int i0 = k0 + 1 + wid * EIG_MV_ROWS; i0 < n; i0 += EIG_WAVES * EIG_MV_ROWS) {
+        double sx[EIG_MV_ROWS], sy[EIG_MV_ROWS];
+#pragma unroll
+        for (int r = 0; r < EIG_MV_ROWS; ++r) sx[r] = sy[r] = 0.0;
         for (int q = lane; q < L; q += 64) {
           const cd uq = u[q];
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
+          for (int r = 0; r < EIG_MV_ROWS; ++r)
             if (i0 + r < n) {
               const cd t = H[(long long)(i0 + r) * n + c + 1 + q] * uq;
               sx[r] += t.x;
@@ -434,7 +458,7 @@ __device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, 
             }
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < EIG_MV_ROWS; ++r) {
           const double a = wave_sum(sx[r]), b = wave_sum(sy[r]);
           if (lane == 0 && i0 + r < n) vh[i0 + r] = mkc(a, b);
         }
@@ -457,6 +481,9 @@ __device__ __forceinline__ void hessenberg_blocked(cd* H, cd* Zt, int n, cd* V, 
       }
       __syncthreads();
     }
+    if (LDSP)  // the finished panel columns back to H (their rows 0..k0 are updated in H below)
+      for (int e = (k0 + 1) * HB_NB + tid; e < n * HB_NB; e += EIG_THREADS)
+        if (e % HB_NB < nbp) H[(long long)(e / HB_NB) * n + k0 + e % HB_NB] = pan[e];
     // Vt = V T (the left factor of both trailing updates), one thread per row
     for (int i = tid; i < n; i += EIG_THREADS) {
       const cd* vr = V + (long long)i * HB_NB;
@@ -1786,7 +1813,14 @@ __device__ __forceinline__ void eig_body(int n, cd* A, long long sA, cd* Ztg, cd
 
   // ------------------------------------------------------------ 1. Hessenberg
   if (!in_lds)
-    hessenberg_blocked<EIG_THREADS>(H, Zt, n, X, X + (long long)n * HB_NB, X + 2ll * n * HB_NB, u, vh, part, red);
+  {
+    if (flags & 8)
+      hessenberg_blocked<EIG_THREADS, true>(H, Zt, n, X, X + (long long)n * HB_NB, X + 2ll * n * HB_NB, u, vh, part,
+                                            red);
+    else
+      hessenberg_blocked<EIG_THREADS, false>(H, Zt, n, X, X + (long long)n * HB_NB, X + 2ll * n * HB_NB, u, vh, part,
+                                             red);
+  }
   for (int k = 0; in_lds && k + 2 < n; ++k) {
     const int L = n - k - 1;  // reflector acts on rows/cols k+1..n-1
     double part2 = 0.0;
@@ -2401,9 +2435,20 @@ static int launch_eig(hipStream_t s, int n, int batch, size_t lds, Mat A, cd* Zt
   KHX_TRY(ensure_dynamic_lds((const void*)zeig_hess_kernel<T>, lds));
   KHX_TRY(ensure_dynamic_lds(qr, ldsq));
   KHX_TRY(ensure_dynamic_lds((const void*)zeig_vec_kernel<T>, lds));
+  // Hessenberg panel in LDS (flags bit 3) where it fits the launch's LDS, or
+  // where a larger LDS costs nothing (512 or more threads: one matrix per CU)
+  const int hw = T / 64;
+  const size_t ldsh_need =
+      lds + (size_t)std::max(0, hess_part_elems(n, hw) - eig_part_elems(n, hw, false)) * sizeof(cd);
+  const char* hl_env = getenv("KHX_EIG_HESS_LDS");
+  const bool hess_lds = !(hl_env && atoi(hl_env) == 0) && (ldsh_need == lds || T >= 512) && ldsh_need <= 150 * 1024;
+  const size_t ldsh = hess_lds ? ldsh_need : lds;
+  const int hflags = flags | (hess_lds ? 8 : 0);
+  if (ldsh != lds) KHX_TRY(ensure_dynamic_lds((const void*)zeig_hess_kernel<T>, ldsh));
   const int pt = eig_pair_threads(n, T, tq, flags, pad);
   if (pt) KHX_TRY(ensure_dynamic_lds(eig_pair_fn(pt), eig_pair_lds_bytes(n)));
-  zeig_hess_kernel<T><<<dim3(batch), dim3(T), lds, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase, flags);
+  zeig_hess_kernel<T><<<dim3(batch), dim3(T), ldsh, s>>>(n, A.p, A.stride, Zt, X, sZ, lam, lstride, info, tphase,
+                                                         hflags);
   KHX_LAUNCH_CHECK();
   if (pt) {
     size_t ldsp = eig_pair_lds_bytes(n);

@@ -486,6 +486,43 @@ def test_eig_paired_schur_matches_single(gpu, monkeypatch, pw, mode):
     np.testing.assert_array_equal(infos["0"], infos[mode])
 
 
+@pytest.mark.parametrize("pw,big", [((7, 7), False), ((9, 9), False), ((9, 9), True), ((11, 11), False),
+                                    ((13, 13), False)])
+def test_eig_hessenberg_panel_in_lds_matches_global(gpu, monkeypatch, pw, big):
+    """The Hessenberg panel factored in LDS against the same steps on H in
+    global memory (KHX_EIG_HESS_LDS=0): same operations in the same order, so
+    eigenvalues and eigenvectors agree bit for bit.  Small batches run 512
+    threads per matrix (the panel copy may enlarge the launch's LDS), a batch
+    larger than the CU count 256 threads (the copy only where it fits)."""
+    from khepri_amd import ops
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    pix = W.disc_pixmap()
+    C = rcwa.convolution_matrix(pix, pw)
+    g = rcwa.g_vectors(pw, np.eye(2))
+    rng = np.random.default_rng(sum(pw) + big)
+    mats = []
+    for i in range(3):
+        kx, ky, _ = rcwa.k_vectors(g, tuple(rng.uniform(-0.4, 0.4, 2)), rng.uniform(0.9, 2.2))
+        P, Q = rcwa.structured_pq(kx, ky, C)
+        mats.append(P @ Q)
+    n = mats[0].shape[0]
+    mats.append(crand(rng, n, n))
+    M = np.stack(mats)
+    if big:
+        M = M[rng.integers(0, len(M), cus + 5)]
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("KHX_EIG_HESS_LDS", flag)
+        lam, Wm, info = ops.eig(dev(M, gpu))
+        out[flag] = (lam.cpu().numpy(), Wm.cpu().numpy(), info.cpu().numpy())
+    assert (out["1"][2] == 0).all()
+    for a, b in zip(out["0"], out["1"]):
+        np.testing.assert_array_equal(a, b)
+    lam, Wm, _ = out["1"]
+    for b in range(4):
+        assert np.abs(M[b] @ Wm[b] - Wm[b] * lam[b]).max() / np.abs(M[b]).max() < 1e-11
+
+
 @pytest.mark.parametrize("n,m", [(162, 64), (162, 324), (250, 100)])
 def test_lu_mixed_refined(gpu, monkeypatch, n, m):
     """KHX_LU_MIXED=1: FP64 LU, triangular-solve products on the FP32 matrix
