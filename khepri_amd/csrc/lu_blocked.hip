@@ -795,6 +795,12 @@ static bool panel_reg_enabled() {
   const char* env = getenv("KHX_LU_PANEL_REG");
   return !(env && atoi(env) == 0);
 }
+// threads of the 16-column register panel for short panels: 64 (one
+// wavefront, up to 192 rows) or 128 (up to 256 rows) through KHX_LU_PANEL_TH
+static int panel_reg_threads() {
+  const char* env = getenv("KHX_LU_PANEL_TH");
+  return env ? atoi(env) : LB_THREADS;
+}
 
 template <int NB>
 static int launch_panel(hipStream_t s, int n, int m, int j0, Mat A, Mat B, int batch, int* ipiv, int* info, int mode,
@@ -812,7 +818,14 @@ static int launch_panel(hipStream_t s, int n, int m, int j0, Mat A, Mat B, int b
     const int pr = n - j0;
     const size_t l11 = NB * (NB + 1) * sizeof(cd);
     if (pr <= 3 * 512 && panel_reg_enabled()) {
-      if (pr <= LB_THREADS)
+      const int th = panel_reg_threads();
+      if (th == 64 && pr <= 3 * 64)
+        zlu_panel_reg_kernel<NB, 3, 64><<<dim3(batch), dim3(64), l11, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p, B.ld,
+                                                                          B.stride, ipiv, info, mode, cend);
+      else if (th == 128 && pr <= 2 * 128)
+        zlu_panel_reg_kernel<NB, 2, 128><<<dim3(batch), dim3(128), l11, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p,
+                                                                            B.ld, B.stride, ipiv, info, mode, cend);
+      else if (pr <= LB_THREADS)
         zlu_panel_reg_kernel<NB, 1><<<dim3(batch), dim3(LB_THREADS), l11, s>>>(n, m, j0, A.p, A.ld, A.stride, B.p,
                                                                               B.ld, B.stride, ipiv, info, mode, cend);
       else if (pr <= 2 * LB_THREADS)

@@ -547,17 +547,21 @@ def test_lu_mixed_refined(gpu, monkeypatch, n, m):
     assert np.abs(X64.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-11
 
 
-@pytest.mark.parametrize("n,nb", [(97, 16), (162, 16), (250, 16), (338, 16), (600, 16), (1250, 16),
-                                  (97, 32), (162, 32), (250, 32), (338, 32)])
-def test_lu_register_panel_matches_lds_panel(gpu, monkeypatch, n, nb):
+@pytest.mark.parametrize("n,nb,th", [(97, 16, 256), (162, 16, 256), (250, 16, 256), (338, 16, 256),
+                                     (600, 16, 256), (1250, 16, 256), (97, 32, 256), (162, 32, 256),
+                                     (250, 32, 256), (338, 32, 256), (97, 16, 64), (162, 16, 64),
+                                     (250, 16, 64), (162, 16, 128), (250, 16, 128)])
+def test_lu_register_panel_matches_lds_panel(gpu, monkeypatch, n, nb, th):
     """The register-resident panel kernel (16 columns: panel rows in registers
-    up to 512 rows on 256 threads, up to 1 536 on 512; 32 columns: up to 256
-    rows) and the LDS (or, past 150 KB, global-memory) panel kernel make the
-    same pivot choices and the same arithmetic: factors, pivots, solutions and
-    status words agree bit for bit, including an exactly singular system (zero
-    pivot) and a NaN column."""
+    up to 512 rows on 256 threads, up to 1 536 on 512, and through
+    KHX_LU_PANEL_TH up to 192 rows on one wavefront or 256 on 128 threads;
+    32 columns: up to 256 rows) and the LDS (or, past 150 KB, global-memory)
+    panel kernel make the same pivot choices and the same arithmetic:
+    factors, pivots, solutions and status words agree bit for bit, including
+    an exactly singular system (zero pivot) and a NaN column."""
     from khepri_amd import ops
     monkeypatch.setenv("KHX_LU_NB", str(nb))
+    monkeypatch.setenv("KHX_LU_PANEL_TH", str(th))
     rng = np.random.default_rng(n)
     A = crand(rng, 4, n, n)
     A[1, :, 5] = 0.0                       # singular: zero pivot in column 5
