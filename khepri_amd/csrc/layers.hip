@@ -517,7 +517,19 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   x_rhs_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(lam, k0d, n, batch, Bm, nn, R1, 2 * nn, xdiag);
   KHX_LAUNCH_CHECK();
   KHX_TRY(zcopy(s, Mat{Ac, n, nn}, Mat{Am, n, nn}, n, n, batch));
-  KHX_TRY(zgesv(s, n, 2 * n, Mat{Ac, n, nn}, Mat{R1, 2 * n, 2 * nn}, batch, info_lu3));
+  const char* ai_env = getenv("KHX_LAYER_AINV");
+  if (!(ai_env && atoi(ai_env) == 0)) {
+    // through A^-1: one solve with n right-hand sides, then G2 = A^-1 B as a
+    // GEMM and G1 = A^-1 X as a column scaling (X is diagonal); the
+    // reference's two solves with A (khepri/alternative.py:187-191) stay
+    // behind KHX_LAYER_AINV=0
+    KHX_TRY(zset_identity(s, Mat{R1, 2 * n, 2 * nn}, n, n, one, batch));
+    KHX_TRY(zgesv(s, n, n, Mat{Ac, n, nn}, Mat{R1, 2 * n, 2 * nn}, batch, info_lu3));
+    KHX_TRY(zgemm(s, n, n, n, one, Mat{R1, 2 * n, 2 * nn}, Mat{Bm, n, nn}, zero, Mat{R1 + n, 2 * n, 2 * nn}, batch));
+    KHX_TRY(zdiag_scale(s, Mat{R1, 2 * n, 2 * nn}, nullptr, xdiag, n, n, batch, Mat{R1, 2 * n, 2 * nn}));
+  } else {
+    KHX_TRY(zgesv(s, n, 2 * n, Mat{Ac, n, nn}, Mat{R1, 2 * n, 2 * nn}, batch, info_lu3));
+  }
   // M = X B G1
   KHX_TRY(zrow_scale(s, Mat{XB, n, nn}, xdiag, n, Mat{Bm, n, nn}, n, n, batch));
   KHX_TRY(zgemm(s, n, n, n, one, Mat{XB, n, nn}, Mat{R1, 2 * n, 2 * nn}, zero, Mat{M, n, nn}, batch));

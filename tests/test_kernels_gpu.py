@@ -114,10 +114,14 @@ def _layer_inputs(pw, kp, wl):
     return g, kx, ky
 
 
+@pytest.mark.parametrize("ainv", ["1", "0"])
 @pytest.mark.parametrize("pw,kp,wl", [((5, 5), (0.0, 0.0), 2.0), ((5, 5), (0.4, -0.3), 1.7),
                                       ((9, 9), (0.2, 0.1), 1.9)])
-def test_patterned_layer(gpu, pw, kp, wl):
+def test_patterned_layer(gpu, monkeypatch, pw, kp, wl, ainv):
+    """The layer S-matrix against the oracle's build_scatmat, with [G1 | G2]
+    through A^-1 (default) and through the reference's solves (KHX_LAYER_AINV=0)."""
     from khepri_amd import ops
+    monkeypatch.setenv("KHX_LAYER_AINV", ainv)
     pix = W.disc_pixmap()
     C = rcwa.convolution_matrix(pix, pw)
     g, kx, ky = _layer_inputs(pw, kp, wl)
