@@ -41,12 +41,14 @@ def counted_flops_per_solve(n):
     """Real flops the library's own op sequence spends per disc_stack_bz solve (one unique
     patterned layer, the R/T-only chain), complex multiply-add = 8 flops:
     P Q (two N x N x n GEMMs) 4 n^3; eigensolver 106 n^3 and its
-    back-transform W = Z X 8 n^3; V = Q W (two GEMMs) 4 n^3; W^-1 and V^-1 V0
-    (LU + n right-hand sides) 2 x (8/3 + 8) n^3; A^-1 (LU + n right-hand sides)
-    and G2 = A^-1 B (one GEMM) 8/3 + 8 + 8 n^3; T^-1 R2 (LU + 2n right-hand
-    sides) 8/3 + 16 n^3; four n x n x n GEMMs (X B G1, M B, M A, B G2) 32 n^3;
-    the one chain junction (one GEMM, one LU, vector solves) 11 n^3."""
-    return (4 + 106 + 8 + 4 + 2 * (8 / 3 + 8) + (8 / 3 + 8 + 8) + (8 / 3 + 16) + 32 + 11) * n ** 3
+    back-transform W = Z X 8 n^3; V = Q W (two GEMMs) 4 n^3; W^-1 from the
+    eigensolver's factors (one upper triangular solve with n right-hand sides
+    and one GEMM) 4 + 8 n^3; V^-1 V0 (LU + n right-hand sides) 8/3 + 8 n^3;
+    A^-1 (LU + n right-hand sides) and G2 = A^-1 B (one GEMM) 8/3 + 8 + 8 n^3;
+    T^-1 R2 (LU + 2n right-hand sides) 8/3 + 16 n^3; four n x n x n GEMMs
+    (X B G1, M B, M A, B G2) 32 n^3; the one chain junction (one GEMM, one LU,
+    vector solves) 11 n^3."""
+    return (4 + 106 + 8 + 4 + (4 + 8) + (8 / 3 + 8) + (8 / 3 + 8 + 8) + (8 / 3 + 16) + 32 + 11) * n ** 3
 
 
 def cpu_info():

@@ -2333,7 +2333,8 @@ __global__ KHX_EIG_BOUNDS(EIG_THREADS) void zeig_vec_kernel(KHX_EIG_ARGS) {
 }
 
 // Column normalisation of W: unit 2-norm, largest |component| made real.
-__global__ __launch_bounds__(256) void znormalize_cols_kernel(int n, cd* W, long long sW) {
+// rscale (optional, n per item): 1 / the factor applied to each column.
+__global__ __launch_bounds__(256) void znormalize_cols_kernel(int n, cd* W, long long sW, cd* rscale) {
   const long long item = blockIdx.x;
   cd* w = W + item * sW;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -2354,6 +2355,7 @@ __global__ __launch_bounds__(256) void znormalize_cols_kernel(int n, cd* W, long
     const cd f = inv * rot;
     for (int i = 0; i < n; ++i) w[(long long)i * n + k] = f * w[(long long)i * n + k];
     w[(long long)bi * n + k].y = 0.0;
+    if (rscale) rscale[item * n + k] = sqrt(ss) * conjc(rot);  // 1 / f (|rot| = 1)
   }
 }
 
@@ -2515,7 +2517,7 @@ int zeig_slots(int n) {
 }
 
 int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws,
-         unsigned long long* tphase) {
+         unsigned long long* tphase, cd* rscale) {
   if (batch <= 0) return 0;
   KHX_REQUIRE(n >= 1 && n <= 8192, "zeig: bad order");
   KHX_REQUIRE(A.ld == n && W.ld == n && A.stride != 0 && W.stride != 0,
@@ -2568,7 +2570,7 @@ int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int bat
   }
   // W = Z X = Zt^T X
   KHX_TRY(zgemm_ta(s, n, n, n, mkc(1.0, 0.0), Mat{Zt, n, sZ}, Mat{X, n, sZ}, mkc(0.0, 0.0), W, batch));
-  znormalize_cols_kernel<<<dim3(batch), dim3(256), 0, s>>>(n, W.p, W.stride);
+  znormalize_cols_kernel<<<dim3(batch), dim3(256), 0, s>>>(n, W.p, W.stride, rscale);
   KHX_LAUNCH_CHECK();
   return 0;
 }

@@ -62,6 +62,8 @@ int convmat_islands(hipStream_t s, const int* kinds, const double* params, const
                     const double* recip, double sigma, int P, int Q, cd* C, cd* Fws);
 // B <- A^-1 B from zgetrf factors: interchanges, then blocked triangular solves
 int zlu_solve_factored(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch);
+// B <- U^-1 B for the upper triangle U of A (its diagonal included)
+int ztrsm_upper(hipStream_t s, int n, int m, Mat A, Mat B, int batch);
 int zlu_blocked(hipStream_t s, int n, int m, Mat A, Mat B, int batch, int* ipiv, int* info, bool keepL);
 
 // elementwise helpers (all batched, n_rows x n_cols views)
@@ -95,10 +97,13 @@ int longitudinal_batch(hipStream_t s, const cd* kx, const cd* ky, const cd* F, i
 int idft_phase_batch(hipStream_t s, const cd* kx, const cd* ky, const double* wl, const double* x, const double* y,
                      int N, int npts, int batch, cd* E);
 
-// eigen-decomposition: A destroyed; lam (n per item), W (n x n, unit columns)
+// eigen-decomposition: A destroyed; lam (n per item), W (n x n, unit columns).
+// The workspace keeps Zt (the Schur vectors, transposed) and X (the unit upper
+// triangular eigenvectors of the Schur form) after the call: W = Zt^T X diag(f),
+// and rscale (optional, n per item) receives 1 / f.
 size_t zeig_workspace_bytes(int n, int batch);
 int zeig(hipStream_t s, int n, Mat A, cd* lam, long long lstride, Mat W, int batch, int* info, void* ws,
-         unsigned long long* tphase = nullptr);
+         unsigned long long* tphase = nullptr, cd* rscale = nullptr);
 // matrices in flight at once on the current device (a full round of workgroups)
 int zeig_slots(int n);
 // zeig with its workspace in a static device array (no allocation, no ws argument)

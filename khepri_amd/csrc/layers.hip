@@ -428,6 +428,17 @@ static dim3 grid_for(long long total) {
 //   t1(1) t2(1) Wc(2) A(1) B(1) Ac(1) R1(2) M(1) T(1) R2(2) XB(1)
 static long long patterned_items(int n) { return 22; }
 
+// A <- conj(A) over `total` elements
+__global__ void conj_inplace_kernel(cd* A, long long total) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x)
+    A[e].y = -A[e].y;
+}
+
+__global__ void zero_status_kernel(int* p, int count) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) p[e] = 0;
+}
+
 size_t patterned_layer_ws_bytes(int n, int batch) {
   return (size_t)patterned_items(n) * n * n * sizeof(cd) * batch + (size_t)3 * n * sizeof(cd) * batch +
          (size_t)batch * sizeof(int) * 8 + 4096;  // 5 status words per item fit in 8
@@ -484,8 +495,15 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   // PQ[:, 0:N] -= P[:, N:] C ;  PQ[:, N:] += P[:, 0:N] C
   KHX_TRY(zgemm(s, n, N, N, mone, Mat{P + N, n, nn}, Mat{const_cast<cd*>(C), N, cstride}, one, Mat{PQ, n, nn}, batch));
   KHX_TRY(zgemm(s, n, N, N, one, Mat{P, n, nn}, Mat{const_cast<cd*>(C), N, cstride}, one, Mat{PQ + N, n, nn}, batch));
+  // W^-1 from the eigensolver's factors: W = Z X diag(f) with Z unitary (Zt =
+  // Z^T in the workspace) and X unit upper triangular, so W^-1 = diag(1/f)
+  // X^-1 conj(Zt), with no factorization of W; 1/f goes to xdiag's room, unused
+  // until X is built.  KHX_LAYER_WINV=0: the reference's solve with W
+  // (khepri/alternative.py:182)
+  const char* wi_env = getenv("KHX_LAYER_WINV");
+  const bool winv = !(wi_env && atoi(wi_env) == 0);
   // (lam^2, W) = eig(PQ)
-  KHX_TRY(zeig(s, n, Mat{PQ, n, nn}, lam, n, Mat{W, n, nn}, batch, info_eig, eigws));
+  KHX_TRY(zeig(s, n, Mat{PQ, n, nn}, lam, n, Mat{W, n, nn}, batch, info_eig, eigws, nullptr, winv ? xdiag : nullptr));
   // lam = sqrt(lam^2 + 0j); V = Q W / lam   (Q W: diagonal part + C blocks)
   qw_diag_kernel<<<grid_for((long long)batch * N * n), 256, 0, s>>>(W, kx, ky, N, batch, V, lam);
   KHX_LAUNCH_CHECK();
@@ -497,6 +515,23 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   KHX_LAUNCH_CHECK();
 
   // ---- build_scatmat
+  if (winv) {
+    // t1 = W^-1 W0 (W0 = I) = diag(1/f) X^-1 conj(Zt); t2 = V^-1 V0 by one solve
+    cd* Zt = eigws;
+    cd* Xs = eigws + (long long)batch * nn;
+    KHX_TRY(zset_identity(s, Mat{Am, n, nn}, n, n, one, batch));
+    KHX_TRY(ztrsm_upper(s, n, n, Mat{Xs, n, nn}, Mat{Am, n, nn}, batch));
+    conj_inplace_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(Zt, (long long)batch * nn);
+    KHX_LAUNCH_CHECK();
+    KHX_TRY(zgemm(s, n, n, n, one, Mat{Am, n, nn}, Mat{Zt, n, nn}, zero, Mat{t1, n, nn}, batch));
+    KHX_TRY(zrow_scale(s, Mat{t1, n, nn}, xdiag, n, Mat{t1, n, nn}, n, n, batch));
+    zero_status_kernel<<<dim3((batch + 255) / 256), 256, 0, s>>>(info_lu1, batch);
+    KHX_LAUNCH_CHECK();
+    KHX_TRY(zcopy(s, Mat{Wc, n, nn}, Mat{V, n, nn}, n, n, batch));
+    fill_v0_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(kx, ky, N, batch, t2, nn, n);
+    KHX_LAUNCH_CHECK();
+    KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{t2, n, nn}, batch, info_lu2));
+  } else {
   // t1 = W^-1 W0 (W0 = I) and t2 = V^-1 V0 as one batch of 2 batch systems
   // (status words: info_lu1 then info_lu2, which follow it)
   KHX_TRY(zcopy(s, Mat{Wc, n, nn}, Mat{W, n, nn}, n, n, batch));
@@ -509,6 +544,7 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   } else {  // the multi-launch LU takes at most 65535 systems per call
     KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{t1, n, nn}, batch, info_lu1));
     KHX_TRY(zgesv(s, n, n, Mat{Wc + (long long)batch * nn, n, nn}, Mat{t2, n, nn}, batch, info_lu2));
+  }
   }
   // A = t1 + t2, B = t1 - t2
   KHX_TRY(zaxpby(s, Mat{Am, n, nn}, one, Mat{t1, n, nn}, one, Mat{t2, n, nn}, n, n, batch));

@@ -772,6 +772,12 @@ __global__ __launch_bounds__(256) void zlaswp_kernel(int n, int m, const int* __
     }
 }
 
+int ztrsm_upper(hipStream_t s, int n, int m, Mat A, Mat B, int batch) {
+  if (batch <= 0 || n <= 0 || m <= 0) return 0;
+  KHX_REQUIRE(batch <= 65535, "ztrsm_upper: at most 65535 systems per call");
+  return trsm_upper_rec<16>(s, n, m, A, B, batch, 0, n);
+}
+
 // B <- A^-1 B from zgetrf factors (any path): interchanges, then the two
 // triangular solves split in halves into GEMMs (trsm_*_rec).
 int zlu_solve_factored(hipStream_t s, int n, int m, Mat A, const int* ipiv, Mat B, int batch) {

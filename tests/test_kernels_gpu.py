@@ -114,14 +114,18 @@ def _layer_inputs(pw, kp, wl):
     return g, kx, ky
 
 
-@pytest.mark.parametrize("ainv", ["1", "0"])
+@pytest.mark.parametrize("env", [{}, {"KHX_LAYER_AINV": "0"}, {"KHX_LAYER_WINV": "0"}],
+                         ids=["default", "AINV=0", "WINV=0"])
 @pytest.mark.parametrize("pw,kp,wl", [((5, 5), (0.0, 0.0), 2.0), ((5, 5), (0.4, -0.3), 1.7),
                                       ((9, 9), (0.2, 0.1), 1.9)])
-def test_patterned_layer(gpu, monkeypatch, pw, kp, wl, ainv):
-    """The layer S-matrix against the oracle's build_scatmat, with [G1 | G2]
-    through A^-1 (default) and through the reference's solves (KHX_LAYER_AINV=0)."""
+def test_patterned_layer(gpu, monkeypatch, pw, kp, wl, env):
+    """The layer S-matrix against the oracle's build_scatmat: by default W^-1
+    from the eigensolver's Schur vectors and triangular eigenvectors and
+    [G1 | G2] through A^-1; also with the reference's solves for either
+    (KHX_LAYER_WINV=0, KHX_LAYER_AINV=0)."""
     from khepri_amd import ops
-    monkeypatch.setenv("KHX_LAYER_AINV", ainv)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
     pix = W.disc_pixmap()
     C = rcwa.convolution_matrix(pix, pw)
     g, kx, ky = _layer_inputs(pw, kp, wl)
