@@ -173,6 +173,14 @@ size_t khx_patterned_layer_ws_bytes(int n, int batch);
 int khx_patterned_layer(const double* C, const double* IC, int per_item, const double* kx, const double* ky,
                         int N, int batch, const double* k0d, double* S, double* W, double* V, double* lam,
                         int* info, void* ws, void* stream);
+/* The same, also keeping the four solves' results for the backward twin
+ * (khx_patterned_layer_bwd_cached): cache (batch, 6, n, n) = W^-1, V^-1 V0,
+ * [G1 | G2] = A^-1 [X | B] (n x 2n), [S11 | S12] (n x 2n).  Same reference
+ * code as khx_patterned_layer; for gradients, where the forward pass's
+ * intermediates are needed again.                                          */
+int khx_patterned_layer_cached(const double* C, const double* IC, int per_item, const double* kx, const double* ky,
+                               int N, int batch, const double* k0d, double* S, double* W, double* V, double* lam,
+                               int* info, double* cache, void* ws, void* stream);
 
 /* Uniform layer of permittivity eps (khepri/layer.py:166-168:
  * khepri/alternative.py:130-156 solve_uniform_layer + build_scatmat) and the
@@ -344,6 +352,12 @@ int khx_patterned_layer_bwd(const double* C, const double* IC, int per_item, con
                             int N, int batch, const double* k0d, const double* W, const double* V,
                             const double* lam, const double* Sbar, double eta_rel, double* Cbar, int* info, void* ws,
                             void* stream);
+/* khx_patterned_layer_bwd with the forward intermediates taken from the cache
+ * written by khx_patterned_layer_cached for the same inputs (no re-solve). */
+int khx_patterned_layer_bwd_cached(const double* C, const double* IC, int per_item, const double* kx,
+                                   const double* ky, int N, int batch, const double* k0d, const double* W,
+                                   const double* V, const double* lam, const double* cache, const double* Sbar,
+                                   double eta_rel, double* Cbar, int* info, void* ws, void* stream);
 size_t khx_redheffer_bwd_ws_bytes(int n, int batch);
 int khx_redheffer_bwd(const double* SA, const double* SB, int n, int batch, const double* Sbar, double* SAbar,
                       double* SBbar, int* info, void* ws, void* stream);

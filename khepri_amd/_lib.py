@@ -30,6 +30,7 @@ SIGNATURES = {
     "khx_eig_noalloc_bytes": (_sz, []),
     "khx_patterned_layer_ws_bytes": (_sz, [_i, _i]),
     "khx_patterned_layer": (_i, [_dp, _dp, _i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _ip, _vp, _vp]),
+    "khx_patterned_layer_cached": (_i, [_dp, _dp, _i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _vp, _vp]),
     "khx_uniform_layer": (_i, [_dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "khx_halfspace": (_i, [_i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _vp]),
     "khx_redheffer_ws_bytes": (_sz, [_i, _i]),
@@ -70,6 +71,8 @@ SIGNATURES = {
     "khx_patterned_layer_bwd_ws_bytes": (_sz, [_i, _i]),
     "khx_patterned_layer_bwd": (_i, [_dp, _dp, _i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _ip,
                                      _vp, _vp]),
+    "khx_patterned_layer_bwd_cached": (_i, [_dp, _dp, _i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                            _dp, _ip, _vp, _vp]),
     "khx_redheffer_bwd_ws_bytes": (_sz, [_i, _i]),
     "khx_redheffer_bwd": (_i, [_dp, _dp, _i, _i, _dp, _dp, _dp, _ip, _vp, _vp]),
     "khx_beam_fields": (_i, [_i, _vp, _vp, _vp, C.c_longlong, C.c_double, C.c_double, C.c_double, _i, _i, _vp, _vp,

@@ -143,10 +143,11 @@ def _layers_forward(cl, names, kx, ky, k0):
     if len(pat) >= 2 and B * len(pat) <= ops.eig_slots(2 * N) and os.environ.get("KHX_FUSE_LAYERS", "1") == "1":
         mats = [cl.layers[nm].convolution_matrices(kx.device) for nm in pat]
         k0ds = [(k0 * cl.layers[nm].depth).contiguous() for nm in pat]
-        res = ops.patterned_layers_fused([m[0] for m in mats], [m[1] for m in mats], kx, ky, k0ds, modes=True)
-        for nm, (C, IC), k0d, (S, (W, V, lam), info) in zip(pat, mats, k0ds, res):
+        res = ops.patterned_layers_fused([m[0] for m in mats], [m[1] for m in mats], kx, ky, k0ds, modes=True,
+                                         cache=True)
+        for nm, (C, IC), k0d, (S, (W, V, lam), info, cache) in zip(pat, mats, k0ds, res):
             ops.raise_on_info(info, "patterned layer")
-            out[nm] = ("dense", S, dict(C=C, IC=IC, W=W, V=V, lam=lam, k0d=k0d))
+            out[nm] = ("dense", S, dict(C=C, IC=IC, W=W, V=V, lam=lam, k0d=k0d, cache=cache))
     for nm in names:
         if nm not in out:
             out[nm] = _layer_forward(cl.layers[nm], kx, ky, k0)
@@ -159,9 +160,11 @@ def _layer_forward(layer, kx, ky, k0):
     f = layer.formulation
     if f in (Formulation.FFT, Formulation.ANALYTICAL):
         C, IC = layer.convolution_matrices(kx.device)
-        S, (W, V, lam), info = ops.patterned_layer(C, IC, kx, ky, k0d, modes=True)
+        B, N = kx.shape
+        cache = torch.empty((B, 6, 2 * N, 2 * N), dtype=ops.CDT, device=kx.device)
+        S, (W, V, lam), info = ops.patterned_layer(C, IC, kx, ky, k0d, modes=True, cache=cache)
         ops.raise_on_info(info, "patterned layer")
-        return ("dense", S, dict(C=C, IC=IC, W=W, V=V, lam=lam, k0d=k0d))
+        return ("dense", S, dict(C=C, IC=IC, W=W, V=V, lam=lam, k0d=k0d, cache=cache))
     if f in (Formulation.HALF_SPACE_INC, Formulation.HALF_SPACE_TRN):
         if _gap(layer):
             return ("identity", None, None)
@@ -222,7 +225,8 @@ def _gradient_chunk(cl, names, wrt, g, N, kp, wl, te, tm, wR, wT, eta_rel, reduc
         Cbar_all, info = ops.patterned_layer_bwd(
             Cf, ICf, kx.repeat(L, 1), ky.repeat(L, 1), torch.cat([ex["k0d"].reshape(Bc) for ex in exs]),
             torch.cat([ex["W"] for ex in exs]), torch.cat([ex["V"] for ex in exs]),
-            torch.cat([ex["lam"] for ex in exs]), torch.cat([Sbar[nm] for nm in live]), eta_rel)
+            torch.cat([ex["lam"] for ex in exs]), torch.cat([Sbar[nm] for nm in live]), eta_rel,
+            cache=torch.cat([ex["cache"] for ex in exs]))
         ops.raise_on_info(info, "layer adjoint")
         for l, nm in enumerate(live):
             Cbars[nm] = Cbar_all[l * Bc:(l + 1) * Bc]
@@ -237,7 +241,7 @@ def _gradient_chunk(cl, names, wrt, g, N, kp, wl, te, tm, wR, wT, eta_rel, reduc
             Cbar = Cbars[nm]
         else:
             Cbar, info = ops.patterned_layer_bwd(ex["C"], ex["IC"], kx, ky, ex["k0d"], ex["W"], ex["V"], ex["lam"],
-                                                 Sbar[nm], eta_rel)
+                                                 Sbar[nm], eta_rel, cache=ex["cache"])
             ops.raise_on_info(info, "layer adjoint")
         eps = np.asarray(cl.layers[nm].epsilon)
         if reduce:

@@ -36,7 +36,7 @@ int ensure_dynamic_lds(const void* fn, size_t bytes) {
 size_t patterned_layer_bwd_ws_bytes(int n, int batch);
 int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstride, const cd* kx, const cd* ky,
                         int N, int batch, const double* k0d, const cd* W, const cd* V, const cd* lam, const cd* Sbar,
-                        double eta_rel, cd* Cbar, int* info, void* ws);
+                        double eta_rel, cd* Cbar, int* info, void* ws, const cd* cache);
 size_t star_bwd_ws_bytes(int n, int batch);
 int star_bwd(hipStream_t s, const cd* SA, long long sA, const cd* SB, long long sB, int n, int batch, const cd* Sbar,
              cd* SAbar, cd* SBbar, int* info, void* ws);
@@ -50,7 +50,8 @@ int kvectors(hipStream_t s, const double* g, int N, const cd* kp, const double* 
 int convmat(hipStream_t s, const double* eps, int cplx, int nx, int ny, int P, int Q, int count, cd* C, cd* Fws);
 size_t patterned_layer_ws_bytes(int n, int batch);
 int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride, const cd* kx, const cd* ky, int N,
-                    int batch, const double* k0d, cd* S, cd* Wout, cd* Vout, cd* lamout, int* info, void* ws);
+                    int batch, const double* k0d, cd* S, cd* Wout, cd* Vout, cd* lamout, int* info, void* ws,
+                    cd* cache);
 int analytic_layer(hipStream_t s, int kind, const cd* kx, const cd* ky, int N, int batch, cd eps,
                    const double* k0d, cd* S, cd* W, cd* V, cd* lam);
 size_t star_ws_bytes(int n, int batch);
@@ -224,7 +225,17 @@ int khx_patterned_layer(const double* C, const double* IC, int per_item, const d
   KHX_ENTRY_BEGIN
   KHX_REQUIRE(N > 0 && batch >= 0, "khx_patterned_layer: bad sizes");
   return khx::patterned_layer(as_stream(stream), as_cd(C), as_cd(IC), per_item ? (long long)N * N : 0, as_cd(kx),
-                              as_cd(ky), N, batch, k0d, as_cd(S), as_cd(W), as_cd(V), as_cd(lam), info, ws);
+                              as_cd(ky), N, batch, k0d, as_cd(S), as_cd(W), as_cd(V), as_cd(lam), info, ws, nullptr);
+}
+
+int khx_patterned_layer_cached(const double* C, const double* IC, int per_item, const double* kx, const double* ky,
+                               int N, int batch, const double* k0d, double* S, double* W, double* V, double* lam,
+                               int* info, double* cache, void* ws, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0 && cache, "khx_patterned_layer_cached: bad arguments");
+  return khx::patterned_layer(as_stream(stream), as_cd(C), as_cd(IC), per_item ? (long long)N * N : 0, as_cd(kx),
+                              as_cd(ky), N, batch, k0d, as_cd(S), as_cd(W), as_cd(V), as_cd(lam), info, ws,
+                              as_cd(cache));
 }
 
 int khx_uniform_layer(const double* kx, const double* ky, int N, int batch, const double* eps_host,
@@ -370,7 +381,20 @@ int khx_patterned_layer_bwd(const double* C, const double* IC, int per_item, con
   KHX_REQUIRE(batch <= 65535, "khx_patterned_layer_bwd: at most 65535 items per call");
   return khx::patterned_layer_bwd(as_stream(stream), as_cd(C), as_cd(IC), per_item ? (long long)N * N : 0, as_cd(kx),
                                   as_cd(ky), N, batch, k0d, as_cd(W), as_cd(V), as_cd(lam), as_cd(Sbar), eta_rel,
-                                  as_cd(Cbar), info, ws);
+                                  as_cd(Cbar), info, ws, nullptr);
+}
+
+int khx_patterned_layer_bwd_cached(const double* C, const double* IC, int per_item, const double* kx,
+                                   const double* ky, int N, int batch, const double* k0d, const double* W,
+                                   const double* V, const double* lam, const double* cache, const double* Sbar,
+                                   double eta_rel, double* Cbar, int* info, void* ws, void* stream) {
+  KHX_ENTRY_BEGIN
+  KHX_REQUIRE(N > 0 && batch >= 0 && ws && W && V && lam && cache && Sbar && Cbar && eta_rel >= 0.0,
+              "khx_patterned_layer_bwd_cached: bad arguments");
+  KHX_REQUIRE(batch <= 65535, "khx_patterned_layer_bwd_cached: at most 65535 items per call");
+  return khx::patterned_layer_bwd(as_stream(stream), as_cd(C), as_cd(IC), per_item ? (long long)N * N : 0, as_cd(kx),
+                                  as_cd(ky), N, batch, k0d, as_cd(W), as_cd(V), as_cd(lam), as_cd(Sbar), eta_rel,
+                                  as_cd(Cbar), info, ws, as_cd(cache));
 }
 
 size_t khx_redheffer_bwd_ws_bytes(int n, int batch) { return khx::star_bwd_ws_bytes(n, batch); }

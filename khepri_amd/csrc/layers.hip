@@ -450,7 +450,7 @@ size_t patterned_layer_ws_bytes(int n, int batch) {
 //   khepri/alternative.py:181-195 (build_scatmat) with W0 = I.
 int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride, const cd* kx, const cd* ky, int N,
                     int batch, const double* k0d, cd* S, cd* Wout, cd* Vout, cd* lamout, int* info,
-                    void* wsv) {
+                    void* wsv, cd* cache) {
   if (batch <= 0) return 0;
   const int n = 2 * N;
   const long long nn = (long long)n * n;
@@ -583,6 +583,13 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   KHX_TRY(zgesv(s, n, 2 * n, Mat{T, n, nn}, Mat{R2, 2 * n, 2 * nn}, batch, info_lu4));
   scatter_sym_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(R2, 2 * nn, n, batch, S);
   KHX_LAUNCH_CHECK();
+  if (cache) {  // t1, t2, [G1 | G2], [S11 | S12] for patterned_layer_bwd (6 n^2 per item)
+    const long long sc = 6 * nn;
+    KHX_TRY(zcopy(s, Mat{cache, n, sc}, Mat{t1, n, nn}, n, n, batch));
+    KHX_TRY(zcopy(s, Mat{cache + nn, n, sc}, Mat{t2, n, nn}, n, n, batch));
+    KHX_TRY(zcopy(s, Mat{cache + 2 * nn, 2 * n, sc}, Mat{R1, 2 * n, 2 * nn}, n, 2 * n, batch));
+    KHX_TRY(zcopy(s, Mat{cache + 4 * nn, 2 * n, sc}, Mat{R2, 2 * n, 2 * nn}, n, 2 * n, batch));
+  }
   if (info) {
     info_combine_kernel<<<dim3((batch + 255) / 256), 256, 0, s>>>(infos, batch, 5, info);
     KHX_LAUNCH_CHECK();
@@ -756,7 +763,7 @@ size_t patterned_layer_bwd_ws_bytes(int n, int batch) {
 //   P, Q from C and IC = C^-1 (khepri/alternative.py:158-195).
 int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstride, const cd* kx, const cd* ky,
                         int N, int batch, const double* k0d, const cd* W, const cd* V, const cd* lam, const cd* Sbar,
-                        double eta_rel, cd* Cbar, int* info, void* wsv) {
+                        double eta_rel, cd* Cbar, int* info, void* wsv, const cd* cache) {
   if (batch <= 0) return 0;
   const int n = 2 * N;
   const long long nn = (long long)n * n, n2 = 2 * nn, NN = (long long)N * N;
@@ -822,37 +829,55 @@ int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstr
   auto sbar = [&](int q) { return Mat{const_cast<cd*>(Sbar) + q * nn, n, 4 * nn}; };
   const long long nb = (long long)batch * n;
 
-  // ---- recompute the forward intermediates
+  // ---- recompute the forward intermediates (the four solves come from the
+  // forward pass's cache when it kept one: t1, t2, [G1 | G2], [S11 | S12])
   KHX_TRY(build_pq(s, C, IC, cstride, kx, ky, N, batch, P, Q));
   KHX_TRY(zgemm(s, n, n, n, one, mQ, mW, zero, mU, batch));
   mode_vectors_kernel<<<grid_for(nb), 256, 0, s>>>(lam, k0d, n, batch, xc, ilc, mu);
   KHX_LAUNCH_CHECK();
-  KHX_TRY(zcopy(s, mWc, mW, n, n, batch));
-  KHX_TRY(zset_identity(s, mt1, n, n, one, batch));
-  KHX_TRY(zgesv(s, n, n, mWc, mt1, batch, infos));
-  KHX_TRY(zcopy(s, mWc, mV, n, n, batch));
-  fill_v0_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(kx, ky, N, batch, t2, nn, n);
-  KHX_LAUNCH_CHECK();
-  KHX_TRY(zgesv(s, n, n, mWc, mt2, batch, infos + batch));
+  const long long sc = 6 * nn;
+  if (cache) {
+    KHX_TRY(zcopy(s, mt1, Mat{const_cast<cd*>(cache), n, sc}, n, n, batch));
+    KHX_TRY(zcopy(s, mt2, Mat{const_cast<cd*>(cache) + nn, n, sc}, n, n, batch));
+    zero_status_kernel<<<dim3((4 * batch + 255) / 256), 256, 0, s>>>(infos, 4 * batch);
+    KHX_LAUNCH_CHECK();
+  } else {
+    KHX_TRY(zcopy(s, mWc, mW, n, n, batch));
+    KHX_TRY(zset_identity(s, mt1, n, n, one, batch));
+    KHX_TRY(zgesv(s, n, n, mWc, mt1, batch, infos));
+    KHX_TRY(zcopy(s, mWc, mV, n, n, batch));
+    fill_v0_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(kx, ky, N, batch, t2, nn, n);
+    KHX_LAUNCH_CHECK();
+    KHX_TRY(zgesv(s, n, n, mWc, mt2, batch, infos + batch));
+  }
   KHX_TRY(zaxpby(s, mA, one, mt1, one, mt2, n, n, batch));
   KHX_TRY(zaxpby(s, mB, one, mt1, mone, mt2, n, n, batch));
   KHX_TRY(conj_transpose(s, mA, n, n, mAH, batch));
+  // x (and, without a cache, R1 = [X | B])
   x_rhs_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(lam, k0d, n, batch, Bm, nn, R1, n2, x);
   KHX_LAUNCH_CHECK();
-  KHX_TRY(zcopy(s, mWc, mA, n, n, batch));
-  KHX_TRY(zgesv(s, n, 2 * n, mWc, mR1, batch, infos + 2 * batch));
+  if (cache) {
+    KHX_TRY(zcopy(s, mR1, Mat{const_cast<cd*>(cache) + 2 * nn, 2 * n, sc}, n, 2 * n, batch));
+  } else {
+    KHX_TRY(zcopy(s, mWc, mA, n, n, batch));
+    KHX_TRY(zgesv(s, n, 2 * n, mWc, mR1, batch, infos + 2 * batch));
+  }
   KHX_TRY(zgemm(s, n, n, n, one, mB, mG1, zero, mN1, batch));
   KHX_TRY(zrow_scale(s, mM, x, n, mN1, n, n, batch));
   KHX_TRY(zcopy(s, mT, mA, n, n, batch));
   KHX_TRY(zgemm(s, n, n, n, mone, mM, mB, one, mT, batch));
-  KHX_TRY(zaxpby(s, mR2L, mone, mB, zero, Mat{nullptr, 0, 0}, n, n, batch));
-  KHX_TRY(zgemm(s, n, n, n, one, mM, mA, one, mR2L, batch));
   KHX_TRY(zcopy(s, mH, mA, n, n, batch));
   KHX_TRY(zgemm(s, n, n, n, mone, mB, mG2, one, mH, batch));
-  KHX_TRY(zrow_scale(s, mR2R, x, n, mH, n, n, batch));
   KHX_TRY(conj_transpose(s, mT, n, n, mTH, batch));
-  KHX_TRY(zcopy(s, mWc, mT, n, n, batch));
-  KHX_TRY(zgesv(s, n, 2 * n, mWc, mR2, batch, infos + 3 * batch));
+  if (cache) {
+    KHX_TRY(zcopy(s, mR2, Mat{const_cast<cd*>(cache) + 4 * nn, 2 * n, sc}, n, 2 * n, batch));
+  } else {
+    KHX_TRY(zaxpby(s, mR2L, mone, mB, zero, Mat{nullptr, 0, 0}, n, n, batch));
+    KHX_TRY(zgemm(s, n, n, n, one, mM, mA, one, mR2L, batch));
+    KHX_TRY(zrow_scale(s, mR2R, x, n, mH, n, n, batch));
+    KHX_TRY(zcopy(s, mWc, mT, n, n, batch));
+    KHX_TRY(zgesv(s, n, 2 * n, mWc, mR2, batch, infos + 3 * batch));
+  }
 
   // ---- reverse sweep
   // (1) [K1 | K2] = T^-H [S11bar + S22bar | S12bar + S21bar]

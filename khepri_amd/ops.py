@@ -310,9 +310,11 @@ def eig(A, overwrite_a=False):
     return lam, W, info
 
 
-def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None):
+def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None, cache=None):
     """S (B, 2, 2, n, n) of a patterned layer; (W, V, lam) too when modes.
-    C, IC: (N, N) shared by the batch or (B, N, N) one per item."""
+    C, IC: (N, N) shared by the batch or (B, N, N) one per item.  cache: a
+    (B, 6, n, n) complex tensor that receives the solves' results for
+    patterned_layer_bwd (khx_patterned_layer_cached)."""
     lib = _lib.load()
     B, N = kx.shape
     n = 2 * N
@@ -331,6 +333,14 @@ def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None):
     C, IC = C.contiguous(), IC.contiguous()
     if per_item and (C.shape[0] != B or IC.shape[0] != B):
         raise ValueError("per-item convolution matrices must match the batch")
+    if cache is not None:
+        _require(cache, CDT, "cache")
+        if tuple(cache.shape) != (B, 6, n, n) or not cache.is_contiguous():
+            raise ValueError(f"cache must be a contiguous ({B}, 6, {n}, {n}) complex tensor")
+        check(lib.khx_patterned_layer_cached(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(S), _p(W),
+                                             _p(V), _p(lam), _p(info), _p(cache), _p(ws), _stream()),
+              "khx_patterned_layer_cached")
+        return S, (W, V, lam), info
     check(lib.khx_patterned_layer(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(S), _p(W), _p(V),
                                   _p(lam), _p(info), _p(ws), _stream()), "khx_patterned_layer")
     return S, (W, V, lam), info
@@ -368,24 +378,28 @@ def convmat_islands(islands, eps_host, pw, reciprocal, sigma, device):
     return C
 
 
-def patterned_layers_fused(Cs, ICs, kx, ky, k0ds, modes=False):
+def patterned_layers_fused(Cs, ICs, kx, ky, k0ds, modes=False, cache=False):
     """patterned_layer for several layers over the same sources in one call:
     item l*B + b is layer l at source b, so the eigen-solver and every other
     stage runs once over L*B items (a small batch of one layer leaves most CUs
-    idle).  Returns one (S, (W, V, lam), info) per layer, as views."""
+    idle).  Returns one (S, (W, V, lam), info) per layer, as views; with
+    cache, one (S, (W, V, lam), info, cache) per layer (cache: the
+    patterned_layer_bwd intermediates, (B, 6, n, n))."""
     L = len(Cs)
     B, N = kx.shape
+    n = 2 * N
     C = torch.stack([c.expand(B, N, N) for c in Cs]).reshape(L * B, N, N)
     IC = torch.stack([c.expand(B, N, N) for c in ICs]).reshape(L * B, N, N)
     kxl = kx.repeat(L, 1)
     kyl = ky.repeat(L, 1)
     k0dl = torch.cat([k.reshape(B) for k in k0ds])
-    S, (W, V, lam), info = patterned_layer(C, IC, kxl, kyl, k0dl, modes=modes)
+    cb = torch.empty((L * B, 6, n, n), dtype=CDT, device=kx.device) if cache else None
+    S, (W, V, lam), info = patterned_layer(C, IC, kxl, kyl, k0dl, modes=modes, cache=cb)
     out = []
     for l in range(L):
         sl = slice(l * B, (l + 1) * B)
-        out.append((S[sl], (W[sl] if modes else None, V[sl] if modes else None, lam[sl] if modes else None),
-                    info[sl]))
+        item = (S[sl], (W[sl] if modes else None, V[sl] if modes else None, lam[sl] if modes else None), info[sl])
+        out.append(item + (cb[sl],) if cache else item)
     return out
 
 
@@ -430,9 +444,11 @@ def redheffer(SA, SB, out=None, ws=None):
     return S, info
 
 
-def patterned_layer_bwd(C, IC, kx, ky, k0d, W, V, lam, Sbar, eta_rel=1e-12):
+def patterned_layer_bwd(C, IC, kx, ky, k0d, W, V, lam, Sbar, eta_rel=1e-12, cache=None):
     """Cbar (B, N, N): adjoint of the convolution matrix of a patterned layer
-    given the adjoint Sbar (B, 2, 2, n, n) of its S-matrix and its modes."""
+    given the adjoint Sbar (B, 2, 2, n, n) of its S-matrix and its modes;
+    cache: the forward pass's intermediates (patterned_layer(..., cache=)),
+    which spare the backward pass four solves."""
     lib = _lib.load()
     B, N = kx.shape
     n = 2 * N
@@ -443,6 +459,15 @@ def patterned_layer_bwd(C, IC, kx, ky, k0d, W, V, lam, Sbar, eta_rel=1e-12):
     Cbar = torch.empty((B, N, N), dtype=CDT, device=dev)
     info = torch.empty(B, dtype=torch.int32, device=dev)
     ws = _ws(lib.khx_patterned_layer_bwd_ws_bytes(n, B), dev)
+    if cache is not None:
+        cache = cache.contiguous()
+        if tuple(cache.shape) != (B, 6, n, n):
+            raise ValueError(f"cache must be ({B}, 6, {n}, {n})")
+        check(lib.khx_patterned_layer_bwd_cached(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(W),
+                                                 _p(V), _p(lam), _p(cache), _p(Sbar), float(eta_rel), _p(Cbar),
+                                                 _p(info), _p(ws), _stream()),
+              "khx_patterned_layer_bwd_cached")
+        return Cbar, info
     check(lib.khx_patterned_layer_bwd(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(W),
                                       _p(V), _p(lam), _p(Sbar),
                                       float(eta_rel), _p(Cbar), _p(info), _p(ws), _stream()),

@@ -124,6 +124,15 @@ def test_patterned_layer_bwd_matches_autograd(gpu, pw, theta):
     assert (info.cpu().numpy() == 0).all()
     got = ops.convmat_adjoint(Cbar, pw, shape).cpu().numpy()[0]
     np.testing.assert_allclose(got, ref, atol=1e-9 * np.abs(ref).max())
+    # the same with the forward pass's intermediates kept (khx_patterned_layer_cached /
+    # khx_patterned_layer_bwd_cached): S as before, the gradient to the same tolerance
+    cache = torch.empty((1, 6, n, n), dtype=ops.CDT, device=gpu)
+    Sc, (Wc, Vc, lamc), info = ops.patterned_layer(C, IC[0], kxd, kyd, k0d, modes=True, cache=cache)
+    np.testing.assert_array_equal(Sc.cpu().numpy(), Sg.cpu().numpy())
+    Cbarc, info = ops.patterned_layer_bwd(C, IC[0], kxd, kyd, k0d, Wc, Vc, lamc, dev(Sbar, gpu), cache=cache)
+    assert (info.cpu().numpy() == 0).all()
+    gotc = ops.convmat_adjoint(Cbarc, pw, shape).cpu().numpy()[0]
+    np.testing.assert_allclose(gotc, ref, atol=1e-9 * np.abs(ref).max())
 
 
 def _crystal(pw, pixmaps, depths, gap=None, device="cuda:0"):
