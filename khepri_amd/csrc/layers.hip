@@ -561,10 +561,16 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
     // behind KHX_LAYER_AINV=0
     KHX_TRY(zset_identity(s, Mat{R1, 2 * n, 2 * nn}, n, n, one, batch));
     KHX_TRY(zgesv(s, n, n, Mat{Ac, n, nn}, Mat{R1, 2 * n, 2 * nn}, batch, info_lu3));
+    if (cache) KHX_TRY(zcopy(s, Mat{cache + 6 * nn, n, 7 * nn}, Mat{R1, 2 * n, 2 * nn}, n, n, batch));
     KHX_TRY(zgemm(s, n, n, n, one, Mat{R1, 2 * n, 2 * nn}, Mat{Bm, n, nn}, zero, Mat{R1 + n, 2 * n, 2 * nn}, batch));
     KHX_TRY(zdiag_scale(s, Mat{R1, 2 * n, 2 * nn}, nullptr, xdiag, n, n, batch, Mat{R1, 2 * n, 2 * nn}));
   } else {
     KHX_TRY(zgesv(s, n, 2 * n, Mat{Ac, n, nn}, Mat{R1, 2 * n, 2 * nn}, batch, info_lu3));
+    if (cache) {  // A^-1 for the backward pass
+      KHX_TRY(zcopy(s, Mat{Ac, n, nn}, Mat{Am, n, nn}, n, n, batch));
+      KHX_TRY(zset_identity(s, Mat{cache + 6 * nn, n, 7 * nn}, n, n, one, batch));
+      KHX_TRY(zgesv(s, n, n, Mat{Ac, n, nn}, Mat{cache + 6 * nn, n, 7 * nn}, batch, info_lu3));
+    }
   }
   // M = X B G1
   KHX_TRY(zrow_scale(s, Mat{XB, n, nn}, xdiag, n, Mat{Bm, n, nn}, n, n, batch));
@@ -583,8 +589,8 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   KHX_TRY(zgesv(s, n, 2 * n, Mat{T, n, nn}, Mat{R2, 2 * n, 2 * nn}, batch, info_lu4));
   scatter_sym_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(R2, 2 * nn, n, batch, S);
   KHX_LAUNCH_CHECK();
-  if (cache) {  // t1, t2, [G1 | G2], [S11 | S12] for patterned_layer_bwd (6 n^2 per item)
-    const long long sc = 6 * nn;
+  if (cache) {  // t1, t2, [G1 | G2], [S11 | S12], A^-1 for patterned_layer_bwd (7 n^2 per item)
+    const long long sc = 7 * nn;
     KHX_TRY(zcopy(s, Mat{cache, n, sc}, Mat{t1, n, nn}, n, n, batch));
     KHX_TRY(zcopy(s, Mat{cache + nn, n, sc}, Mat{t2, n, nn}, n, n, batch));
     KHX_TRY(zcopy(s, Mat{cache + 2 * nn, 2 * n, sc}, Mat{R1, 2 * n, 2 * nn}, n, 2 * n, batch));
@@ -835,7 +841,7 @@ int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstr
   KHX_TRY(zgemm(s, n, n, n, one, mQ, mW, zero, mU, batch));
   mode_vectors_kernel<<<grid_for(nb), 256, 0, s>>>(lam, k0d, n, batch, xc, ilc, mu);
   KHX_LAUNCH_CHECK();
-  const long long sc = 6 * nn;
+  const long long sc = 7 * nn;
   if (cache) {
     KHX_TRY(zcopy(s, mt1, Mat{const_cast<cd*>(cache), n, sc}, n, n, batch));
     KHX_TRY(zcopy(s, mt2, Mat{const_cast<cd*>(cache) + nn, n, sc}, n, n, batch));
@@ -852,7 +858,7 @@ int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstr
   }
   KHX_TRY(zaxpby(s, mA, one, mt1, one, mt2, n, n, batch));
   KHX_TRY(zaxpby(s, mB, one, mt1, mone, mt2, n, n, batch));
-  KHX_TRY(conj_transpose(s, mA, n, n, mAH, batch));
+  if (!cache) KHX_TRY(conj_transpose(s, mA, n, n, mAH, batch));
   // x (and, without a cache, R1 = [X | B])
   x_rhs_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(lam, k0d, n, batch, Bm, nn, R1, n2, x);
   KHX_LAUNCH_CHECK();
@@ -905,8 +911,16 @@ int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstr
   KHX_TRY(zrow_scale(s, mN1b, xc, n, mMb, n, n, batch));
   KHX_TRY(zgemm_op(s, 0, 2, n, n, n, one, mN1b, mG1, one, mBb, batch));
   KHX_TRY(zgemm_op(s, 2, 0, n, n, n, one, mB, mN1b, zero, mGJ, batch));
-  // (7) [G1 | G2] = A^-1 [X | B]
-  KHX_TRY(zgesv(s, n, 2 * n, mAH, mGJ, batch, infos + 5 * batch));
+  // (7) [G1 | G2] = A^-1 [X | B]: [J1 | J2] = A^-H [G1bar | G2bar] (with the
+  // cached A^-1 one GEMM, through RB, which is no longer read)
+  if (cache) {
+    KHX_TRY(zgemm_op(s, 2, 0, n, 2 * n, n, one, Mat{const_cast<cd*>(cache) + 6 * nn, n, sc}, mGJ, zero, mRB, batch));
+    KHX_TRY(zcopy(s, mGJ, mRB, n, 2 * n, batch));
+    zero_status_kernel<<<dim3((batch + 255) / 256), 256, 0, s>>>(infos + 5 * batch, batch);
+    KHX_LAUNCH_CHECK();
+  } else {
+    KHX_TRY(zgesv(s, n, 2 * n, mAH, mGJ, batch, infos + 5 * batch));
+  }
   KHX_TRY(zgemm_op(s, 0, 2, n, n, 2 * n, mone, mGJ, mR1, one, mAb, batch));
   diag_accum_kernel<<<grid_for(nb), 256, 0, s>>>(GJ, 2 * n, n2, n, batch, xbar);
   KHX_LAUNCH_CHECK();

@@ -90,12 +90,16 @@ def test_redheffer_bwd_matches_autograd(gpu):
         np.testing.assert_allclose(SBb[b].cpu().numpy(), tb.grad.numpy(), atol=1e-10 * np.abs(tb.grad.numpy()).max())
 
 
+@pytest.mark.parametrize("ainv", ["1", "0"])
 @pytest.mark.parametrize("pw,theta", [((3, 3), 0.0), ((5, 5), 23.0)])
-def test_patterned_layer_bwd_matches_autograd(gpu, pw, theta):
+def test_patterned_layer_bwd_matches_autograd(gpu, monkeypatch, pw, theta, ainv):
     """L = Re <Sbar, S(eps)> for a random Sbar: the pixel gradient of the
-    layer adjoint (eigen adjoint included) against torch CPU autograd."""
+    layer adjoint (eigen adjoint included) against torch CPU autograd, with
+    and without the forward pass's cache, for both ways the forward forms
+    A^-1 [X | B] (KHX_LAYER_AINV)."""
     from oracle import rcwa
     from khepri_amd import ops
+    monkeypatch.setenv("KHX_LAYER_AINV", ainv)
     rng = np.random.default_rng(pw[0] + int(theta))
     shape = (20, 20)
     eps = rng.uniform(2, 12, size=shape)
@@ -126,7 +130,7 @@ def test_patterned_layer_bwd_matches_autograd(gpu, pw, theta):
     np.testing.assert_allclose(got, ref, atol=1e-9 * np.abs(ref).max())
     # the same with the forward pass's intermediates kept (khx_patterned_layer_cached /
     # khx_patterned_layer_bwd_cached): S as before, the gradient to the same tolerance
-    cache = torch.empty((1, 6, n, n), dtype=ops.CDT, device=gpu)
+    cache = torch.empty((1, 7, n, n), dtype=ops.CDT, device=gpu)
     Sc, (Wc, Vc, lamc), info = ops.patterned_layer(C, IC[0], kxd, kyd, k0d, modes=True, cache=cache)
     np.testing.assert_array_equal(Sc.cpu().numpy(), Sg.cpu().numpy())
     Cbarc, info = ops.patterned_layer_bwd(C, IC[0], kxd, kyd, k0d, Wc, Vc, lamc, dev(Sbar, gpu), cache=cache)
