@@ -561,15 +561,15 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
     // behind KHX_LAYER_AINV=0
     KHX_TRY(zset_identity(s, Mat{R1, 2 * n, 2 * nn}, n, n, one, batch));
     KHX_TRY(zgesv(s, n, n, Mat{Ac, n, nn}, Mat{R1, 2 * n, 2 * nn}, batch, info_lu3));
-    if (cache) KHX_TRY(zcopy(s, Mat{cache + 6 * nn, n, 7 * nn}, Mat{R1, 2 * n, 2 * nn}, n, n, batch));
+    if (cache) KHX_TRY(zcopy(s, Mat{cache + 6 * nn, n, 8 * nn}, Mat{R1, 2 * n, 2 * nn}, n, n, batch));
     KHX_TRY(zgemm(s, n, n, n, one, Mat{R1, 2 * n, 2 * nn}, Mat{Bm, n, nn}, zero, Mat{R1 + n, 2 * n, 2 * nn}, batch));
     KHX_TRY(zdiag_scale(s, Mat{R1, 2 * n, 2 * nn}, nullptr, xdiag, n, n, batch, Mat{R1, 2 * n, 2 * nn}));
   } else {
     KHX_TRY(zgesv(s, n, 2 * n, Mat{Ac, n, nn}, Mat{R1, 2 * n, 2 * nn}, batch, info_lu3));
     if (cache) {  // A^-1 for the backward pass
       KHX_TRY(zcopy(s, Mat{Ac, n, nn}, Mat{Am, n, nn}, n, n, batch));
-      KHX_TRY(zset_identity(s, Mat{cache + 6 * nn, n, 7 * nn}, n, n, one, batch));
-      KHX_TRY(zgesv(s, n, n, Mat{Ac, n, nn}, Mat{cache + 6 * nn, n, 7 * nn}, batch, info_lu3));
+      KHX_TRY(zset_identity(s, Mat{cache + 6 * nn, n, 8 * nn}, n, n, one, batch));
+      KHX_TRY(zgesv(s, n, n, Mat{Ac, n, nn}, Mat{cache + 6 * nn, n, 8 * nn}, batch, info_lu3));
     }
   }
   // M = X B G1
@@ -585,17 +585,24 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   KHX_TRY(zcopy(s, Mat{Ac, n, nn}, Mat{Am, n, nn}, n, n, batch));
   KHX_TRY(zgemm(s, n, n, n, mone, Mat{Bm, n, nn}, Mat{R1 + n, 2 * n, 2 * nn}, one, Mat{Ac, n, nn}, batch));
   KHX_TRY(zrow_scale(s, Mat{R2 + n, 2 * n, 2 * nn}, xdiag, n, Mat{Ac, n, nn}, n, n, batch));
-  // [S11 | S12] = T^-1 R2
-  KHX_TRY(zgesv(s, n, 2 * n, Mat{T, n, nn}, Mat{R2, 2 * n, 2 * nn}, batch, info_lu4));
-  scatter_sym_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(R2, 2 * nn, n, batch, S);
-  KHX_LAUNCH_CHECK();
-  if (cache) {  // t1, t2, [G1 | G2], [S11 | S12], A^-1 for patterned_layer_bwd (7 n^2 per item)
-    const long long sc = 7 * nn;
+  // [S11 | S12] = T^-1 R2; with a cache (gradients) through T^-1, which is kept
+  // for the backward pass: one solve with n right-hand sides and a GEMM
+  const long long sc = 8 * nn;
+  if (cache) {
     KHX_TRY(zcopy(s, Mat{cache, n, sc}, Mat{t1, n, nn}, n, n, batch));
     KHX_TRY(zcopy(s, Mat{cache + nn, n, sc}, Mat{t2, n, nn}, n, n, batch));
     KHX_TRY(zcopy(s, Mat{cache + 2 * nn, 2 * n, sc}, Mat{R1, 2 * n, 2 * nn}, n, 2 * n, batch));
+    KHX_TRY(zset_identity(s, Mat{cache + 7 * nn, n, sc}, n, n, one, batch));
+    KHX_TRY(zgesv(s, n, n, Mat{T, n, nn}, Mat{cache + 7 * nn, n, sc}, batch, info_lu4));
+    KHX_TRY(zcopy(s, Mat{R1, 2 * n, 2 * nn}, Mat{R2, 2 * n, 2 * nn}, n, 2 * n, batch));
+    KHX_TRY(zgemm(s, n, 2 * n, n, one, Mat{cache + 7 * nn, n, sc}, Mat{R1, 2 * n, 2 * nn}, zero,
+                  Mat{R2, 2 * n, 2 * nn}, batch));
     KHX_TRY(zcopy(s, Mat{cache + 4 * nn, 2 * n, sc}, Mat{R2, 2 * n, 2 * nn}, n, 2 * n, batch));
+  } else {
+    KHX_TRY(zgesv(s, n, 2 * n, Mat{T, n, nn}, Mat{R2, 2 * n, 2 * nn}, batch, info_lu4));
   }
+  scatter_sym_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(R2, 2 * nn, n, batch, S);
+  KHX_LAUNCH_CHECK();
   if (info) {
     info_combine_kernel<<<dim3((batch + 255) / 256), 256, 0, s>>>(infos, batch, 5, info);
     KHX_LAUNCH_CHECK();
@@ -841,7 +848,7 @@ int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstr
   KHX_TRY(zgemm(s, n, n, n, one, mQ, mW, zero, mU, batch));
   mode_vectors_kernel<<<grid_for(nb), 256, 0, s>>>(lam, k0d, n, batch, xc, ilc, mu);
   KHX_LAUNCH_CHECK();
-  const long long sc = 7 * nn;
+  const long long sc = 8 * nn;
   if (cache) {
     KHX_TRY(zcopy(s, mt1, Mat{const_cast<cd*>(cache), n, sc}, n, n, batch));
     KHX_TRY(zcopy(s, mt2, Mat{const_cast<cd*>(cache) + nn, n, sc}, n, n, batch));
@@ -874,7 +881,7 @@ int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstr
   KHX_TRY(zgemm(s, n, n, n, mone, mM, mB, one, mT, batch));
   KHX_TRY(zcopy(s, mH, mA, n, n, batch));
   KHX_TRY(zgemm(s, n, n, n, mone, mB, mG2, one, mH, batch));
-  KHX_TRY(conj_transpose(s, mT, n, n, mTH, batch));
+  if (!cache) KHX_TRY(conj_transpose(s, mT, n, n, mTH, batch));
   if (cache) {
     KHX_TRY(zcopy(s, mR2, Mat{const_cast<cd*>(cache) + 4 * nn, 2 * n, sc}, n, 2 * n, batch));
   } else {
@@ -889,7 +896,14 @@ int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstr
   // (1) [K1 | K2] = T^-H [S11bar + S22bar | S12bar + S21bar]
   KHX_TRY(zaxpby(s, mK1, one, sbar(0), one, sbar(3), n, n, batch));
   KHX_TRY(zaxpby(s, mK2, one, sbar(1), one, sbar(2), n, n, batch));
-  KHX_TRY(zgesv(s, n, 2 * n, mTH, mRB, batch, infos + 4 * batch));
+  if (cache) {  // with the cached T^-1 one GEMM, through GJ (first written in step 6)
+    KHX_TRY(zgemm_op(s, 2, 0, n, 2 * n, n, one, Mat{const_cast<cd*>(cache) + 7 * nn, n, sc}, mRB, zero, mGJ, batch));
+    KHX_TRY(zcopy(s, mRB, mGJ, n, 2 * n, batch));
+    zero_status_kernel<<<dim3((batch + 255) / 256), 256, 0, s>>>(infos + 4 * batch, batch);
+    KHX_LAUNCH_CHECK();
+  } else {
+    KHX_TRY(zgesv(s, n, 2 * n, mTH, mRB, batch, infos + 4 * batch));
+  }
   // (2) Tbar = -[K1 | K2] [S11 | S12]^H
   KHX_TRY(zgemm_op(s, 0, 2, n, n, 2 * n, mone, mRB, mR2, zero, mTb, batch));
   // (3) right block X H: xbar = rowdot(K2, H), Hbar = conj(X) K2; H = A - B G2

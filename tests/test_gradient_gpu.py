@@ -129,10 +129,11 @@ def test_patterned_layer_bwd_matches_autograd(gpu, monkeypatch, pw, theta, ainv)
     got = ops.convmat_adjoint(Cbar, pw, shape).cpu().numpy()[0]
     np.testing.assert_allclose(got, ref, atol=1e-9 * np.abs(ref).max())
     # the same with the forward pass's intermediates kept (khx_patterned_layer_cached /
-    # khx_patterned_layer_bwd_cached): S as before, the gradient to the same tolerance
-    cache = torch.empty((1, 7, n, n), dtype=ops.CDT, device=gpu)
+    # khx_patterned_layer_bwd_cached; [S11 | S12] then goes through T^-1): S and the
+    # gradient to the same tolerances
+    cache = torch.empty((1, 8, n, n), dtype=ops.CDT, device=gpu)
     Sc, (Wc, Vc, lamc), info = ops.patterned_layer(C, IC[0], kxd, kyd, k0d, modes=True, cache=cache)
-    np.testing.assert_array_equal(Sc.cpu().numpy(), Sg.cpu().numpy())
+    np.testing.assert_allclose(Sc[0].cpu().numpy(), Sref, atol=1e-10)
     Cbarc, info = ops.patterned_layer_bwd(C, IC[0], kxd, kyd, k0d, Wc, Vc, lamc, dev(Sbar, gpu), cache=cache)
     assert (info.cpu().numpy() == 0).all()
     gotc = ops.convmat_adjoint(Cbarc, pw, shape).cpu().numpy()[0]
