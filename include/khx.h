@@ -174,8 +174,8 @@ int khx_patterned_layer(const double* C, const double* IC, int per_item, const d
                         int N, int batch, const double* k0d, double* S, double* W, double* V, double* lam,
                         int* info, void* ws, void* stream);
 /* The same, also keeping the four solves' results for the backward twin
- * (khx_patterned_layer_bwd_cached): cache (batch, 8, n, n) = W^-1, V^-1 V0,
- * [G1 | G2] = A^-1 [X | B] (n x 2n), [S11 | S12] (n x 2n), A^-1, T^-1.  Same reference
+ * (khx_patterned_layer_bwd_cached): cache (batch, 9, n, n) = W^-1, V^-1 V0,
+ * [G1 | G2] = A^-1 [X | B] (n x 2n), [S11 | S12] (n x 2n), A^-1, T^-1, V^-1.  Same reference
  * code as khx_patterned_layer; for gradients, where the forward pass's
  * intermediates are needed again.                                          */
 int khx_patterned_layer_cached(const double* C, const double* IC, int per_item, const double* kx, const double* ky,

@@ -161,7 +161,7 @@ def _layer_forward(layer, kx, ky, k0):
     if f in (Formulation.FFT, Formulation.ANALYTICAL):
         C, IC = layer.convolution_matrices(kx.device)
         B, N = kx.shape
-        cache = torch.empty((B, 8, 2 * N, 2 * N), dtype=ops.CDT, device=kx.device)
+        cache = torch.empty((B, 9, 2 * N, 2 * N), dtype=ops.CDT, device=kx.device)
         S, (W, V, lam), info = ops.patterned_layer(C, IC, kx, ky, k0d, modes=True, cache=cache)
         ops.raise_on_info(info, "patterned layer")
         return ("dense", S, dict(C=C, IC=IC, W=W, V=V, lam=lam, k0d=k0d, cache=cache))

@@ -530,7 +530,14 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
     KHX_TRY(zcopy(s, Mat{Wc, n, nn}, Mat{V, n, nn}, n, n, batch));
     fill_v0_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(kx, ky, N, batch, t2, nn, n);
     KHX_LAUNCH_CHECK();
-    KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{t2, n, nn}, batch, info_lu2));
+    if (cache) {  // V^-1, kept for the backward pass; t2 = V^-1 V0 as a GEMM (Am holds V0 meanwhile)
+      KHX_TRY(zset_identity(s, Mat{cache + 8 * nn, n, 9 * nn}, n, n, one, batch));
+      KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{cache + 8 * nn, n, 9 * nn}, batch, info_lu2));
+      KHX_TRY(zcopy(s, Mat{Am, n, nn}, Mat{t2, n, nn}, n, n, batch));
+      KHX_TRY(zgemm(s, n, n, n, one, Mat{cache + 8 * nn, n, 9 * nn}, Mat{Am, n, nn}, zero, Mat{t2, n, nn}, batch));
+    } else {
+      KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{t2, n, nn}, batch, info_lu2));
+    }
   } else {
   // t1 = W^-1 W0 (W0 = I) and t2 = V^-1 V0 as one batch of 2 batch systems
   // (status words: info_lu1 then info_lu2, which follow it)
@@ -544,6 +551,11 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   } else {  // the multi-launch LU takes at most 65535 systems per call
     KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{t1, n, nn}, batch, info_lu1));
     KHX_TRY(zgesv(s, n, n, Mat{Wc + (long long)batch * nn, n, nn}, Mat{t2, n, nn}, batch, info_lu2));
+  }
+  if (cache) {  // V^-1 for the backward pass
+    KHX_TRY(zcopy(s, Mat{Wc, n, nn}, Mat{V, n, nn}, n, n, batch));
+    KHX_TRY(zset_identity(s, Mat{cache + 8 * nn, n, 9 * nn}, n, n, one, batch));
+    KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{cache + 8 * nn, n, 9 * nn}, batch, info_lu2));
   }
   }
   // A = t1 + t2, B = t1 - t2
@@ -561,15 +573,15 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
     // behind KHX_LAYER_AINV=0
     KHX_TRY(zset_identity(s, Mat{R1, 2 * n, 2 * nn}, n, n, one, batch));
     KHX_TRY(zgesv(s, n, n, Mat{Ac, n, nn}, Mat{R1, 2 * n, 2 * nn}, batch, info_lu3));
-    if (cache) KHX_TRY(zcopy(s, Mat{cache + 6 * nn, n, 8 * nn}, Mat{R1, 2 * n, 2 * nn}, n, n, batch));
+    if (cache) KHX_TRY(zcopy(s, Mat{cache + 6 * nn, n, 9 * nn}, Mat{R1, 2 * n, 2 * nn}, n, n, batch));
     KHX_TRY(zgemm(s, n, n, n, one, Mat{R1, 2 * n, 2 * nn}, Mat{Bm, n, nn}, zero, Mat{R1 + n, 2 * n, 2 * nn}, batch));
     KHX_TRY(zdiag_scale(s, Mat{R1, 2 * n, 2 * nn}, nullptr, xdiag, n, n, batch, Mat{R1, 2 * n, 2 * nn}));
   } else {
     KHX_TRY(zgesv(s, n, 2 * n, Mat{Ac, n, nn}, Mat{R1, 2 * n, 2 * nn}, batch, info_lu3));
     if (cache) {  // A^-1 for the backward pass
       KHX_TRY(zcopy(s, Mat{Ac, n, nn}, Mat{Am, n, nn}, n, n, batch));
-      KHX_TRY(zset_identity(s, Mat{cache + 6 * nn, n, 8 * nn}, n, n, one, batch));
-      KHX_TRY(zgesv(s, n, n, Mat{Ac, n, nn}, Mat{cache + 6 * nn, n, 8 * nn}, batch, info_lu3));
+      KHX_TRY(zset_identity(s, Mat{cache + 6 * nn, n, 9 * nn}, n, n, one, batch));
+      KHX_TRY(zgesv(s, n, n, Mat{Ac, n, nn}, Mat{cache + 6 * nn, n, 9 * nn}, batch, info_lu3));
     }
   }
   // M = X B G1
@@ -587,7 +599,7 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
   KHX_TRY(zrow_scale(s, Mat{R2 + n, 2 * n, 2 * nn}, xdiag, n, Mat{Ac, n, nn}, n, n, batch));
   // [S11 | S12] = T^-1 R2; with a cache (gradients) through T^-1, which is kept
   // for the backward pass: one solve with n right-hand sides and a GEMM
-  const long long sc = 8 * nn;
+  const long long sc = 9 * nn;
   if (cache) {
     KHX_TRY(zcopy(s, Mat{cache, n, sc}, Mat{t1, n, nn}, n, n, batch));
     KHX_TRY(zcopy(s, Mat{cache + nn, n, sc}, Mat{t2, n, nn}, n, n, batch));
@@ -848,7 +860,7 @@ int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstr
   KHX_TRY(zgemm(s, n, n, n, one, mQ, mW, zero, mU, batch));
   mode_vectors_kernel<<<grid_for(nb), 256, 0, s>>>(lam, k0d, n, batch, xc, ilc, mu);
   KHX_LAUNCH_CHECK();
-  const long long sc = 8 * nn;
+  const long long sc = 9 * nn;
   if (cache) {
     KHX_TRY(zcopy(s, mt1, Mat{const_cast<cd*>(cache), n, sc}, n, n, batch));
     KHX_TRY(zcopy(s, mt2, Mat{const_cast<cd*>(cache) + nn, n, sc}, n, n, batch));
@@ -942,9 +954,16 @@ int patterned_layer_bwd(hipStream_t s, const cd* C, const cd* IC, long long cstr
   // (8) A = t1 + t2, B = t1 - t2
   KHX_TRY(zaxpby(s, mt1b, one, mAb, one, mBb, n, n, batch));
   KHX_TRY(zaxpby(s, mt2b, one, mAb, mone, mBb, n, n, batch));
-  // (9) t2 = V^-1 V0: Vbar = -(V^-H t2bar) t2^H
-  KHX_TRY(conj_transpose(s, mV, n, n, mWc, batch));
-  KHX_TRY(zgesv(s, n, n, mWc, mt2b, batch, infos + 6 * batch));
+  // (9) t2 = V^-1 V0: Vbar = -(V^-H t2bar) t2^H (with the cached V^-1 one GEMM, through tmp)
+  if (cache) {
+    KHX_TRY(zgemm_op(s, 2, 0, n, n, n, one, Mat{const_cast<cd*>(cache) + 8 * nn, n, sc}, mt2b, zero, mtmp, batch));
+    KHX_TRY(zcopy(s, mt2b, mtmp, n, n, batch));
+    zero_status_kernel<<<dim3((batch + 255) / 256), 256, 0, s>>>(infos + 6 * batch, batch);
+    KHX_LAUNCH_CHECK();
+  } else {
+    KHX_TRY(conj_transpose(s, mV, n, n, mWc, batch));
+    KHX_TRY(zgesv(s, n, n, mWc, mt2b, batch, infos + 6 * batch));
+  }
   KHX_TRY(zgemm_op(s, 0, 2, n, n, n, mone, mt2b, mt2, zero, mVb, batch));
   // (10) t1 = W^-1: Wbar = -t1^H t1bar t1^H
   KHX_TRY(zgemm_op(s, 0, 2, n, n, n, one, mt1b, mt1, zero, mtmp, batch));

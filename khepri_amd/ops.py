@@ -313,7 +313,7 @@ def eig(A, overwrite_a=False):
 def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None, cache=None):
     """S (B, 2, 2, n, n) of a patterned layer; (W, V, lam) too when modes.
     C, IC: (N, N) shared by the batch or (B, N, N) one per item.  cache: a
-    (B, 8, n, n) complex tensor that receives the solves' results for
+    (B, 9, n, n) complex tensor that receives the solves' results for
     patterned_layer_bwd (khx_patterned_layer_cached)."""
     lib = _lib.load()
     B, N = kx.shape
@@ -335,8 +335,8 @@ def patterned_layer(C, IC, kx, ky, k0d, modes=False, ws=None, cache=None):
         raise ValueError("per-item convolution matrices must match the batch")
     if cache is not None:
         _require(cache, CDT, "cache")
-        if tuple(cache.shape) != (B, 8, n, n) or not cache.is_contiguous():
-            raise ValueError(f"cache must be a contiguous ({B}, 8, {n}, {n}) complex tensor")
+        if tuple(cache.shape) != (B, 9, n, n) or not cache.is_contiguous():
+            raise ValueError(f"cache must be a contiguous ({B}, 9, {n}, {n}) complex tensor")
         check(lib.khx_patterned_layer_cached(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(S), _p(W),
                                              _p(V), _p(lam), _p(info), _p(cache), _p(ws), _stream()),
               "khx_patterned_layer_cached")
@@ -384,7 +384,7 @@ def patterned_layers_fused(Cs, ICs, kx, ky, k0ds, modes=False, cache=False):
     stage runs once over L*B items (a small batch of one layer leaves most CUs
     idle).  Returns one (S, (W, V, lam), info) per layer, as views; with
     cache, one (S, (W, V, lam), info, cache) per layer (cache: the
-    patterned_layer_bwd intermediates, (B, 8, n, n))."""
+    patterned_layer_bwd intermediates, (B, 9, n, n))."""
     L = len(Cs)
     B, N = kx.shape
     n = 2 * N
@@ -393,7 +393,7 @@ def patterned_layers_fused(Cs, ICs, kx, ky, k0ds, modes=False, cache=False):
     kxl = kx.repeat(L, 1)
     kyl = ky.repeat(L, 1)
     k0dl = torch.cat([k.reshape(B) for k in k0ds])
-    cb = torch.empty((L * B, 8, n, n), dtype=CDT, device=kx.device) if cache else None
+    cb = torch.empty((L * B, 9, n, n), dtype=CDT, device=kx.device) if cache else None
     S, (W, V, lam), info = patterned_layer(C, IC, kxl, kyl, k0dl, modes=modes, cache=cb)
     out = []
     for l in range(L):
@@ -461,8 +461,8 @@ def patterned_layer_bwd(C, IC, kx, ky, k0d, W, V, lam, Sbar, eta_rel=1e-12, cach
     ws = _ws(lib.khx_patterned_layer_bwd_ws_bytes(n, B), dev)
     if cache is not None:
         cache = cache.contiguous()
-        if tuple(cache.shape) != (B, 8, n, n):
-            raise ValueError(f"cache must be ({B}, 8, {n}, {n})")
+        if tuple(cache.shape) != (B, 9, n, n):
+            raise ValueError(f"cache must be ({B}, 9, {n}, {n})")
         check(lib.khx_patterned_layer_bwd_cached(_p(C), _p(IC), per_item, _p(kx), _p(ky), N, B, _p(k0d), _p(W),
                                                  _p(V), _p(lam), _p(cache), _p(Sbar), float(eta_rel), _p(Cbar),
                                                  _p(info), _p(ws), _stream()),

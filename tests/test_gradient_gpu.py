@@ -131,7 +131,7 @@ def test_patterned_layer_bwd_matches_autograd(gpu, monkeypatch, pw, theta, ainv)
     # the same with the forward pass's intermediates kept (khx_patterned_layer_cached /
     # khx_patterned_layer_bwd_cached; [S11 | S12] then goes through T^-1): S and the
     # gradient to the same tolerances
-    cache = torch.empty((1, 8, n, n), dtype=ops.CDT, device=gpu)
+    cache = torch.empty((1, 9, n, n), dtype=ops.CDT, device=gpu)
     Sc, (Wc, Vc, lamc), info = ops.patterned_layer(C, IC[0], kxd, kyd, k0d, modes=True, cache=cache)
     np.testing.assert_allclose(Sc[0].cpu().numpy(), Sref, atol=1e-10)
     Cbarc, info = ops.patterned_layer_bwd(C, IC[0], kxd, kyd, k0d, Wc, Vc, lamc, dev(Sbar, gpu), cache=cache)
