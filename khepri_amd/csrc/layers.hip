@@ -362,6 +362,18 @@ __global__ void col_div_kernel(cd* D, const cd* __restrict__ S, const cd* __rest
   }
 }
 
+// D[b][i][j] = S[b][i][j] / lam[b][i]
+__global__ void row_div_kernel(cd* D, const cd* __restrict__ S, const cd* __restrict__ lam, int n, int batch) {
+  const long long nn = (long long)n * n;
+  const long long total = (long long)batch * nn;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long b = e / nn;
+    const int i = (int)((e % nn) / n);
+    D[e] = divc(S[e], lam[b * n + i]);
+  }
+}
+
 // X = exp(-lam * k0 d) on the diagonal of the left half of R (n x 2n, zero
 // elsewhere) and B in the right half.
 __global__ void x_rhs_kernel(const cd* __restrict__ lam, const double* __restrict__ k0d, int n, int batch,
@@ -530,7 +542,16 @@ int patterned_layer(hipStream_t s, const cd* C, const cd* IC, long long cstride,
     KHX_TRY(zcopy(s, Mat{Wc, n, nn}, Mat{V, n, nn}, n, n, batch));
     fill_v0_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(kx, ky, N, batch, t2, nn, n);
     KHX_LAUNCH_CHECK();
-    if (cache) {  // V^-1, kept for the backward pass; t2 = V^-1 V0 as a GEMM (Am holds V0 meanwhile)
+    const char* vi_env = getenv("KHX_LAYER_VIDENT");
+    if (!cache && vi_env && atoi(vi_env) != 0) {
+      // t2 = V^-1 V0 = diag(1/lam) W^-1 (P V0) (P V = W diag(lam)): two GEMMs, no factorization of V
+      KHX_TRY(zgemm(s, n, n, n, one, Mat{P, n, nn}, Mat{t2, n, nn}, zero, Mat{Am, n, nn}, batch));
+      KHX_TRY(zgemm(s, n, n, n, one, Mat{t1, n, nn}, Mat{Am, n, nn}, zero, Mat{t2, n, nn}, batch));
+      row_div_kernel<<<grid_for((long long)batch * nn), 256, 0, s>>>(t2, t2, lam, n, batch);
+      KHX_LAUNCH_CHECK();
+      zero_status_kernel<<<dim3((batch + 255) / 256), 256, 0, s>>>(info_lu2, batch);
+      KHX_LAUNCH_CHECK();
+    } else if (cache) {  // V^-1, kept for the backward pass; t2 = V^-1 V0 as a GEMM (Am holds V0 meanwhile)
       KHX_TRY(zset_identity(s, Mat{cache + 8 * nn, n, 9 * nn}, n, n, one, batch));
       KHX_TRY(zgesv(s, n, n, Mat{Wc, n, nn}, Mat{cache + 8 * nn, n, 9 * nn}, batch, info_lu2));
       KHX_TRY(zcopy(s, Mat{Am, n, nn}, Mat{t2, n, nn}, n, n, batch));

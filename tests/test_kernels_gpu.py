@@ -114,15 +114,16 @@ def _layer_inputs(pw, kp, wl):
     return g, kx, ky
 
 
-@pytest.mark.parametrize("env", [{}, {"KHX_LAYER_AINV": "0"}, {"KHX_LAYER_WINV": "0"}],
-                         ids=["default", "AINV=0", "WINV=0"])
+@pytest.mark.parametrize("env", [{}, {"KHX_LAYER_AINV": "0"}, {"KHX_LAYER_WINV": "0"}, {"KHX_LAYER_VIDENT": "1"}],
+                         ids=["default", "AINV=0", "WINV=0", "VIDENT=1"])
 @pytest.mark.parametrize("pw,kp,wl", [((5, 5), (0.0, 0.0), 2.0), ((5, 5), (0.4, -0.3), 1.7),
                                       ((9, 9), (0.2, 0.1), 1.9)])
 def test_patterned_layer(gpu, monkeypatch, pw, kp, wl, env):
     """The layer S-matrix against the oracle's build_scatmat: by default W^-1
     from the eigensolver's Schur vectors and triangular eigenvectors and
     [G1 | G2] through A^-1; also with the reference's solves for either
-    (KHX_LAYER_WINV=0, KHX_LAYER_AINV=0)."""
+    (KHX_LAYER_WINV=0, KHX_LAYER_AINV=0) and with V^-1 V0 through the
+    identity V^-1 = diag(1/lam) W^-1 P (KHX_LAYER_VIDENT=1)."""
     from khepri_amd import ops
     for k, v in env.items():
         monkeypatch.setenv(k, v)
